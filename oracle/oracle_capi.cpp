@@ -1,0 +1,358 @@
+// oracle_capi.cpp -- extern "C" face of oracle.hpp for the Python test suite
+// and the cpu_baseline leg of bench.py.  TEST INFRASTRUCTURE ONLY: the product
+// library never links this file.
+//
+// Conventions: matrices are row-major, sizes are uint64, every function returns
+// 0 on success and a negative value after an exception (message available from
+// orc_last_error()).
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "oracle.hpp"
+
+namespace
+{
+thread_local std::string g_err;
+
+oracle::udp make_udp(int kind, uint64_t dim, const double *centers, uint64_t nobj)
+{
+    oracle::udp p;
+    p.kind = kind;
+    if (kind == oracle::LENNARD_JONES) {
+        p.atoms = static_cast<unsigned>(dim);
+    } else {
+        p.dim = static_cast<unsigned>(dim);
+    }
+    if (kind == oracle::MO_SPHERE) {
+        p.nobj = static_cast<unsigned>(nobj);
+        p.centers.assign(centers, centers + nobj * dim);
+    }
+    return p;
+}
+
+template <typename F>
+int guarded(F &&f)
+{
+    try {
+        f();
+        return 0;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+} // namespace
+
+extern "C" {
+
+const char *orc_last_error(void)
+{
+    return g_err.c_str();
+}
+
+// n outputs of std::mt19937(seed) after discarding `skip` outputs.
+int orc_mt19937_u32(uint32_t seed, uint64_t skip, uint64_t n, uint32_t *out)
+{
+    return guarded([&] {
+        std::mt19937 e(seed);
+        e.discard(skip);
+        for (uint64_t i = 0; i < n; ++i) {
+            out[i] = static_cast<uint32_t>(e());
+        }
+    });
+}
+
+// n draws of std::uniform_real_distribution<double>(a, b) on std::mt19937(seed).
+int orc_uniform_real(uint32_t seed, double a, double b, uint64_t n, double *out)
+{
+    return guarded([&] {
+        std::mt19937 e(seed);
+        std::uniform_real_distribution<double> d(a, b);
+        for (uint64_t i = 0; i < n; ++i) {
+            out[i] = d(e);
+        }
+    });
+}
+
+// Problem metadata: nx, nf, bounds (lb/ub may be null).
+int orc_problem_info(int kind, uint64_t dim, uint64_t nobj, uint64_t *nx, uint64_t *nf, double *lb, double *ub)
+{
+    return guarded([&] {
+        std::vector<double> c(nobj * (kind == oracle::MO_SPHERE ? dim : 0), 0.);
+        auto p = make_udp(kind, dim, c.data(), nobj);
+        *nx = p.get_nx();
+        *nf = p.get_nf();
+        const auto b = p.get_bounds();
+        if (lb) std::memcpy(lb, b.first.data(), b.first.size() * sizeof(double));
+        if (ub) std::memcpy(ub, b.second.data(), b.second.size() * sizeof(double));
+    });
+}
+
+// Single fitness call on an arbitrary-length x (the reference's known answers
+// evaluate e.g. a 3-vector on a 1-dimensional ackley).
+int orc_fitness_one(int kind, uint64_t dim, const double *centers, uint64_t nobj, const double *x, uint64_t n,
+                    double *f)
+{
+    return guarded([&] {
+        auto p = make_udp(kind, dim, centers, nobj);
+        const auto r = p.fitness(std::vector<double>(x, x + n));
+        std::memcpy(f, r.data(), r.size() * sizeof(double));
+    });
+}
+
+// bfe::operator() with thread_bfe on n_threads host threads.
+int orc_bfe(int kind, uint64_t dim, const double *centers, uint64_t nobj, const double *dvs, uint64_t n_dvs,
+            uint64_t nx, double *fvs, uint32_t n_threads)
+{
+    return guarded([&] {
+        auto p = make_udp(kind, dim, centers, nobj);
+        std::vector<double> in(dvs, dvs + n_dvs * nx);
+        const auto r = oracle::bfe_call(p, in, n_threads);
+        std::memcpy(fvs, r.data(), r.size() * sizeof(double));
+    });
+}
+
+// Timed bfe: runs `reps` calls and returns the median wall time per call in *sec.
+int orc_bfe_timed(int kind, uint64_t dim, const double *centers, uint64_t nobj, const double *dvs, uint64_t n_dvs,
+                  uint64_t nx, double *fvs, uint32_t n_threads, uint32_t reps, double *sec)
+{
+    return guarded([&] {
+        auto p = make_udp(kind, dim, centers, nobj);
+        std::vector<double> in(dvs, dvs + n_dvs * nx);
+        std::vector<double> t;
+        std::vector<double> r;
+        for (uint32_t k = 0; k < reps; ++k) {
+            const auto t0 = std::chrono::steady_clock::now();
+            r = oracle::bfe_call(p, in, n_threads);
+            const auto t1 = std::chrono::steady_clock::now();
+            t.push_back(std::chrono::duration<double>(t1 - t0).count());
+        }
+        std::sort(t.begin(), t.end());
+        *sec = t[t.size() / 2];
+        std::memcpy(fvs, r.data(), r.size() * sizeof(double));
+    });
+}
+
+// Analytic gradient of every dv (n_dvs x nx).
+int orc_gradient(int kind, uint64_t dim, const double *dvs, uint64_t n_dvs, uint64_t nx, double *grad)
+{
+    return guarded([&] {
+        auto p = make_udp(kind, dim, nullptr, 1);
+        for (uint64_t i = 0; i < n_dvs; ++i) {
+            const auto g = p.gradient(std::vector<double>(dvs + i * nx, dvs + (i + 1) * nx));
+            std::memcpy(grad + i * nx, g.data(), nx * sizeof(double));
+        }
+    });
+}
+
+// estimate_gradient (high = 0) or estimate_gradient_h (high = 1) of the
+// problem's fitness at x; out has nf * nx entries (layout grad[j + i*nx]).
+int orc_estimate_gradient(int kind, uint64_t dim, const double *centers, uint64_t nobj, const double *x, uint64_t nx,
+                          double dx, int high, double *out)
+{
+    return guarded([&] {
+        auto p = make_udp(kind, dim, centers, nobj);
+        auto f = [&p](const std::vector<double> &v) { return p.fitness(v); };
+        const std::vector<double> xv(x, x + nx);
+        const auto g = high ? oracle::estimate_gradient_h(f, xv, dx) : oracle::estimate_gradient(f, xv, dx);
+        std::memcpy(out, g.data(), g.size() * sizeof(double));
+    });
+}
+
+// The known-answer problem of tests/gradients_and_hessians.cpp:120-142
+// (dummy_problem_easy_grad), for pinning estimate_gradient(_h).
+int orc_estimate_gradient_kat(const double *x, double dx, int high, double *out)
+{
+    return guarded([&] {
+        auto f = [](const std::vector<double> &dv) {
+            std::vector<double> r(3, 0.);
+            r[0] = dv[0] + dv[1] * dv[1] + dv[2] * dv[2] + dv[3] * dv[3] * dv[3];
+            r[1] = dv[1] * dv[2] * dv[3] * dv[0];
+            r[2] = std::sin(dv[0]);
+            return r;
+        };
+        const std::vector<double> xv(x, x + 4);
+        const auto g = high ? oracle::estimate_gradient_h(f, xv, dx) : oracle::estimate_gradient(f, xv, dx);
+        std::memcpy(out, g.data(), g.size() * sizeof(double));
+    });
+}
+
+int orc_pareto_dominance(const double *a, const double *b, uint64_t m, int *out)
+{
+    return guarded([&] { *out = oracle::pareto_dominance(a, b, m) ? 1 : 0; });
+}
+
+// fast_non_dominated_sorting: handle-based because output sizes are data
+// dependent.  orc_fnds_run -> orc_fnds_sizes -> orc_fnds_copy -> orc_fnds_free.
+void *orc_fnds_run(const double *objs, uint64_t n, uint64_t m)
+{
+    try {
+        return new oracle::fnds_result(oracle::fast_non_dominated_sorting(objs, n, m));
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void orc_fnds_sizes(void *h, uint64_t *n_fronts, uint64_t *n_dom_items)
+{
+    auto r = static_cast<oracle::fnds_result *>(h);
+    *n_fronts = r->fronts.size();
+    uint64_t t = 0;
+    for (const auto &d : r->dom_list) t += d.size();
+    *n_dom_items = t;
+}
+
+// front_offsets: n_fronts+1, front_items: n; dom_offsets: n+1, dom_items: total.
+void orc_fnds_copy(void *h, uint64_t *dom_count, uint64_t *rank, uint64_t *front_offsets, uint64_t *front_items,
+                   uint64_t *dom_offsets, uint64_t *dom_items)
+{
+    auto r = static_cast<oracle::fnds_result *>(h);
+    const auto n = r->dom_count.size();
+    for (std::size_t i = 0; i < n; ++i) {
+        dom_count[i] = r->dom_count[i];
+        rank[i] = r->rank[i];
+    }
+    uint64_t o = 0;
+    front_offsets[0] = 0;
+    for (std::size_t f = 0; f < r->fronts.size(); ++f) {
+        for (auto v : r->fronts[f]) front_items[o++] = v;
+        front_offsets[f + 1] = o;
+    }
+    o = 0;
+    dom_offsets[0] = 0;
+    for (std::size_t i = 0; i < n; ++i) {
+        for (auto v : r->dom_list[i]) dom_items[o++] = v;
+        dom_offsets[i + 1] = o;
+    }
+}
+
+void orc_fnds_free(void *h)
+{
+    delete static_cast<oracle::fnds_result *>(h);
+}
+
+// Timed fnds (single-threaded, as in the reference): median seconds per call.
+int orc_fnds_timed(const double *objs, uint64_t n, uint64_t m, uint32_t reps, double *sec)
+{
+    return guarded([&] {
+        std::vector<double> t;
+        for (uint32_t k = 0; k < reps; ++k) {
+            const auto t0 = std::chrono::steady_clock::now();
+            auto r = oracle::fast_non_dominated_sorting(objs, n, m);
+            const auto t1 = std::chrono::steady_clock::now();
+            t.push_back(std::chrono::duration<double>(t1 - t0).count());
+        }
+        std::sort(t.begin(), t.end());
+        *sec = t[t.size() / 2];
+    });
+}
+
+// population(problem, pop_size, seed): X (pop x nx) and F (pop x nf).
+int orc_population_init(int kind, uint64_t dim, const double *centers, uint64_t nobj, uint64_t pop_size,
+                        uint32_t seed, double *X, double *F)
+{
+    return guarded([&] {
+        auto p = make_udp(kind, dim, centers, nobj);
+        std::vector<double> x, f;
+        oracle::population_init(p, pop_size, seed, x, f);
+        std::memcpy(X, x.data(), x.size() * sizeof(double));
+        std::memcpy(F, f.data(), f.size() * sizeof(double));
+    });
+}
+
+// batch_random_decision_vector(prob, n, mt19937(seed)).
+int orc_batch_random_dv(int kind, uint64_t dim, uint64_t n, uint32_t seed, double *out)
+{
+    return guarded([&] {
+        auto p = make_udp(kind, dim, nullptr, 1);
+        std::mt19937 e(seed);
+        const auto r = oracle::batch_random_decision_vector(p, n, e);
+        std::memcpy(out, r.data(), r.size() * sizeof(double));
+    });
+}
+
+// Uniform draws over explicit bounds in batch_random_decision_vector order.
+int orc_batch_uniform(const double *lb, const double *ub, uint64_t nx, uint64_t n, uint32_t seed, double *out)
+{
+    return guarded([&] {
+        std::mt19937 e(seed);
+        std::uniform_real_distribution<double> rdist;
+        for (uint64_t i = 0; i < n; ++i) {
+            for (uint64_t j = 0; j < nx; ++j) {
+                out[i * nx + j] = (lb[j] == ub[j]) ? lb[j]
+                                                   : rdist(e, std::uniform_real_distribution<double>::param_type(
+                                                                  lb[j], ub[j]));
+            }
+        }
+    });
+}
+
+// pso_gen(gen, omega, eta1, eta2, max_vel, variant, neighb_type, neighb_param,
+// memory, seed).evolve(pop) with X/F in and out.  use_bfe selects the bfe
+// branch (thread_bfe on n_threads) versus per-individual fitness calls.
+// trace_X (gen x pop x nx) and trace_F (gen x pop) may be null.
+int orc_pso_gen_evolve(int kind, uint64_t dim, uint64_t pop, double *X, double *F, uint32_t gen, double omega,
+                       double eta1, double eta2, double max_vel, uint32_t variant, uint32_t neighb_type,
+                       uint32_t neighb_param, int memory, uint32_t seed, int use_bfe, uint32_t n_threads,
+                       double *trace_X, double *trace_F)
+{
+    return guarded([&] {
+        auto p = make_udp(kind, dim, nullptr, 1);
+        const auto nx = p.get_nx();
+        oracle::pso_params prm;
+        prm.gen = gen;
+        prm.omega = omega;
+        prm.eta1 = eta1;
+        prm.eta2 = eta2;
+        prm.max_vel = max_vel;
+        prm.variant = variant;
+        prm.neighb_type = neighb_type;
+        prm.neighb_param = neighb_param;
+        prm.memory = memory != 0;
+        prm.seed = seed;
+        oracle::pso_gen algo(prm);
+        std::vector<double> x(X, X + pop * nx), f(F, F + pop);
+        oracle::batch_eval_fn bfe = [n_threads](const oracle::udp &q, const std::vector<double> &d) {
+            return oracle::bfe_call(q, d, n_threads);
+        };
+        std::vector<double> tx, tf;
+        algo.evolve(p, x, f, use_bfe ? &bfe : nullptr, trace_X ? &tx : nullptr, trace_F ? &tf : nullptr);
+        std::memcpy(X, x.data(), x.size() * sizeof(double));
+        std::memcpy(F, f.data(), f.size() * sizeof(double));
+        if (trace_X) std::memcpy(trace_X, tx.data(), tx.size() * sizeof(double));
+        if (trace_F) std::memcpy(trace_F, tf.data(), tf.size() * sizeof(double));
+    });
+}
+
+// Timed pso_gen evolve (median seconds over reps, each from the same input).
+int orc_pso_gen_timed(int kind, uint64_t dim, uint64_t pop, const double *X, const double *F, uint32_t gen,
+                      uint32_t seed, uint32_t n_threads, uint32_t reps, double *sec)
+{
+    return guarded([&] {
+        auto p = make_udp(kind, dim, nullptr, 1);
+        const auto nx = p.get_nx();
+        std::vector<double> t;
+        for (uint32_t k = 0; k < reps; ++k) {
+            oracle::pso_params prm;
+            prm.gen = gen;
+            prm.seed = seed;
+            oracle::pso_gen algo(prm);
+            std::vector<double> x(X, X + pop * nx), f(F, F + pop);
+            oracle::batch_eval_fn bfe = [n_threads](const oracle::udp &q, const std::vector<double> &d) {
+                return oracle::bfe_call(q, d, n_threads);
+            };
+            const auto t0 = std::chrono::steady_clock::now();
+            algo.evolve(p, x, f, &bfe);
+            const auto t1 = std::chrono::steady_clock::now();
+            t.push_back(std::chrono::duration<double>(t1 - t0).count());
+        }
+        std::sort(t.begin(), t.end());
+        *sec = t[t.size() / 2];
+    });
+}
+
+} // extern "C"

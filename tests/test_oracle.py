@@ -1,0 +1,354 @@
+"""Pins the CPU oracle before anything is compared against it.
+
+Each check is chosen so that a plausible restatement mistake (a dropped term,
+a swapped index, a wrong loop bound, a different draw order) fails it: the
+reference's own known answers, closed forms at best_known(), invariants and
+brute force on small inputs.  CPU only.
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+KIND = {"rosenbrock": O.ROSENBROCK, "rastrigin": O.RASTRIGIN, "ackley": O.ACKLEY, "schwefel": O.SCHWEFEL,
+        "griewank": O.GRIEWANK, "lennard_jones": O.LENNARD_JONES}
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+# --- reference known answers -------------------------------------------------
+
+def test_fitness_kats(kats):
+    for k in kats["fitness"]:
+        f = O.fitness_one(KIND[k["kind"]], k["dim"], k["x"])
+        if k.get("exact"):
+            assert f[0] == k["f"][0], k["src"]
+        else:
+            assert rel(f[0], k["f"][0]) <= k["rel_tol"], (k["src"], f[0])
+
+
+def test_gradient_kats(kats):
+    for k in kats["gradient"]:
+        g = O.gradient(KIND[k["kind"]], k["dim"], np.array([k["x"]]))[0]
+        for gi, ei in zip(g, k["g"]):
+            if "abs_tol" in k:
+                assert abs(gi - ei) < k["abs_tol"], k["src"]
+            else:
+                assert rel(gi, ei) <= k["rel_tol"], k["src"]
+
+
+def test_estimate_gradient_h_kat(kats):
+    k = kats["estimate_gradient_h"]
+    gh = O.estimate_gradient_kat(k["x"], k["dx"], True)
+    for tol in k["rel_tol"]:
+        for a, b in zip(gh, k["g"]):
+            if b == 0:
+                assert a == 0
+            else:
+                assert rel(a, b) <= tol
+    # The low-order estimate of the same test (computed but unchecked there)
+    # must at least agree to the finite-difference accuracy.
+    g = O.estimate_gradient_kat(k["x"], 1e-8, False)
+    np.testing.assert_allclose(g, k["g"], rtol=1e-6, atol=1e-9)
+
+
+def test_pareto_dominance_kats(kats):
+    for k in kats["pareto_dominance"]:
+        assert O.pareto_dominance(k["a"], k["b"]) == k["r"], k["src"]
+    with pytest.raises(ValueError):
+        O.pareto_dominance([1, 2], [3, 4, 5])
+
+
+def _fnds_lists(r):
+    fo, fi = r["front_offsets"], r["front_items"]
+    fronts = [list(map(int, fi[fo[i]:fo[i + 1]])) for i in range(len(fo) - 1)]
+    do, di = r["dom_offsets"], r["dom_items"]
+    dl = [list(map(int, di[do[i]:do[i + 1]])) for i in range(len(do) - 1)]
+    return fronts, dl
+
+
+def test_fnds_kats(kats):
+    for k in kats["fnds"]:
+        pts = np.array(k["points"], np.float64).reshape(len(k["points"]), -1)
+        r = O.fnds(pts)
+        fronts, dl = _fnds_lists(r)
+        assert fronts == k["fronts"], k["src"]
+        assert dl == k["dom_list"], k["src"]
+        assert list(r["dom_count"]) == k["dom_count"], k["src"]
+        assert list(r["rank"]) == k["rank"], k["src"]
+    for k in kats["fnds_throws"]:
+        pts = np.array(k["points"], np.float64).reshape(len(k["points"]), -1 if k["points"] else 0)
+        with pytest.raises(O.OracleError, match="At least two points"):
+            O.fnds(pts)
+
+
+def test_mt19937_kat(kats):
+    k = kats["mt19937"]
+    assert int(O.mt19937_u32(k["seed"], 1, skip=k["index"] - 1)[0]) == k["value"]
+
+
+# --- closed forms ------------------------------------------------------------
+
+def test_best_known_closed_forms():
+    assert O.fitness_one(O.ROSENBROCK, 10, np.ones(10))[0] == 0.0       # rosenbrock.cpp:98-101
+    assert O.fitness_one(O.RASTRIGIN, 7, np.zeros(7))[0] == 0.0         # rastrigin.cpp:154-157
+    assert abs(O.fitness_one(O.ACKLEY, 7, np.zeros(7))[0]) < 1e-15      # ackley.cpp:96-99
+    assert O.fitness_one(O.GRIEWANK, 7, np.zeros(7))[0] == 0.0          # griewank.cpp:95-98
+    assert abs(O.fitness_one(O.SCHWEFEL, 7, np.full(7, 420.9687))[0]) < 1e-3  # schwefel.cpp:88-91
+
+
+def test_fitness_identities_numpy():
+    """Independent float64 numpy evaluations of the textbook formulas."""
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-5, 5, size=(20, 13))
+    for row in x:
+        ros = O.fitness_one(O.ROSENBROCK, 13, row)[0]
+        ref = np.sum(100 * (row[:-1] ** 2 - row[1:]) ** 2 + (row[:-1] - 1) ** 2)
+        assert rel(ros, ref) < 1e-13
+        ras = O.fitness_one(O.RASTRIGIN, 13, row)[0]
+        assert rel(ras, 10 * 13 + np.sum(row ** 2 - 10 * np.cos(2 * np.pi * row))) < 1e-12
+        ack = O.fitness_one(O.ACKLEY, 13, row)[0]
+        ref = (-20 * np.exp(-0.2 * np.sqrt(np.mean(row ** 2))) - np.exp(np.mean(np.cos(2 * np.pi * row)))
+               + 20 + np.e)
+        assert rel(ack, ref) < 1e-12
+        sch = O.fitness_one(O.SCHWEFEL, 13, row)[0]
+        assert rel(sch, 418.9828872724338 * 13 - np.sum(row * np.sin(np.sqrt(np.abs(row))))) < 1e-12
+        gri = O.fitness_one(O.GRIEWANK, 13, row)[0]
+        ref = np.sum(row ** 2) / 4000 - np.prod(np.cos(row / np.sqrt(np.arange(1, 14)))) + 1
+        assert rel(gri, ref) < 1e-12
+
+
+def _lj_positions(x):
+    atoms = (len(x) + 6) // 3
+    p = np.zeros((atoms, 3))
+    p[1, 2] = x[0]
+    p[2, 1:] = x[1:3]
+    p[3:] = np.asarray(x[3:]).reshape(-1, 3)
+    return p
+
+
+def test_lennard_jones_brute_force():
+    rng = np.random.default_rng(2)
+    for atoms in (3, 4, 7, 13):
+        x = rng.uniform(-3, 3, 3 * atoms - 6)
+        p = _lj_positions(x)
+        e = 0.0
+        for i, j in itertools.combinations(range(atoms), 2):
+            r2 = np.sum((p[i] - p[j]) ** 2)
+            e += r2 ** -6 - r2 ** -3
+        assert rel(O.fitness_one(O.LENNARD_JONES, atoms, x)[0], 4 * e) < 1e-12
+
+
+def test_lennard_jones_coincident_atoms():
+    # lennard_jones.cpp:81-83,89: dist == 0 -> DBL_MAX, final *4 -> +inf.
+    x = np.array([0.0, 0.0, 1.0])  # atom 1 at the origin, on top of atom 0
+    assert O.fitness_one(O.LENNARD_JONES, 3, x)[0] == math.inf
+
+
+def lj_lattice(atoms, rng, spacing=1.1, jitter=0.1):
+    """Decision vector of a jittered cubic lattice (pairs stay ~1 apart), in the
+    encoding of lennard_jones.cpp:132-151 (atom 0 at the origin, atom 1 on z,
+    atom 2 in the yz plane)."""
+    grid = np.array(list(itertools.product(range(4), repeat=3)), np.float64)[:atoms] * spacing
+    p = grid + rng.uniform(-jitter, jitter, grid.shape)
+    p -= p[0]
+    p[1, :2] = 0.0
+    p[2, 0] = 0.0
+    return np.concatenate([[p[1, 2]], p[2, 1:], p[3:].reshape(-1)])
+
+
+def test_lennard_jones_gradient_fd():
+    rng = np.random.default_rng(3)
+    for atoms in (3, 5, 9, 20):
+        x = lj_lattice(atoms, rng)
+        g = O.gradient(O.LENNARD_JONES, atoms, x[None, :])[0]
+        h = 1e-6
+        fd = np.empty_like(x)
+        for k in range(x.size):
+            xp, xm = x.copy(), x.copy()
+            xp[k] += h
+            xm[k] -= h
+            fd[k] = (O.fitness_one(O.LENNARD_JONES, atoms, xp)[0] - O.fitness_one(O.LENNARD_JONES, atoms, xm)[0]) / (2 * h)
+        assert np.linalg.norm(fd - g) / np.linalg.norm(g) < 1e-7
+        gh = O.estimate_gradient(O.LENNARD_JONES, atoms, x, 1e-3, True)
+        assert np.linalg.norm(gh - g) / np.linalg.norm(g) < 1e-7
+
+
+def test_rosenbrock_rastrigin_gradient_vs_estimate():
+    rng = np.random.default_rng(4)
+    for kind in (O.ROSENBROCK, O.RASTRIGIN):
+        x = rng.uniform(-2, 2, 9)
+        g = O.gradient(kind, 9, x[None, :])[0]
+        gh = O.estimate_gradient(kind, 9, x, 1e-2, True)
+        assert np.linalg.norm(gh - g) / np.linalg.norm(g) < 1e-6
+
+
+# --- fnds invariants -------------------------------------------------------
+
+def _brute(points):
+    n = len(points)
+    dom = [[O.pareto_dominance(points[i], points[j]) for j in range(n)] for i in range(n)]
+    count = [sum(dom[j][i] for j in range(n)) for i in range(n)]
+    rank = [None] * n
+    for _ in range(n):  # longest chain by relaxation
+        for i in range(n):
+            rs = [rank[j] for j in range(n) if dom[j][i]]
+            if all(r is not None for r in rs):
+                rank[i] = 1 + max(rs) if rs else 0
+    lists = [[j for j in range(n) if dom[i][j]] for i in range(n)]
+    return lists, count, rank
+
+
+@pytest.mark.parametrize("m,dup", [(2, False), (3, True), (1, True), (4, False)])
+def test_fnds_brute_force(m, dup):
+    rng = np.random.default_rng(10 + m)
+    pts = rng.integers(0, 6, size=(40, m)).astype(np.float64)  # many ties
+    if dup:
+        pts[30:] = pts[:10]
+    r = O.fnds(pts)
+    fronts, dl = _fnds_lists(r)
+    lists, count, rank = _brute(pts)
+    assert dl == lists
+    assert list(map(int, r["dom_count"])) == count
+    assert list(map(int, r["rank"])) == rank
+    assert sorted(itertools.chain(*fronts)) == list(range(len(pts)))
+    for f, items in enumerate(fronts):
+        assert all(rank[i] == f for i in items)
+    assert fronts[0] == sorted(fronts[0])
+
+
+def test_fnds_nan_ordering():
+    # custom_comparisons.hpp:54-100: NaN sorts after +inf, NaN == NaN.
+    nan, inf = math.nan, math.inf
+    assert O.pareto_dominance([inf, 0], [nan, 0])
+    assert not O.pareto_dominance([nan, 0], [inf, 0])
+    assert not O.pareto_dominance([nan, 1], [nan, 1])
+    assert O.pareto_dominance([nan, 0], [nan, 1])
+
+
+# --- bfe restatement ---------------------------------------------------------
+
+@pytest.mark.parametrize("kind,dim", [(O.ROSENBROCK, 7), (O.RASTRIGIN, 9), (O.ACKLEY, 5), (O.SCHWEFEL, 6),
+                                      (O.GRIEWANK, 8), (O.LENNARD_JONES, 6)])
+def test_thread_bfe_equals_serial(kind, dim):
+    # tests/thread_bfe.cpp:66-96: exact equality with per-dv fitness.
+    nx, _, lb, ub = O.problem_info(kind, dim)
+    dvs = O.batch_uniform(lb, ub, 101, 7)
+    f1 = O.bfe(kind, dim, dvs, threads=1)
+    f4 = O.bfe(kind, dim, dvs, threads=4)
+    assert np.array_equal(f1, f4)
+    for i in range(0, 101, 17):
+        assert f1[i, 0] == O.fitness_one(kind, dim, dvs[i])[0]
+
+
+def test_bfe_input_message(kats):
+    msg = kats["bfe_messages"]["input_not_multiple"].format(n=7, nx=3)
+    lib = O.lib()
+    dvs = np.zeros(7)
+    out = np.zeros(7)
+    rc = lib.orc_bfe(O.ROSENBROCK, 3, None, 1, dvs, 1, 7, out, 1)
+    assert rc != 0
+    assert lib.orc_last_error().decode() == msg
+
+
+def test_population_matches_batch_random(kats):
+    # tests/population.cpp:399-451: serial ctor and bfe ctor draw the same dvs.
+    X, F = O.population_init(O.ROSENBROCK, 10, 64, 42)
+    B = O.batch_random_dv(O.ROSENBROCK, 10, 64, 42)
+    assert np.array_equal(X, B)
+    assert X.min() >= -5 and X.max() < 10
+    assert np.array_equal(F[:, 0], O.bfe(O.ROSENBROCK, 10, X)[:, 0])
+
+
+# --- RNG mapping ---------------------------------------------------------------
+
+def canonical(u0, u1):
+    """libstdc++ generate_canonical<double,53> over two mt19937 outputs
+    (bits/random.tcc, GCC 11): (u0 + u1*2^32) / 2^64 rounded, clamped below 1."""
+    r = (float(u0) + float(u1) * 4294967296.0) / 18446744073709551616.0
+    return math.nextafter(1.0, 0.0) if r >= 1.0 else r
+
+
+def test_uniform_real_mapping():
+    u = O.mt19937_u32(42, 2000)
+    d = O.uniform_real(42, -5.0, 10.0, 1000)
+    for i in range(1000):
+        assert d[i] == canonical(u[2 * i], u[2 * i + 1]) * 15.0 + -5.0
+
+
+# --- pso_gen -----------------------------------------------------------------
+
+def test_pso_gen_bfe_equals_serial():
+    # tests/pso_gen.cpp:302-318: evolve with and without a bfe give equal results.
+    X, F = O.population_init(O.ROSENBROCK, 10, 64, 42)
+    a = O.pso_gen_evolve(O.ROSENBROCK, 10, X, F, 30, 42, use_bfe=True, threads=3)
+    b = O.pso_gen_evolve(O.ROSENBROCK, 10, X, F, 30, 42, use_bfe=False)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("topo", [1, 2, 3, 4])
+def test_pso_gen_determinism(variant, topo):
+    # tests/pso_gen.cpp:154-240: the same seed gives the same result.
+    X, F = O.population_init(O.RASTRIGIN, 6, 24, 3)
+    a = O.pso_gen_evolve(O.RASTRIGIN, 6, X, F, 10, 5, variant=variant, neighb_type=topo)
+    b = O.pso_gen_evolve(O.RASTRIGIN, 6, X, F, 10, 5, variant=variant, neighb_type=topo)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    # the memory only ever improves (pso_gen.cpp:443-449)
+    assert np.all(a[1] <= F[:, 0])
+
+
+def _pso_one_gen_python(X, F, seed, lb, ub, fit):
+    """Independent single-generation pso_gen (variant 5, lbest radius 2)
+    written from pso_gen.cpp:177-200,203-218,232-363,443-449."""
+    pop, nx = X.shape
+    u = O.mt19937_u32(seed, 4 * pop * nx + 2 * pop * nx)
+    k = 0
+
+    def draw(a, b):
+        nonlocal k
+        c = canonical(u[k], u[k + 1])
+        k += 2
+        return c * (b - a) + a
+
+    maxv = [(ub[j] - lb[j]) * 0.5 for j in range(nx)]
+    minv = [-1.0 * m for m in maxv]
+    V = [[draw(minv[j], maxv[j]) for j in range(nx)] for _ in range(pop)]
+    neigh = [[(p - 2) % pop, (p - 1) % pop, (p + 1) % pop, (p + 2) % pop] for p in range(pop)]
+    Xn = [list(r) for r in X]
+    for p in range(pop):
+        b = neigh[p][0]
+        for q in neigh[p][1:]:
+            if F[q] <= F[b]:
+                b = q
+        for d in range(nx):
+            r1 = draw(0.0, 1.0)
+            r2 = draw(0.0, 1.0)
+            V[p][d] = 0.7298 * (V[p][d] + 2.05 * r1 * (X[p][d] - X[p][d]) + 2.05 * r2 * (X[b][d] - X[p][d]))
+    for p in range(pop):
+        for d in range(nx):
+            v = min(max(V[p][d], minv[d]), maxv[d]) if (V[p][d] > maxv[d] or V[p][d] < minv[d]) else V[p][d]
+            nxv = Xn[p][d] + v
+            if nxv < lb[d]:
+                nxv = lb[d]
+            elif nxv > ub[d]:
+                nxv = ub[d]
+            Xn[p][d] = nxv
+    Xn = np.array(Xn)
+    fn = np.array([fit(r) for r in Xn])
+    better = fn <= F
+    return np.where(better[:, None], Xn, X), np.where(better, fn, F)
+
+
+def test_pso_gen_single_generation_independent():
+    X, F = O.population_init(O.ROSENBROCK, 5, 12, 9)
+    nx, _, lb, ub = O.problem_info(O.ROSENBROCK, 5)
+    Xo, Fo = O.pso_gen_evolve(O.ROSENBROCK, 5, X, F, 1, 77)
+    Xp, Fp = _pso_one_gen_python(X, F[:, 0], 77, lb, ub, lambda r: O.fitness_one(O.ROSENBROCK, 5, r)[0])
+    assert np.array_equal(Xo, Xp)
+    assert np.array_equal(Fo, Fp)
