@@ -1,0 +1,190 @@
+/* pgh.h -- C ABI of libpagmo_hip.so, the MI355X (gfx950) batch fitness
+ * evaluator for pagmo 2.x.
+ *
+ * Plain pointers and sizes only.  Matrices are row-major and 64-bit sizes
+ * count elements.  Every entry point returns a pgh_status; on failure a
+ * thread-local message is available from pgh_last_error().  Entry points whose
+ * name ends in _host take host buffers and are synchronous; the others take
+ * device pointers and enqueue on `stream` (a hipStream_t, NULL = the default
+ * stream) without synchronising.
+ *
+ * Citations are to the pagmo 2.19.1 sources each entry point replaces or
+ * backs.
+ */
+#ifndef PAGMO_HIP_PGH_H
+#define PAGMO_HIP_PGH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PGH_ABI_VERSION 1
+
+/* Status codes.  The C++ layer (pagmo_hip/hip_bfe.hpp) maps them onto the
+ * exception types of the reference: INVALID_ARGUMENT -> std::invalid_argument
+ * (src/detail/bfe_impl.cpp:71-77,100-118), OVERFLOW -> std::overflow_error
+ * (src/batch_evaluators/thread_bfe.cpp:84-87), anything else ->
+ * std::runtime_error.  UNSUPPORTED means "no kernel for this problem": the
+ * caller falls back to the CPU evaluator. */
+typedef enum pgh_status {
+    PGH_OK = 0,
+    PGH_INVALID_ARGUMENT = 1,
+    PGH_OVERFLOW = 2,
+    PGH_UNSUPPORTED = 3,
+    PGH_DEVICE_ERROR = 4,
+    PGH_NO_DEVICE = 5,
+    PGH_OUT_OF_MEMORY = 6
+} pgh_status;
+
+/* Problems with a device kernel.  The first six are pagmo's UDPs
+ * (src/problems/{rosenbrock,rastrigin,ackley,schwefel,griewank,lennard_jones}.cpp);
+ * MO_SPHERE is the user-defined problem f_k(x) = sum_i (x_i - c_{k,i})^2. */
+typedef enum pgh_udp_kind {
+    PGH_ROSENBROCK = 1,
+    PGH_RASTRIGIN = 2,
+    PGH_ACKLEY = 3,
+    PGH_SCHWEFEL = 4,
+    PGH_GRIEWANK = 5,
+    PGH_LENNARD_JONES = 6,
+    PGH_MO_SPHERE = 7
+} pgh_udp_kind;
+
+/* What hip_bfe extracts from a pagmo::problem (problem.hpp:1104-1147).
+ *  dim: rosenbrock m_dim (rosenbrock.hpp:91); rastrigin/ackley/schwefel/
+ *       griewank get_nx(); lennard_jones atom count (nx = 3*atoms-6,
+ *       lennard_jones.cpp:99-111); MO_SPHERE nx.
+ *  nobj, centers: MO_SPHERE only (nobj x dim, row-major; a device pointer for
+ *       device entry points, a host pointer for _host ones). */
+typedef struct pgh_udp {
+    int32_t kind;
+    uint32_t reserved;
+    uint64_t dim;
+    uint64_t nobj;
+    const double *centers;
+} pgh_udp;
+
+/* ---- library ------------------------------------------------------------ */
+
+/* Thread-local description of the last failure on this thread. */
+const char *pgh_last_error(void);
+int pgh_abi_version(void);
+/* 1 if a gfx950 device is usable by this process, 0 otherwise (never fails). */
+int pgh_device_available(void);
+
+/* nx / nf of a problem description (problem::get_nx/get_nf,
+ * problem.hpp:1360-1400).  INVALID_ARGUMENT for an impossible description
+ * (rosenbrock dim < 2: rosenbrock.cpp:43-49; lennard_jones atoms < 3:
+ * lennard_jones.cpp:51-62; others dim < 1). */
+pgh_status pgh_problem_dims(const pgh_udp *udp, uint64_t *nx, uint64_t *nf);
+
+/* 1 if a kernel exists for this problem, 0 otherwise (no side effects). */
+int pgh_supported(const pgh_udp *udp);
+
+/* ---- batch fitness: bfe::operator() (include/pagmo/bfe.hpp:320,
+ *      src/bfe.cpp:91-110), backing the UDBFE call operator of hip_bfe.
+ *      dvs: n_dvs x nx in, fvs: n_dvs x nf out.  Counters are not touched
+ *      (the pagmo::bfe wrapper increments fevals, bfe.cpp:102-107). ------- */
+pgh_status pgh_bfe_eval(const pgh_udp *udp, const double *dvs, uint64_t n_dvs, uint64_t nx, double *fvs,
+                        uint64_t nf, void *stream);
+
+/* Host-buffer form with the reference's validation: dvs_len must be a
+ * multiple of nx (bfe_impl.cpp:63-88, same message); fvs_len must equal
+ * n_dvs*nf.  Synchronous. */
+pgh_status pgh_bfe_eval_host(const pgh_udp *udp, const double *dvs, uint64_t dvs_len, double *fvs,
+                             uint64_t fvs_len);
+
+/* ---- batched gradient: problem::gradient (src/problem.cpp:456-467) for a
+ *      whole population, dense layout n_dvs x (nf*nx) (problem.cpp:488-507).
+ *      rosenbrock.cpp:83-92, rastrigin.cpp:98-107, and an adjoint kernel for
+ *      lennard_jones (which has no gradient in the reference).  fvs may be
+ *      NULL; when given it receives the fitness computed in the same pass. */
+pgh_status pgh_batch_gradient(const pgh_udp *udp, const double *dvs, uint64_t n_dvs, uint64_t nx, double *grad,
+                              double *fvs, void *stream);
+pgh_status pgh_batch_gradient_host(const pgh_udp *udp, const double *dvs, uint64_t dvs_len, double *grad,
+                                   uint64_t grad_len, double *fvs);
+
+/* ---- fast_non_dominated_sorting (src/utils/multi_objective.cpp:200-257,
+ *      return type include/pagmo/utils/multi_objective.hpp:68-72) over a
+ *      device-resident n x m objective matrix.  Outputs are CSR:
+ *        fronts:   front_offsets[n_fronts+1], front_items[n]
+ *        dom_list: dom_offsets[n+1], dom_items[n_dom_items] (ascending rows)
+ *        dom_count[n], rank[n]
+ *      all uint64 (pop_size_t).  A workspace owns the device results:
+ *      run (async) -> sizes (synchronises) -> copy (to host or device). ---- */
+typedef struct pgh_fnds_ws pgh_fnds_ws;
+pgh_status pgh_fnds_create(pgh_fnds_ws **ws);
+pgh_status pgh_fnds_destroy(pgh_fnds_ws *ws);
+/* INVALID_ARGUMENT when n < 2, with the reference's message
+ * (multi_objective.cpp:203-207). */
+pgh_status pgh_fnds_run(pgh_fnds_ws *ws, const double *objs, uint64_t n, uint64_t m, void *stream);
+pgh_status pgh_fnds_sizes(pgh_fnds_ws *ws, uint64_t *n_fronts, uint64_t *n_dom_items);
+/* Any output pointer may be NULL.  to_host != 0: destinations are host
+ * memory (synchronous); 0: device memory (async on the run's stream). */
+pgh_status pgh_fnds_copy(pgh_fnds_ws *ws, uint64_t *dom_count, uint64_t *rank, uint64_t *front_offsets,
+                         uint64_t *front_items, uint64_t *dom_offsets, uint64_t *dom_items, int to_host);
+
+/* ---- seeded inputs: std::mt19937 (include/pagmo/rng.hpp:43) and
+ *      batch_random_decision_vector (include/pagmo/utils/generic.hpp:326-388)
+ *      generated on the device, bit-identical to libstdc++. ------------- */
+pgh_status pgh_mt19937_generate(uint32_t seed, uint64_t skip, uint64_t n, uint32_t *out, void *stream);
+pgh_status pgh_batch_random_dv(const pgh_udp *udp, uint64_t n, uint32_t seed, double *out, void *stream);
+
+/* ---- pso_gen (src/algorithms/pso_gen.cpp:120-530) on the device.
+ *      Parameters and defaults of pso_gen.hpp:109-111.  Supported on the
+ *      device: variant 5 with the lbest topology (neighb_type 2); other
+ *      combinations return UNSUPPORTED. ------------------------------------ */
+typedef struct pgh_pso_params {
+    uint32_t gen;
+    uint32_t variant;
+    double omega;
+    double eta1;
+    double eta2;
+    double max_vel;
+    uint32_t neighb_type;
+    uint32_t neighb_param;
+    uint32_t memory;
+    uint32_t seed;
+} pgh_pso_params;
+
+/* ---- archipelago of independent pso_gen islands (archipelago::evolve,
+ *      src/archipelago.cpp:325-330, one island per task_queue thread in the
+ *      reference, src/detail/task_queue.cpp:40-107).  Islands may differ in
+ *      problem, dimension and population size.  X0/F0: per-island host
+ *      arrays (pop x nx, pop x 1) of the initial populations. ------------- */
+typedef struct pgh_archi pgh_archi;
+typedef struct pgh_island_spec {
+    pgh_udp udp;
+    uint64_t pop;
+    pgh_pso_params algo; /* per-island seed in algo.seed */
+} pgh_island_spec;
+
+enum pgh_archi_mode {
+    PGH_ARCHI_BATCHED = 0,   /* every phase of a generation is one launch over all islands, graph-captured */
+    PGH_ARCHI_PER_ISLAND = 1 /* each island launches its own kernels, one island after another */
+};
+
+pgh_status pgh_archi_create(pgh_archi **a, const pgh_island_spec *islands, uint64_t n_islands,
+                            const double *const *X0, const double *const *F0);
+pgh_status pgh_archi_destroy(pgh_archi *a);
+/* One pso_gen::evolve call of algo.gen generations on every island
+ * (velocities re-drawn first unless memory is set, pso_gen.cpp:193-200). */
+pgh_status pgh_archi_evolve(pgh_archi *a, int mode, void *stream);
+/* Population of island k after the last evolve (pop.set_xf(i, lbX, lbfit),
+ * pso_gen.cpp:526-528), to host memory.  Synchronous. */
+pgh_status pgh_archi_get(pgh_archi *a, uint64_t k, double *X, double *F);
+/* Every generation's evaluated fitness of island k (gen x pop), recorded
+ * when the archipelago was created with tracing on; see pgh_archi_set_trace. */
+pgh_status pgh_archi_set_trace(pgh_archi *a, int on);
+pgh_status pgh_archi_get_trace(pgh_archi *a, uint64_t k, double *trace_F);
+
+/* Single-island convenience: pso_gen(prm).evolve(pop) with host X/F in/out. */
+pgh_status pgh_pso_gen_evolve_host(const pgh_udp *udp, const pgh_pso_params *prm, double *X, double *F,
+                                   uint64_t pop, uint64_t nx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PAGMO_HIP_PGH_H */

@@ -1,0 +1,295 @@
+"""ctypes binding of libpagmo_hip.so (include/pagmo_hip/pgh.h).
+
+Device entry points take torch tensors that live on the GPU (PyTorch supplies
+device memory and the stream; the computation is the library's HIP kernels).
+Host entry points take numpy arrays.  Nothing here computes a result itself:
+if the library is missing the import of this module fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpagmo_hip.so")
+
+OK, INVALID_ARGUMENT, OVERFLOW, UNSUPPORTED, DEVICE_ERROR, NO_DEVICE, OUT_OF_MEMORY = range(7)
+ROSENBROCK, RASTRIGIN, ACKLEY, SCHWEFEL, GRIEWANK, LENNARD_JONES, MO_SPHERE = range(1, 8)
+ARCHI_BATCHED, ARCHI_PER_ISLAND = 0, 1
+
+# every symbol include/pagmo_hip/pgh.h declares
+EXPORTS = [
+    "pgh_last_error", "pgh_abi_version", "pgh_device_available", "pgh_problem_dims", "pgh_supported",
+    "pgh_bfe_eval", "pgh_bfe_eval_host", "pgh_batch_gradient", "pgh_batch_gradient_host",
+    "pgh_fnds_create", "pgh_fnds_destroy", "pgh_fnds_run", "pgh_fnds_sizes", "pgh_fnds_copy",
+    "pgh_mt19937_generate", "pgh_batch_random_dv",
+    "pgh_archi_create", "pgh_archi_destroy", "pgh_archi_evolve", "pgh_archi_get", "pgh_archi_set_trace",
+    "pgh_archi_get_trace", "pgh_pso_gen_evolve_host",
+]
+
+
+class PghError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(msg)
+        self.status = status
+
+
+class pgh_udp(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("reserved", C.c_uint32), ("dim", C.c_uint64), ("nobj", C.c_uint64),
+                ("centers", C.c_void_p)]
+
+
+class pgh_pso_params(C.Structure):
+    _fields_ = [("gen", C.c_uint32), ("variant", C.c_uint32), ("omega", C.c_double), ("eta1", C.c_double),
+                ("eta2", C.c_double), ("max_vel", C.c_double), ("neighb_type", C.c_uint32),
+                ("neighb_param", C.c_uint32), ("memory", C.c_uint32), ("seed", C.c_uint32)]
+
+
+class pgh_island_spec(C.Structure):
+    _fields_ = [("udp", pgh_udp), ("pop", C.c_uint64), ("algo", pgh_pso_params)]
+
+
+def pso_params(gen=1, omega=0.7298, eta1=2.05, eta2=2.05, max_vel=0.5, variant=5, neighb_type=2, neighb_param=4,
+               memory=False, seed=0) -> pgh_pso_params:
+    """pso_gen's constructor arguments and defaults (pso_gen.hpp:109-111)."""
+    return pgh_pso_params(gen, variant, omega, eta1, eta2, max_vel, neighb_type, neighb_param, int(memory), seed)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built: run `python -m pagmo2_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L):
+    vp, u64, u32, ci = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int
+    udp_p = C.POINTER(pgh_udp)
+    sigs = {
+        "pgh_abi_version": [],
+        "pgh_device_available": [],
+        "pgh_problem_dims": [udp_p, C.POINTER(u64), C.POINTER(u64)],
+        "pgh_supported": [udp_p],
+        "pgh_bfe_eval": [udp_p, vp, u64, u64, vp, u64, vp],
+        "pgh_bfe_eval_host": [udp_p, vp, u64, vp, u64],
+        "pgh_batch_gradient": [udp_p, vp, u64, u64, vp, vp, vp],
+        "pgh_batch_gradient_host": [udp_p, vp, u64, vp, u64, vp],
+        "pgh_fnds_create": [C.POINTER(vp)],
+        "pgh_fnds_destroy": [vp],
+        "pgh_fnds_run": [vp, vp, u64, u64, vp],
+        "pgh_fnds_sizes": [vp, C.POINTER(u64), C.POINTER(u64)],
+        "pgh_fnds_copy": [vp, vp, vp, vp, vp, vp, vp, ci],
+        "pgh_mt19937_generate": [u32, u64, u64, vp, vp],
+        "pgh_batch_random_dv": [udp_p, u64, u32, vp, vp],
+        "pgh_archi_create": [C.POINTER(vp), C.POINTER(pgh_island_spec), u64, C.POINTER(vp), C.POINTER(vp)],
+        "pgh_archi_destroy": [vp],
+        "pgh_archi_evolve": [vp, ci, vp],
+        "pgh_archi_get": [vp, u64, vp, vp],
+        "pgh_archi_set_trace": [vp, ci],
+        "pgh_archi_get_trace": [vp, u64, vp],
+        "pgh_pso_gen_evolve_host": [udp_p, C.POINTER(pgh_pso_params), vp, vp, u64, u64],
+    }
+    for name, args in sigs.items():
+        if not hasattr(L, name):
+            continue  # reported by missing_exports()
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ci
+    L.pgh_last_error.argtypes = []
+    L.pgh_last_error.restype = C.c_char_p
+
+
+def missing_exports():
+    L = lib()
+    return [n for n in EXPORTS if not hasattr(L, n)]
+
+
+def check(status: int) -> None:
+    if status != OK:
+        raise PghError(status, lib().pgh_last_error().decode())
+
+
+def make_udp(kind: int, dim: int, centers=None, nobj: int = 1) -> pgh_udp:
+    """centers: a device tensor (device entry points) or a numpy array (host)."""
+    ptr = None
+    if centers is not None:
+        ptr = centers.data_ptr() if hasattr(centers, "data_ptr") else centers.ctypes.data
+    return pgh_udp(kind, 0, dim, nobj, ptr)
+
+
+def dims(kind: int, dim: int, nobj: int = 1):
+    nx, nf = C.c_uint64(), C.c_uint64()
+    u = make_udp(kind, dim, None, nobj)
+    if kind == MO_SPHERE:
+        u.centers = 1  # only the shape is resolved here
+    check(lib().pgh_problem_dims(C.byref(u), C.byref(nx), C.byref(nf)))
+    return nx.value, nf.value
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    return getattr(stream, "cuda_stream", stream)
+
+
+def _dptr(t):
+    return None if t is None else t.data_ptr()
+
+
+def _hptr(a):
+    return None if a is None else a.ctypes.data
+
+
+# ---- device entry points (torch tensors on the GPU) -----------------------
+
+def bfe_eval(kind, dim, dvs, fvs, centers=None, nobj=1, stream=None):
+    u = make_udp(kind, dim, centers, nobj)
+    n, nx = dvs.shape
+    check(lib().pgh_bfe_eval(C.byref(u), dvs.data_ptr(), n, nx, fvs.data_ptr(), fvs.shape[1], _stream(stream)))
+
+
+def batch_gradient(kind, dim, dvs, grad, fvs=None, stream=None):
+    u = make_udp(kind, dim)
+    n, nx = dvs.shape
+    check(lib().pgh_batch_gradient(C.byref(u), dvs.data_ptr(), n, nx, grad.data_ptr(), _dptr(fvs), _stream(stream)))
+
+
+def mt19937_generate(seed, n, out, skip=0, stream=None):
+    check(lib().pgh_mt19937_generate(seed, skip, n, out.data_ptr(), _stream(stream)))
+
+
+def batch_random_dv(kind, dim, n, seed, out, stream=None):
+    u = make_udp(kind, dim)
+    if kind == MO_SPHERE:
+        u.centers = 1
+        u.nobj = 1
+    check(lib().pgh_batch_random_dv(C.byref(u), n, seed, out.data_ptr(), _stream(stream)))
+
+
+class Fnds:
+    """fast_non_dominated_sorting workspace (device-resident results)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().pgh_fnds_create(C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().pgh_fnds_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def run(self, objs, stream=None):
+        n, m = objs.shape
+        check(lib().pgh_fnds_run(self.h, objs.data_ptr(), n, m, _stream(stream)))
+
+    def sizes(self):
+        a, b = C.c_uint64(), C.c_uint64()
+        check(lib().pgh_fnds_sizes(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def to_host(self, n):
+        n_fronts, n_dom = self.sizes()
+        r = dict(dom_count=np.empty(n, np.uint64), rank=np.empty(n, np.uint64),
+                 front_offsets=np.empty(n_fronts + 1, np.uint64), front_items=np.empty(n, np.uint64),
+                 dom_offsets=np.empty(n + 1, np.uint64), dom_items=np.empty(max(n_dom, 1), np.uint64))
+        check(lib().pgh_fnds_copy(self.h, *(_hptr(r[k]) for k in ("dom_count", "rank", "front_offsets",
+                                                                   "front_items", "dom_offsets", "dom_items")), 1))
+        r["dom_items"] = r["dom_items"][:n_dom]
+        return r
+
+
+# ---- host entry points (numpy) ------------------------------------------------
+
+def bfe_eval_host(kind, dim, dvs: np.ndarray, centers=None, nobj=1) -> np.ndarray:
+    dvs = np.ascontiguousarray(dvs, np.float64)
+    c = None if centers is None else np.ascontiguousarray(centers, np.float64)
+    u = make_udp(kind, dim, c, nobj)
+    nf = nobj if kind == MO_SPHERE else 1
+    nx = dims(kind, dim, nobj)[0]
+    out = np.empty((dvs.size // nx, nf))
+    check(lib().pgh_bfe_eval_host(C.byref(u), dvs.ctypes.data, dvs.size, out.ctypes.data, out.size))
+    return out
+
+
+def batch_gradient_host(kind, dim, dvs: np.ndarray, with_fitness=False):
+    dvs = np.ascontiguousarray(dvs, np.float64)
+    u = make_udp(kind, dim)
+    g = np.empty_like(dvs)
+    nx = dims(kind, dim)[0]
+    f = np.empty(dvs.size // nx) if with_fitness else None
+    check(lib().pgh_batch_gradient_host(C.byref(u), dvs.ctypes.data, dvs.size, g.ctypes.data, g.size, _hptr(f)))
+    return (g, f) if with_fitness else g
+
+
+def pso_gen_evolve_host(kind, dim, X, F, prm: pgh_pso_params):
+    X = np.array(X, np.float64, order="C")
+    F = np.array(F, np.float64, order="C").reshape(-1)
+    u = make_udp(kind, dim)
+    check(lib().pgh_pso_gen_evolve_host(C.byref(u), C.byref(prm), X.ctypes.data, F.ctypes.data, X.shape[0],
+                                        X.shape[1]))
+    return X, F
+
+
+class Archipelago:
+    """Independent pso_gen islands evolved on the device."""
+
+    def __init__(self, islands, trace=False):
+        """islands: list of dicts {kind, dim, X0 (pop x nx), F0 (pop), algo (pgh_pso_params)}."""
+        n = len(islands)
+        specs = (pgh_island_spec * n)()
+        self._keep = []
+        xs = (C.c_void_p * n)()
+        fs = (C.c_void_p * n)()
+        self.shapes = []
+        for k, isl in enumerate(islands):
+            X0 = np.ascontiguousarray(isl["X0"], np.float64)
+            F0 = np.ascontiguousarray(isl["F0"], np.float64).reshape(-1)
+            self._keep += [X0, F0]
+            specs[k].udp = make_udp(isl["kind"], isl["dim"])
+            specs[k].pop = X0.shape[0]
+            specs[k].algo = isl["algo"]
+            xs[k] = X0.ctypes.data
+            fs[k] = F0.ctypes.data
+            self.shapes.append(X0.shape)
+        h = C.c_void_p()
+        check(lib().pgh_archi_create(C.byref(h), specs, n, xs, fs))
+        self.h = h
+        self.gens = [isl["algo"].gen for isl in islands]
+        if trace:
+            check(lib().pgh_archi_set_trace(self.h, 1))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pgh_archi_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def evolve(self, mode=ARCHI_BATCHED, stream=None):
+        check(lib().pgh_archi_evolve(self.h, mode, _stream(stream) if stream is not None else None))
+
+    def get(self, k):
+        pop, nx = self.shapes[k]
+        X = np.empty((pop, nx))
+        F = np.empty(pop)
+        check(lib().pgh_archi_get(self.h, k, X.ctypes.data, F.ctypes.data))
+        return X, F
+
+    def trace(self, k):
+        pop, _ = self.shapes[k]
+        T = np.empty((self.gens[k], pop))
+        check(lib().pgh_archi_get_trace(self.h, k, T.ctypes.data))
+        return T

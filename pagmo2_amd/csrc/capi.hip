@@ -1,0 +1,320 @@
+// capi.hip -- the extern "C" entry points of libpagmo_hip.so: validation,
+// error reporting, host staging and dispatch to the kernel launchers.
+#include <cstring>
+#include <limits>
+#include <string>
+
+#include "pgh_internal.hpp"
+
+namespace pgh
+{
+
+namespace
+{
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string &msg)
+{
+    g_last_error = msg;
+}
+
+pgh_status fail(pgh_status st, const std::string &msg)
+{
+    g_last_error = msg;
+    return st;
+}
+
+pgh_status hip_fail(hipError_t e, const char *what)
+{
+    const pgh_status st = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? PGH_OUT_OF_MEMORY
+                          : (e == hipErrorNoDevice || e == hipErrorInvalidDevice) ? PGH_NO_DEVICE
+                                                                                   : PGH_DEVICE_ERROR;
+    g_last_error = std::string("HIP error '") + hipGetErrorString(e) + "' in " + what;
+    return st;
+}
+
+pgh_status dev_buf::reserve(size_t bytes)
+{
+    if (bytes <= cap) return PGH_OK;
+    if (ptr) {
+        (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+    // round up to 2 MiB so that slowly growing batches do not reallocate
+    const size_t want = (bytes + (size_t(2) << 20) - 1u) & ~((size_t(2) << 20) - 1u);
+    PGH_HIP(hipMalloc(&ptr, want));
+    cap = want;
+    return PGH_OK;
+}
+
+dev_buf::~dev_buf()
+{
+    // Buffers live until their host thread ends; at process exit the HIP
+    // runtime may already be gone, so the release is best effort.
+    if (ptr) (void)hipFree(ptr);
+}
+
+pgh_status get_thread_ctx(thread_ctx **out)
+{
+    thread_local thread_ctx ctx;
+    if (!ctx.stream) {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+            return fail(PGH_NO_DEVICE, "no HIP device is available");
+        }
+        PGH_HIP(hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking));
+    }
+    *out = &ctx;
+    return PGH_OK;
+}
+
+pgh_status resolve(const pgh_udp *udp, prob &out)
+{
+    if (!udp) return fail(PGH_INVALID_ARGUMENT, "null problem description");
+    out = prob{};
+    out.kind = udp->kind;
+    const uint64_t dim = udp->dim;
+    const uint64_t lim = std::numeric_limits<uint32_t>::max() / 4u;
+    switch (udp->kind) {
+        case PGH_ROSENBROCK:
+            // rosenbrock.cpp:43-49
+            if (dim < 2u) {
+                return fail(PGH_INVALID_ARGUMENT, "Rosenbrock Function must have minimum 2 dimensions, "
+                                                      + std::to_string(dim) + " requested");
+            }
+            break;
+        case PGH_RASTRIGIN:
+        case PGH_ACKLEY:
+        case PGH_SCHWEFEL:
+        case PGH_GRIEWANK:
+            if (dim < 1u) {
+                static const char *names[] = {"", "", "Rastrigin", "Ackley", "Schwefel", "Griewank"};
+                return fail(PGH_INVALID_ARGUMENT, std::string(names[udp->kind]) + " Function must have minimum 1 dimension, "
+                                                      + std::to_string(dim) + " requested");
+            }
+            break;
+        case PGH_LENNARD_JONES:
+            // lennard_jones.cpp:51-62
+            if (dim < 3u) {
+                return fail(PGH_INVALID_ARGUMENT, "The number of atoms in a Lennard Jones Clusters problem must be "
+                                                  "positive and greater than 2, while a number of "
+                                                      + std::to_string(dim) + " was detected.");
+            }
+            break;
+        case PGH_MO_SPHERE:
+            if (dim < 1u || udp->nobj < 1u) {
+                return fail(PGH_INVALID_ARGUMENT, "MO_SPHERE needs dim >= 1 and nobj >= 1");
+            }
+            if (!udp->centers) return fail(PGH_INVALID_ARGUMENT, "MO_SPHERE needs centers");
+            if (udp->nobj > 1024u) return fail(PGH_UNSUPPORTED, "MO_SPHERE supports at most 1024 objectives");
+            break;
+        default:
+            return fail(PGH_UNSUPPORTED, "no kernel for problem kind " + std::to_string(udp->kind));
+    }
+    if (dim > lim) return fail(PGH_OVERFLOW, "problem dimension too large: " + std::to_string(dim));
+    out.dim = static_cast<uint32_t>(dim);
+    out.nx = udp->kind == PGH_LENNARD_JONES ? 3u * out.dim - 6u : out.dim;
+    out.nf = udp->kind == PGH_MO_SPHERE ? static_cast<uint32_t>(udp->nobj) : 1u;
+    out.centers = udp->centers;
+    return PGH_OK;
+}
+
+namespace
+{
+
+pgh_status check_shape(const prob &p, uint64_t nx, uint64_t nf)
+{
+    if (nx != p.nx) {
+        return fail(PGH_INVALID_ARGUMENT, "A decision vector is incompatible with the problem: the number of "
+                                          "dimensions of the problem is "
+                                              + std::to_string(p.nx) + ", while the decision vector has a size of "
+                                              + std::to_string(nx) + " (the two values should be equal)");
+    }
+    if (nf != p.nf) {
+        return fail(PGH_INVALID_ARGUMENT, "A fitness vector is incompatible with the problem: the dimension of the "
+                                          "fitness of the problem is "
+                                              + std::to_string(p.nf) + ", while the fitness vector has a size of "
+                                              + std::to_string(nf) + " (the two values should be equal)");
+    }
+    return PGH_OK;
+}
+
+// problem::gradient throws not_implemented_error for UDPs without a gradient
+// (problem.hpp:633-645); lennard_jones gets the new adjoint kernel.
+pgh_status check_has_gradient(const prob &p)
+{
+    if (p.kind == PGH_ROSENBROCK || p.kind == PGH_RASTRIGIN || p.kind == PGH_LENNARD_JONES) return PGH_OK;
+    return fail(PGH_UNSUPPORTED, "The gradient has not been implemented in the user-defined problem");
+}
+
+// src/detail/bfe_impl.cpp:63-88
+pgh_status check_input_len(const prob &p, uint64_t len, uint64_t &n_dvs)
+{
+    if (len % p.nx) {
+        return fail(PGH_INVALID_ARGUMENT,
+                    "Invalid argument for a batch fitness evaluation: the length of the vector representing the "
+                    "decision vectors, "
+                        + std::to_string(len) + ", is not an exact multiple of the dimension of the problem, "
+                        + std::to_string(p.nx));
+    }
+    n_dvs = len / p.nx;
+    return PGH_OK;
+}
+
+// Copies host MO_SPHERE centres to the device buffer `buf` and repoints p.
+pgh_status stage_centers(prob &p, const pgh_udp *udp, dev_buf &buf, hipStream_t s)
+{
+    if (p.kind != PGH_MO_SPHERE) return PGH_OK;
+    const size_t bytes = static_cast<size_t>(p.nf) * p.nx * sizeof(double);
+    pgh_status st = buf.reserve(bytes);
+    if (st != PGH_OK) return st;
+    PGH_HIP(hipMemcpyAsync(buf.ptr, udp->centers, bytes, hipMemcpyHostToDevice, s));
+    p.centers = static_cast<const double *>(buf.ptr);
+    return PGH_OK;
+}
+
+#define PGH_TRY(expr)                                                                                                \
+    do {                                                                                                             \
+        const pgh_status pgh_st_ = (expr);                                                                           \
+        if (pgh_st_ != PGH_OK) return pgh_st_;                                                                       \
+    } while (0)
+
+} // namespace
+} // namespace pgh
+
+using namespace pgh;
+
+extern "C" {
+
+const char *pgh_last_error(void)
+{
+    return g_last_error.c_str();
+}
+
+int pgh_abi_version(void)
+{
+    return PGH_ABI_VERSION;
+}
+
+int pgh_device_available(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
+    return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+}
+
+pgh_status pgh_problem_dims(const pgh_udp *udp, uint64_t *nx, uint64_t *nf)
+{
+    prob p;
+    PGH_TRY(resolve(udp, p));
+    if (nx) *nx = p.nx;
+    if (nf) *nf = p.nf;
+    return PGH_OK;
+}
+
+int pgh_supported(const pgh_udp *udp)
+{
+    const std::string saved = g_last_error;
+    prob p;
+    const bool ok = resolve(udp, p) == PGH_OK;
+    g_last_error = saved;
+    return ok ? 1 : 0;
+}
+
+pgh_status pgh_bfe_eval(const pgh_udp *udp, const double *dvs, uint64_t n_dvs, uint64_t nx, double *fvs, uint64_t nf,
+                        void *stream)
+{
+    prob p;
+    PGH_TRY(resolve(udp, p));
+    PGH_TRY(check_shape(p, nx, nf));
+    if (n_dvs && (!dvs || !fvs)) return fail(PGH_INVALID_ARGUMENT, "null buffer");
+    return launch_fitness(p, dvs, n_dvs, fvs, static_cast<hipStream_t>(stream));
+}
+
+pgh_status pgh_bfe_eval_host(const pgh_udp *udp, const double *dvs, uint64_t dvs_len, double *fvs, uint64_t fvs_len)
+{
+    prob p;
+    PGH_TRY(resolve(udp, p));
+    uint64_t n_dvs = 0;
+    PGH_TRY(check_input_len(p, dvs_len, n_dvs));
+    if (fvs_len != n_dvs * p.nf) {
+        return fail(PGH_INVALID_ARGUMENT, "the fitness buffer holds " + std::to_string(fvs_len)
+                                              + " values, while " + std::to_string(n_dvs * p.nf) + " are produced");
+    }
+    if (n_dvs == 0u) return PGH_OK;
+    thread_ctx *ctx = nullptr;
+    PGH_TRY(get_thread_ctx(&ctx));
+    const size_t in_bytes = dvs_len * sizeof(double), out_bytes = fvs_len * sizeof(double);
+    PGH_TRY(ctx->a.reserve(in_bytes));
+    PGH_TRY(ctx->b.reserve(out_bytes));
+    PGH_TRY(stage_centers(p, udp, ctx->c, ctx->stream));
+    PGH_HIP(hipMemcpyAsync(ctx->a.ptr, dvs, in_bytes, hipMemcpyHostToDevice, ctx->stream));
+    PGH_TRY(launch_fitness(p, static_cast<const double *>(ctx->a.ptr), n_dvs, static_cast<double *>(ctx->b.ptr),
+                           ctx->stream));
+    PGH_HIP(hipMemcpyAsync(fvs, ctx->b.ptr, out_bytes, hipMemcpyDeviceToHost, ctx->stream));
+    PGH_HIP(hipStreamSynchronize(ctx->stream));
+    return PGH_OK;
+}
+
+pgh_status pgh_batch_gradient(const pgh_udp *udp, const double *dvs, uint64_t n_dvs, uint64_t nx, double *grad,
+                              double *fvs, void *stream)
+{
+    prob p;
+    PGH_TRY(resolve(udp, p));
+    PGH_TRY(check_shape(p, nx, p.nf));
+    PGH_TRY(check_has_gradient(p));
+    if (n_dvs && (!dvs || !grad)) return fail(PGH_INVALID_ARGUMENT, "null buffer");
+    return launch_gradient(p, dvs, n_dvs, grad, fvs, static_cast<hipStream_t>(stream));
+}
+
+pgh_status pgh_batch_gradient_host(const pgh_udp *udp, const double *dvs, uint64_t dvs_len, double *grad,
+                                   uint64_t grad_len, double *fvs)
+{
+    prob p;
+    PGH_TRY(resolve(udp, p));
+    PGH_TRY(check_has_gradient(p));
+    uint64_t n_dvs = 0;
+    PGH_TRY(check_input_len(p, dvs_len, n_dvs));
+    if (grad_len != dvs_len) {
+        return fail(PGH_INVALID_ARGUMENT, "the gradient buffer holds " + std::to_string(grad_len)
+                                              + " values, while " + std::to_string(dvs_len) + " are produced");
+    }
+    if (n_dvs == 0u) return PGH_OK;
+    thread_ctx *ctx = nullptr;
+    PGH_TRY(get_thread_ctx(&ctx));
+    const size_t bytes = dvs_len * sizeof(double);
+    PGH_TRY(ctx->a.reserve(bytes));
+    PGH_TRY(ctx->b.reserve(bytes));
+    PGH_TRY(ctx->d.reserve(n_dvs * sizeof(double)));
+    PGH_HIP(hipMemcpyAsync(ctx->a.ptr, dvs, bytes, hipMemcpyHostToDevice, ctx->stream));
+    PGH_TRY(launch_gradient(p, static_cast<const double *>(ctx->a.ptr), n_dvs, static_cast<double *>(ctx->b.ptr),
+                            fvs ? static_cast<double *>(ctx->d.ptr) : nullptr, ctx->stream));
+    PGH_HIP(hipMemcpyAsync(grad, ctx->b.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    if (fvs) {
+        PGH_HIP(hipMemcpyAsync(fvs, ctx->d.ptr, n_dvs * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    PGH_HIP(hipStreamSynchronize(ctx->stream));
+    return PGH_OK;
+}
+
+pgh_status pgh_mt19937_generate(uint32_t seed, uint64_t skip, uint64_t n, uint32_t *out, void *stream)
+{
+    if (n && !out) return fail(PGH_INVALID_ARGUMENT, "null buffer");
+    return launch_mt19937(seed, skip, n, out, static_cast<hipStream_t>(stream));
+}
+
+pgh_status pgh_batch_random_dv(const pgh_udp *udp, uint64_t n, uint32_t seed, double *out, void *stream)
+{
+    prob p;
+    PGH_TRY(resolve(udp, p));
+    if (n && !out) return fail(PGH_INVALID_ARGUMENT, "null buffer");
+    return launch_batch_random_dv(p, n, seed, out, static_cast<hipStream_t>(stream));
+}
+
+} // extern "C"

@@ -1,0 +1,236 @@
+// fitness.hip -- batch fitness and batch gradient kernels (the hip_bfe path).
+//
+// Data layout: dvs is the row-major n_dvs x nx matrix that pagmo packs before
+// every bfe call (src/algorithms/pso_gen.cpp:420-425, src/problem.cpp:383-392);
+// fvs is n_dvs x nf, grad is n_dvs x nx (dense, nf == 1).
+#include "fitness_dev.hpp"
+#include "pgh_internal.hpp"
+
+namespace pgh
+{
+using namespace dev;
+
+namespace
+{
+
+constexpr int kBlock = 256;
+
+// ---- separable problems: an L-lane group per individual -------------------
+template <int KIND, int L, bool ALIGNED>
+__global__ __launch_bounds__(kBlock) void k_separable(const double *__restrict__ dvs, uint64_t n, uint32_t nx,
+                                                      double *__restrict__ fvs)
+{
+    constexpr uint32_t G = kBlock / L;
+    const uint64_t ind = static_cast<uint64_t>(blockIdx.x) * G + threadIdx.x / L;
+    const int r = static_cast<int>(threadIdx.x % L);
+    // a trailing partial group re-reads the last row so that every lane of the
+    // group takes part in the shuffles; its result is dropped
+    const uint64_t row = ind < n ? ind : n - 1u;
+    const double f = separable_eval<KIND, L, ALIGNED>(dvs + row * nx, nx, r);
+    if (ind < n && r == 0) fvs[ind] = f;
+}
+
+// ---- rosenbrock: one lane per individual ------------------------------------
+__global__ __launch_bounds__(kBlock) void k_rosenbrock(const double *__restrict__ dvs, uint64_t n, uint32_t nx,
+                                                       double *__restrict__ fvs)
+{
+    const uint64_t ind = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (ind >= n) return;
+    fvs[ind] = rosenbrock_eval(dvs + ind * nx, nx);
+}
+
+// ---- multi-objective sphere: one lane per individual ------------------------
+__global__ __launch_bounds__(kBlock) void k_mo_sphere(const double *__restrict__ dvs, uint64_t n, uint32_t nx,
+                                                      uint32_t nobj, const double *__restrict__ centers,
+                                                      double *__restrict__ fvs)
+{
+    const uint64_t ind = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (ind >= n) return;
+    const double *x = dvs + ind * nx;
+    for (uint32_t k = 0; k < nobj; ++k) {
+        fvs[ind * nobj + k] = mo_sphere_eval(x, nx, centers + static_cast<uint64_t>(k) * nx);
+    }
+}
+
+// ---- lennard_jones: G individuals per workgroup staged in LDS --------------
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_lj(const double *__restrict__ dvs, uint64_t n, uint32_t atoms,
+                                               uint32_t G, double *__restrict__ fvs)
+{
+    extern __shared__ double lds[];
+    const uint32_t nx = 3u * atoms - 6u;
+    const uint64_t ind0 = static_cast<uint64_t>(blockIdx.x) * G;
+    const uint32_t count = static_cast<uint32_t>(min(static_cast<uint64_t>(G), n - ind0));
+    lj_stage(dvs + ind0 * nx, count, atoms, lds);
+    __syncthreads();
+    const uint32_t g = threadIdx.x / L;
+    const int r = static_cast<int>(threadIdx.x % L);
+    const uint32_t gg = g < count ? g : count - 1u;
+    const double e = lj_energy<L>(lds + gg * 3u * atoms, atoms, r);
+    if (g < count && r == 0) fvs[ind0 + g] = e;
+}
+
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_lj_grad(const double *__restrict__ dvs, uint64_t n, uint32_t atoms,
+                                                    uint32_t G, double *__restrict__ grad, double *__restrict__ fvs)
+{
+    extern __shared__ double lds[];
+    const uint32_t nx = 3u * atoms - 6u;
+    const uint64_t ind0 = static_cast<uint64_t>(blockIdx.x) * G;
+    const uint32_t count = static_cast<uint32_t>(min(static_cast<uint64_t>(G), n - ind0));
+    lj_stage(dvs + ind0 * nx, count, atoms, lds);
+    __syncthreads();
+    const uint32_t g = threadIdx.x / L;
+    const int r = static_cast<int>(threadIdx.x % L);
+    if (g >= count) return; // whole groups retire together: no shuffle partner is lost
+    const double e = lj_energy_grad<L>(lds + g * 3u * atoms, atoms, r, grad + (ind0 + g) * nx);
+    if (fvs && r == 0) fvs[ind0 + g] = e;
+}
+
+// ---- elementwise gradients ---------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_rosenbrock_grad(const double *__restrict__ dvs, uint64_t n, uint32_t nx,
+                                                            double *__restrict__ grad)
+{
+    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= n * nx) return;
+    const uint64_t ind = t / nx;
+    const uint32_t i = static_cast<uint32_t>(t - ind * nx);
+    grad[t] = rosenbrock_grad(dvs + ind * nx, nx, i);
+}
+
+__global__ __launch_bounds__(kBlock) void k_rastrigin_grad(const double *__restrict__ dvs, uint64_t count,
+                                                           double *__restrict__ grad)
+{
+    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= count) return;
+    grad[t] = rastrigin_grad(dvs[t]);
+}
+
+template <int KIND, int L>
+pgh_status launch_sep_l(const double *dvs, uint64_t n, uint32_t nx, double *fvs, hipStream_t s)
+{
+    constexpr uint64_t G = kBlock / L;
+    const uint64_t blocks = (n + G - 1u) / G;
+    const bool aligned = (nx % 2u == 0u) && (reinterpret_cast<uintptr_t>(dvs) % 16u == 0u);
+    if (aligned) {
+        k_separable<KIND, L, true><<<dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, s>>>(dvs, n, nx, fvs);
+    } else {
+        k_separable<KIND, L, false><<<dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, s>>>(dvs, n, nx, fvs);
+    }
+    PGH_HIP(hipGetLastError());
+    return PGH_OK;
+}
+
+template <int KIND>
+pgh_status launch_sep(const double *dvs, uint64_t n, uint32_t nx, double *fvs, hipStream_t s)
+{
+    switch (sep_lanes(nx)) {
+        case 64:
+            return launch_sep_l<KIND, 64>(dvs, n, nx, fvs, s);
+        case 16:
+            return launch_sep_l<KIND, 16>(dvs, n, nx, fvs, s);
+        default:
+            return launch_sep_l<KIND, 4>(dvs, n, nx, fvs, s);
+    }
+}
+
+template <int L>
+pgh_status launch_lj_l(const double *dvs, uint64_t n, uint32_t atoms, double *fvs, double *grad, hipStream_t s)
+{
+    const uint32_t G = lj_group_count(atoms);
+    const uint64_t blocks = (n + G - 1u) / G;
+    const size_t lds = static_cast<size_t>(G) * 3u * atoms * sizeof(double);
+    if (grad) {
+        k_lj_grad<L><<<dim3(static_cast<uint32_t>(blocks)), dim3(G * L), lds, s>>>(dvs, n, atoms, G, grad, fvs);
+    } else {
+        k_lj<L><<<dim3(static_cast<uint32_t>(blocks)), dim3(G * L), lds, s>>>(dvs, n, atoms, G, fvs);
+    }
+    PGH_HIP(hipGetLastError());
+    return PGH_OK;
+}
+
+pgh_status launch_lj(const double *dvs, uint64_t n, uint32_t atoms, double *fvs, double *grad, hipStream_t s)
+{
+    if (lj_group_count(atoms) == 0u) {
+        return fail(PGH_UNSUPPORTED, "lennard_jones with " + std::to_string(atoms)
+                                         + " atoms does not fit the LDS of one workgroup");
+    }
+    switch (lj_lanes(atoms)) {
+        case 16:
+            return launch_lj_l<16>(dvs, n, atoms, fvs, grad, s);
+        case 8:
+            return launch_lj_l<8>(dvs, n, atoms, fvs, grad, s);
+        case 4:
+            return launch_lj_l<4>(dvs, n, atoms, fvs, grad, s);
+        case 2:
+            return launch_lj_l<2>(dvs, n, atoms, fvs, grad, s);
+        default:
+            return launch_lj_l<1>(dvs, n, atoms, fvs, grad, s);
+    }
+}
+
+uint32_t blocks_for(uint64_t count)
+{
+    return static_cast<uint32_t>((count + kBlock - 1u) / kBlock);
+}
+
+} // namespace
+
+// Grid sizes stay below 2^31 workgroups: at most 2^31 individuals per call and
+// 2^39 matrix elements for the elementwise gradient kernels.
+pgh_status check_launch_size(uint64_t n, uint32_t nx)
+{
+    if (n > (uint64_t(1) << 31) || n * nx > (uint64_t(1) << 39)) {
+        return fail(PGH_OVERFLOW, "batch too large for one launch: " + std::to_string(n) + " x "
+                                      + std::to_string(nx));
+    }
+    return PGH_OK;
+}
+
+pgh_status launch_fitness(const prob &p, const double *dvs, uint64_t n, double *fvs, hipStream_t s)
+{
+    if (n == 0u) return PGH_OK;
+    if (check_launch_size(n, p.nx) != PGH_OK) return PGH_OVERFLOW;
+    switch (p.kind) {
+        case ROSENBROCK:
+            k_rosenbrock<<<dim3(blocks_for(n)), dim3(kBlock), 0, s>>>(dvs, n, p.nx, fvs);
+            PGH_HIP(hipGetLastError());
+            return PGH_OK;
+        case RASTRIGIN:
+            return launch_sep<RASTRIGIN>(dvs, n, p.nx, fvs, s);
+        case ACKLEY:
+            return launch_sep<ACKLEY>(dvs, n, p.nx, fvs, s);
+        case SCHWEFEL:
+            return launch_sep<SCHWEFEL>(dvs, n, p.nx, fvs, s);
+        case GRIEWANK:
+            return launch_sep<GRIEWANK>(dvs, n, p.nx, fvs, s);
+        case LENNARD_JONES:
+            return launch_lj(dvs, n, p.dim, fvs, nullptr, s);
+        case MO_SPHERE:
+            k_mo_sphere<<<dim3(blocks_for(n)), dim3(kBlock), 0, s>>>(dvs, n, p.nx, p.nf, p.centers, fvs);
+            PGH_HIP(hipGetLastError());
+            return PGH_OK;
+    }
+    return fail(PGH_UNSUPPORTED, "no fitness kernel for this problem kind");
+}
+
+pgh_status launch_gradient(const prob &p, const double *dvs, uint64_t n, double *grad, double *fvs, hipStream_t s)
+{
+    if (n == 0u) return PGH_OK;
+    if (check_launch_size(n, p.nx) != PGH_OK) return PGH_OVERFLOW;
+    switch (p.kind) {
+        case ROSENBROCK:
+            k_rosenbrock_grad<<<dim3(blocks_for(n * p.nx)), dim3(kBlock), 0, s>>>(dvs, n, p.nx, grad);
+            PGH_HIP(hipGetLastError());
+            return fvs ? launch_fitness(p, dvs, n, fvs, s) : PGH_OK;
+        case RASTRIGIN:
+            k_rastrigin_grad<<<dim3(blocks_for(n * p.nx)), dim3(kBlock), 0, s>>>(dvs, n * p.nx, grad);
+            PGH_HIP(hipGetLastError());
+            return fvs ? launch_fitness(p, dvs, n, fvs, s) : PGH_OK;
+        case LENNARD_JONES:
+            return launch_lj(dvs, n, p.dim, fvs, grad, s);
+    }
+    return fail(PGH_UNSUPPORTED, "The gradient has not been implemented in the user-defined problem");
+}
+
+} // namespace pgh

@@ -1,0 +1,61 @@
+// pgh_internal.hpp -- shared declarations of libpagmo_hip.so (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "pagmo_hip/pgh.h"
+
+namespace pgh
+{
+
+// ---- error state (capi.hip) ------------------------------------------------
+void set_error(const std::string &msg);
+pgh_status fail(pgh_status st, const std::string &msg);
+pgh_status hip_fail(hipError_t e, const char *what);
+
+#define PGH_HIP(call)                                                                                                \
+    do {                                                                                                             \
+        hipError_t pgh_e_ = (call);                                                                                  \
+        if (pgh_e_ != hipSuccess) return ::pgh::hip_fail(pgh_e_, #call);                                             \
+    } while (0)
+
+// ---- problem description after validation ---------------------------------
+struct prob {
+    int kind = 0;
+    uint32_t dim = 0;   // rosenbrock m_dim, lj atoms, otherwise nx
+    uint32_t nx = 0;
+    uint32_t nf = 0;
+    const double *centers = nullptr; // MO_SPHERE, device pointer
+};
+
+// Validates a pgh_udp; fills `out`.  Does not touch centers' memory.
+pgh_status resolve(const pgh_udp *udp, prob &out);
+
+// ---- launchers (fitness.hip) -----------------------------------------------
+pgh_status launch_fitness(const prob &p, const double *dvs, uint64_t n_dvs, double *fvs, hipStream_t s);
+pgh_status launch_gradient(const prob &p, const double *dvs, uint64_t n_dvs, double *grad, double *fvs,
+                           hipStream_t s);
+
+// ---- rng (pso.hip) ------------------------------------------------------------
+pgh_status launch_mt19937(uint32_t seed, uint64_t skip, uint64_t n, uint32_t *out, hipStream_t s);
+pgh_status launch_batch_random_dv(const prob &p, uint64_t n, uint32_t seed, double *out, hipStream_t s);
+
+// ---- device scratch owned by the calling host thread ---------------------
+struct dev_buf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    pgh_status reserve(size_t bytes);
+    ~dev_buf();
+};
+
+// Per-host-thread stream and staging buffers for the _host entry points.
+struct thread_ctx {
+    hipStream_t stream = nullptr;
+    dev_buf a, b, c, d;
+};
+pgh_status get_thread_ctx(thread_ctx **ctx);
+
+} // namespace pgh

@@ -1,0 +1,98 @@
+"""C-ABI host logic, no GPU: the library loads, exports every declared symbol,
+and validates its inputs with the reference's messages before touching a
+device."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pagmo2_amd as P
+from pagmo2_amd import _capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_exports_match_header():
+    with open(os.path.join(ROOT, "include", "pagmo_hip", "pgh.h")) as f:
+        hdr = f.read()
+    declared = set(re.findall(r"\b(pgh_[a-z0-9_]+)\s*\(", hdr))
+    assert declared == set(_capi.EXPORTS)
+    assert P.missing_exports() == []
+
+
+def test_abi_version():
+    assert P.lib().pgh_abi_version() == 1
+
+
+def test_problem_dims():
+    assert P.dims(P.ROSENBROCK, 10) == (10, 1)
+    assert P.dims(P.LENNARD_JONES, 48) == (138, 1)
+    assert P.dims(P.RASTRIGIN, 1000) == (1000, 1)
+    assert P.dims(P.MO_SPHERE, 30, nobj=2) == (30, 2)
+
+
+@pytest.mark.parametrize("kind,dim,msg", [
+    (1, 1, "Rosenbrock Function must have minimum 2 dimensions, 1 requested"),        # rosenbrock.cpp:43-49
+    (2, 0, "Rastrigin Function must have minimum 1 dimension, 0 requested"),         # rastrigin.cpp:46-51
+    (3, 0, "Ackley Function must have minimum 1 dimension, 0 requested"),
+    (4, 0, "Schwefel Function must have minimum 1 dimension, 0 requested"),
+    (5, 0, "Griewank Function must have minimum 1 dimension, 0 requested"),
+    (6, 2, "The number of atoms in a Lennard Jones Clusters problem must be positive and greater than 2, "
+           "while a number of 2 was detected."),                                        # lennard_jones.cpp:51-57
+])
+def test_problem_ctor_errors(kind, dim, msg):
+    with pytest.raises(P.PghError) as e:
+        P.dims(kind, dim)
+    assert e.value.status == P.INVALID_ARGUMENT
+    assert str(e.value) == msg
+
+
+def test_unknown_kind_is_unsupported():
+    u = P.make_udp(42, 3)
+    assert P.lib().pgh_supported(C.byref(u)) == 0
+    assert P.lib().pgh_supported(C.byref(P.make_udp(P.ROSENBROCK, 3))) == 1
+
+
+def test_bfe_host_input_length_message(kats):
+    # src/detail/bfe_impl.cpp:63-88, tested verbatim by tests/bfe.cpp:366-402
+    u = P.make_udp(P.ROSENBROCK, 3)
+    dvs = np.zeros(7)
+    out = np.zeros(2)
+    rc = P.lib().pgh_bfe_eval_host(C.byref(u), dvs.ctypes.data, 7, out.ctypes.data, 2)
+    assert rc == P.INVALID_ARGUMENT
+    assert P.lib().pgh_last_error().decode() == kats["bfe_messages"]["input_not_multiple"].format(n=7, nx=3)
+
+
+def test_bfe_device_shape_mismatch():
+    u = P.make_udp(P.RASTRIGIN, 5)
+    rc = P.lib().pgh_bfe_eval(C.byref(u), None, 0, 4, None, 1, None)
+    assert rc == P.INVALID_ARGUMENT
+    assert "the number of dimensions of the problem is 5" in P.lib().pgh_last_error().decode()
+
+
+def test_gradient_unsupported_kind():
+    # ackley has no gradient in the reference (problem::gradient throws
+    # not_implemented_error, problem.hpp:633-645)
+    u = P.make_udp(P.ACKLEY, 4)
+    rc = P.lib().pgh_batch_gradient(C.byref(u), None, 0, 4, None, None, None)
+    assert rc == P.UNSUPPORTED
+
+
+def test_empty_batch_is_noop():
+    # zero decision vectors: nothing to launch, no device needed
+    u = P.make_udp(P.ROSENBROCK, 3)
+    dvs = np.zeros(0)
+    out = np.zeros(0)
+    assert P.lib().pgh_bfe_eval_host(C.byref(u), dvs.ctypes.data, 0, out.ctypes.data, 0) == P.OK
+
+
+def test_no_device_reported():
+    if P.lib().pgh_device_available():
+        pytest.skip("a GPU is present")
+    u = P.make_udp(P.ROSENBROCK, 3)
+    dvs = np.zeros(6)
+    out = np.zeros(2)
+    rc = P.lib().pgh_bfe_eval_host(C.byref(u), dvs.ctypes.data, 6, out.ctypes.data, 2)
+    assert rc == P.NO_DEVICE

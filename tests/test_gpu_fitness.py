@@ -1,0 +1,215 @@
+"""Parity of the HIP batch fitness / gradient kernels with the oracle.
+
+Bars: rosenbrock and the multi-objective sphere are bit-exact (only + - *
+in the reference's order); the transcendental problems are within 1e-12
+relative; Lennard-Jones within 1e-12 of the sum of the absolute pair terms
+(its energy cancels); gradients as stated per test.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pagmo2_amd as P
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+DEV = "cuda:0"
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def gpu_bfe(kind, dim, dvs, centers=None, nobj=1):
+    d = dev(dvs)
+    nf = nobj if kind == P.MO_SPHERE else 1
+    out = torch.empty((dvs.shape[0], nf), dtype=torch.float64, device=DEV)
+    c = None if centers is None else dev(centers)
+    P.bfe_eval(kind, dim, d, out, centers=c, nobj=nobj)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def inputs(kind, dim, n, seed, lo=None, hi=None):
+    nx, _, lb, ub = O.problem_info(kind, dim)
+    if lo is not None:
+        lb, ub = np.full(nx, lo), np.full(nx, hi)
+    return O.batch_uniform(lb, ub, n, seed)
+
+
+def lj_abs_scale(x):
+    atoms = (x.size + 6) // 3
+    p = np.zeros((atoms, 3))
+    p[1, 2] = x[0]
+    p[2, 1:] = x[1:3]
+    p[3:] = x[3:].reshape(-1, 3)
+    d = p[:, None, :] - p[None, :, :]
+    r2 = np.sum(d * d, axis=-1)[np.triu_indices(atoms, 1)]
+    s = r2 ** -3
+    return 4 * np.sum(np.abs(s * s) + np.abs(s))
+
+
+@pytest.mark.parametrize("n", [1, 3, 64, 257])
+@pytest.mark.parametrize("dim", [2, 3, 10, 37, 500])
+def test_rosenbrock_bit_exact(n, dim):
+    x = inputs(P.ROSENBROCK, dim, n, 100 + dim)
+    assert np.array_equal(gpu_bfe(P.ROSENBROCK, dim, x), O.bfe(P.ROSENBROCK, dim, x, threads=8))
+
+
+@pytest.mark.parametrize("kind", [P.RASTRIGIN, P.ACKLEY, P.SCHWEFEL, P.GRIEWANK])
+@pytest.mark.parametrize("dim", [1, 2, 7, 31, 32, 33, 255, 256, 1000, 1001])
+@pytest.mark.parametrize("n", [1, 5, 130])
+def test_separable_rel_1e12(kind, dim, n):
+    x = inputs(kind, dim, n, 7 * dim + n)
+    g = gpu_bfe(kind, dim, x)
+    r = O.bfe(kind, dim, x, threads=8)
+    np.testing.assert_allclose(g, r, rtol=1e-12, atol=0)
+
+
+def test_separable_unaligned_rows_same_order():
+    # an odd row offset selects the unaligned load path; the reduction order
+    # (and so every bit of the result) must not change
+    x = inputs(P.RASTRIGIN, 1000, 33, 5)
+    d = dev(np.concatenate([np.zeros(1), x.reshape(-1)]))[1:].view(33, 1000)
+    out = torch.empty((33, 1), dtype=torch.float64, device=DEV)
+    P.bfe_eval(P.RASTRIGIN, 1000, d, out)
+    assert np.array_equal(out.cpu().numpy(), gpu_bfe(P.RASTRIGIN, 1000, x))
+
+
+@pytest.mark.parametrize("atoms,n", [(3, 5), (4, 33), (5, 17), (9, 100), (13, 7), (48, 300), (100, 20), (169, 9)])
+def test_lennard_jones(atoms, n):
+    x = inputs(P.LENNARD_JONES, atoms, n, atoms, lo=-3.0, hi=3.0)
+    g = gpu_bfe(P.LENNARD_JONES, atoms, x)[:, 0]
+    r = O.bfe(P.LENNARD_JONES, atoms, x, threads=8)[:, 0]
+    for i in range(n):
+        assert abs(g[i] - r[i]) <= 1e-12 * lj_abs_scale(x[i]), (i, g[i], r[i])
+
+
+def test_lennard_jones_reference_kats(kats):
+    for k in kats["fitness"]:
+        if k["kind"] == "lennard_jones":
+            g = gpu_bfe(P.LENNARD_JONES, 3, np.array([k["x"]]))[0, 0]
+            assert abs(g - k["f"][0]) <= 1e-13 * abs(k["f"][0])
+
+
+def test_lennard_jones_coincident_atoms():
+    # lennard_jones.cpp:81-83,89: dist == 0 overwrites with DBL_MAX -> +inf
+    x = inputs(P.LENNARD_JONES, 6, 4, 1, lo=-3.0, hi=3.0)
+    x[0, 3:6] = 0.0               # atom 3 on atom 0
+    x[1, 0] = 0.0                 # atom 1 on atom 0
+    x[2, 3:6] = x[2, 6:9]         # atoms 3 and 4 coincide
+    g = gpu_bfe(P.LENNARD_JONES, 6, x)[:, 0]
+    r = O.bfe(P.LENNARD_JONES, 6, x)[:, 0]
+    assert g[0] == math.inf and r[0] == math.inf
+    assert g[1] == math.inf and r[1] == math.inf
+    assert g[2] == math.inf and r[2] == math.inf
+    assert np.isfinite(g[3])
+
+
+@pytest.mark.parametrize("kind", [P.RASTRIGIN, P.ACKLEY, P.ROSENBROCK])
+def test_reference_kats_on_device(kats, kind):
+    name = {P.RASTRIGIN: "rastrigin", P.ACKLEY: "ackley", P.ROSENBROCK: "rosenbrock"}[kind]
+    for k in kats["fitness"]:
+        if k["kind"] != name:
+            continue
+        x = np.array([k["x"]])
+        dim = x.shape[1]
+        g = gpu_bfe(kind, dim, x)[0, 0]
+        if k.get("exact"):
+            assert g == k["f"][0], k["src"]
+        else:
+            assert abs(g - k["f"][0]) <= 1e-12 * abs(k["f"][0]), k["src"]
+
+
+def test_mo_sphere_bit_exact():
+    rng = np.random.default_rng(0)
+    c = rng.uniform(0, 1, (2, 30))
+    x = rng.uniform(0, 1, (1001, 30))
+    g = gpu_bfe(P.MO_SPHERE, 30, x, centers=c, nobj=2)
+    r = O.bfe(P.MO_SPHERE, 30, x, centers=c, nobj=2)
+    assert np.array_equal(g, r)
+
+
+def test_host_entry_point_matches_device():
+    x = inputs(P.ACKLEY, 100, 50, 3)
+    h = P.bfe_eval_host(P.ACKLEY, 100, x)
+    assert np.array_equal(h, gpu_bfe(P.ACKLEY, 100, x))
+
+
+# ---- gradients ---------------------------------------------------------------
+
+@pytest.mark.parametrize("dim,n", [(2, 3), (10, 64), (501, 17)])
+def test_rosenbrock_gradient_bit_exact(dim, n):
+    x = inputs(P.ROSENBROCK, dim, n, dim)
+    g = torch.empty((n, dim), dtype=torch.float64, device=DEV)
+    f = torch.empty((n, 1), dtype=torch.float64, device=DEV)
+    P.batch_gradient(P.ROSENBROCK, dim, dev(x), g, f)
+    torch.cuda.synchronize()
+    assert np.array_equal(g.cpu().numpy(), O.gradient(P.ROSENBROCK, dim, x))
+    assert np.array_equal(f.cpu().numpy(), O.bfe(P.ROSENBROCK, dim, x))
+
+
+def test_rastrigin_gradient():
+    x = inputs(P.RASTRIGIN, 300, 40, 2)
+    g = torch.empty((40, 300), dtype=torch.float64, device=DEV)
+    P.batch_gradient(P.RASTRIGIN, 300, dev(x), g)
+    torch.cuda.synchronize()
+    ref = O.gradient(P.RASTRIGIN, 300, x)
+    scale = np.abs(2 * x) + 10 * 2 * np.pi
+    assert np.all(np.abs(g.cpu().numpy() - ref) <= 1e-13 * scale)
+
+
+@pytest.mark.parametrize("atoms,n", [(3, 4), (7, 20), (48, 64), (120, 5)])
+def test_lennard_jones_gradient(atoms, n):
+    x = inputs(P.LENNARD_JONES, atoms, n, 11 + atoms, lo=-3.0, hi=3.0)
+    nx = x.shape[1]
+    g = torch.empty((n, nx), dtype=torch.float64, device=DEV)
+    f = torch.empty((n, 1), dtype=torch.float64, device=DEV)
+    P.batch_gradient(P.LENNARD_JONES, atoms, dev(x), g, f)
+    torch.cuda.synchronize()
+    g, f = g.cpu().numpy(), f.cpu().numpy()[:, 0]
+    ref = O.gradient(P.LENNARD_JONES, atoms, x)
+    rf = O.bfe(P.LENNARD_JONES, atoms, x)[:, 0]
+    for i in range(n):
+        assert np.linalg.norm(g[i] - ref[i]) <= 1e-12 * np.linalg.norm(ref[i])
+        assert abs(f[i] - rf[i]) <= 1e-12 * lj_abs_scale(x[i])
+
+
+def test_lennard_jones_gradient_finite_differences():
+    # central differences, h = 1e-6, and estimate_gradient_h
+    # (gradients_and_hessians.hpp:199-244) at dx = 1e-4; <= 1e-7 relative
+    atoms, n = 48, 4
+    x = inputs(P.LENNARD_JONES, atoms, n, 99, lo=-3.0, hi=3.0)
+    g = P.batch_gradient_host(P.LENNARD_JONES, atoms, x)
+    for i in range(n):
+        fd = np.empty(x.shape[1])
+        for k in range(x.shape[1]):
+            xp, xm = x[i].copy(), x[i].copy()
+            xp[k] += 1e-6
+            xm[k] -= 1e-6
+            fd[k] = (O.fitness_one(P.LENNARD_JONES, atoms, xp)[0] - O.fitness_one(P.LENNARD_JONES, atoms, xm)[0]) / 2e-6
+        assert np.linalg.norm(fd - g[i]) <= 1e-7 * np.linalg.norm(g[i])
+        gh = O.estimate_gradient(P.LENNARD_JONES, atoms, x[i], 1e-4, True)
+        assert np.linalg.norm(gh - g[i]) <= 1e-7 * np.linalg.norm(g[i])
+
+
+# ---- RNG ----------------------------------------------------------------------
+
+@pytest.mark.parametrize("seed,skip,n", [(5489, 0, 10000), (42, 0, 1), (42, 1000, 5000), (7, 623, 700)])
+def test_mt19937_bit_exact(seed, skip, n):
+    out = torch.empty(n, dtype=torch.int32, device=DEV)
+    P.mt19937_generate(seed, n, out, skip=skip)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), O.mt19937_u32(seed, n, skip=skip))
+
+
+@pytest.mark.parametrize("kind,dim,n", [(P.ROSENBROCK, 10, 64), (P.RASTRIGIN, 1000, 33), (P.LENNARD_JONES, 48, 20)])
+def test_batch_random_dv_bit_exact(kind, dim, n):
+    nx, _ = P.dims(kind, dim)
+    out = torch.empty((n, nx), dtype=torch.float64, device=DEV)
+    P.batch_random_dv(kind, dim, n, 42, out)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), O.batch_random_dv(kind, dim, n, 42))
