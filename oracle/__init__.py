@@ -62,7 +62,7 @@ def _declare(L):
         "orc_batch_random_dv": [cint, u64, u64, u32, _dp],
         "orc_batch_uniform": [_dp, _dp, u64, u64, u32, _dp],
         "orc_pso_gen_evolve": [cint, u64, u64, _dp, _dp, u32, dbl, dbl, dbl, dbl, u32, u32, u32, cint, u32,
-                               cint, u32, C.c_void_p, C.c_void_p],
+                               cint, u32, u32, C.c_void_p, C.c_void_p],
         "orc_pso_gen_timed": [cint, u64, u64, _dp, _dp, u32, u32, u32, u32, C.POINTER(dbl)],
     }
     for name, args in sigs.items():
@@ -235,14 +235,15 @@ def batch_uniform(lb, ub, n: int, seed: int) -> np.ndarray:
 
 def pso_gen_evolve(kind: int, dim: int, X: np.ndarray, F: np.ndarray, gen: int, seed: int, omega=0.7298, eta1=2.05,
                    eta2=2.05, max_vel=0.5, variant=5, neighb_type=2, neighb_param=4, memory=False, use_bfe=True,
-                   threads=1, trace=False):
+                   threads=1, trace=False, calls=1):
     X = np.array(X, np.float64, order="C")
     F = np.array(F, np.float64, order="C").reshape(-1)
     pop, nx = X.shape
-    tX = np.empty((gen, pop, nx)) if trace else None
-    tF = np.empty((gen, pop)) if trace else None
+    tX = np.empty((calls * gen, pop, nx)) if trace else None
+    tF = np.empty((calls * gen, pop)) if trace else None
     _chk(lib().orc_pso_gen_evolve(kind, dim, pop, X, F, gen, omega, eta1, eta2, max_vel, variant, neighb_type,
-                                  neighb_param, int(memory), seed, int(use_bfe), threads, _cptr(tX), _cptr(tF)))
+                                  neighb_param, int(memory), seed, int(use_bfe), threads, calls, _cptr(tX),
+                                  _cptr(tF)))
     return (X, F, tX, tF) if trace else (X, F)
 
 

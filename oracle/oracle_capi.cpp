@@ -294,11 +294,12 @@ int orc_batch_uniform(const double *lb, const double *ub, uint64_t nx, uint64_t 
 // pso_gen(gen, omega, eta1, eta2, max_vel, variant, neighb_type, neighb_param,
 // memory, seed).evolve(pop) with X/F in and out.  use_bfe selects the bfe
 // branch (thread_bfe on n_threads) versus per-individual fitness calls.
-// trace_X (gen x pop x nx) and trace_F (gen x pop) may be null.
+// trace_X (calls x gen x pop x nx) and trace_F (calls x gen x pop) may be
+// null.  `calls` successive evolve calls share the algorithm (and its engine).
 int orc_pso_gen_evolve(int kind, uint64_t dim, uint64_t pop, double *X, double *F, uint32_t gen, double omega,
                        double eta1, double eta2, double max_vel, uint32_t variant, uint32_t neighb_type,
                        uint32_t neighb_param, int memory, uint32_t seed, int use_bfe, uint32_t n_threads,
-                       double *trace_X, double *trace_F)
+                       uint32_t calls, double *trace_X, double *trace_F)
 {
     return guarded([&] {
         auto p = make_udp(kind, dim, nullptr, 1);
@@ -320,7 +321,9 @@ int orc_pso_gen_evolve(int kind, uint64_t dim, uint64_t pop, double *X, double *
             return oracle::bfe_call(q, d, n_threads);
         };
         std::vector<double> tx, tf;
-        algo.evolve(p, x, f, use_bfe ? &bfe : nullptr, trace_X ? &tx : nullptr, trace_F ? &tf : nullptr);
+        for (uint32_t c = 0; c < calls; ++c) {
+            algo.evolve(p, x, f, use_bfe ? &bfe : nullptr, trace_X ? &tx : nullptr, trace_F ? &tf : nullptr);
+        }
         std::memcpy(X, x.data(), x.size() * sizeof(double));
         std::memcpy(F, f.data(), f.size() * sizeof(double));
         if (trace_X) std::memcpy(trace_X, tx.data(), tx.size() * sizeof(double));

@@ -1,0 +1,622 @@
+// pso.hip -- pso_gen (src/algorithms/pso_gen.cpp:120-530) for an archipelago
+// of independent islands of uneven size, on the device.
+//
+// Per island the device holds, contiguously: X, V, lbX (pop x nx), fit, lbfit
+// (pop), the island's std::mt19937 state and a word buffer for one generation
+// (pop x nx x 4 words: r1 and r2 of pso_gen.cpp:300-301 in draw order).
+// A generation is three launches:
+//   k_rng     one workgroup per island hands out the generation's words,
+//             exactly the draws of the reference's sequential loop
+//   k_update  one lane per (island, particle, dimension): best neighbour
+//             (pso_gen.cpp:593-622), velocity (variant 5, :302-305), clamp and
+//             position (:336-361) -- arithmetic in the reference's order
+//   k_eval    one workgroup per tile of individuals: fitness with the same
+//             device routine as hip_bfe, then the memory update
+//             (fit <= lbfit, pso_gen.cpp:443-449) and the optional trace
+// Batched mode launches each phase once for all islands and captures the
+// whole generation loop in a hipGraph; per-island mode launches the same
+// kernels island by island.  Both run identical arithmetic per island.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "fitness_dev.hpp"
+#include "mt_dev.hpp"
+#include "pgh_internal.hpp"
+
+namespace pgh
+{
+using namespace dev;
+
+struct island_desc {
+    int32_t kind;
+    uint32_t dim, nx, pop;
+    uint32_t gens, radius, lanes, pad;
+    double omega, eta1, eta2, max_vel;
+    uint64_t xoff; // X, V, lbX offset (doubles)
+    uint64_t foff; // fit, lbfit, offset (doubles)
+    uint64_t woff; // word buffer offset (u32)
+    uint64_t toff; // trace offset (doubles), gens x pop
+    uint64_t eoff; // cumulative pop*nx of the islands before this one
+};
+
+struct tile {
+    uint32_t island, first, count, pad;
+};
+
+namespace
+{
+
+constexpr uint32_t kBlock = 256;
+
+// "a <= b" in the sense of less_than_f(a, b) || equal_to_f(a, b)
+// (custom_comparisons.hpp:54-100): NaN is larger than everything but itself.
+__device__ __forceinline__ bool le_f(double a, double b)
+{
+    const bool an = a != a, bn = b != b;
+    if (an) return bn;
+    if (bn) return true;
+    return a <= b;
+}
+
+__device__ __forceinline__ uint32_t find_island(const island_desc *__restrict__ d, uint32_t i0, uint32_t i1,
+                                                uint64_t e)
+{
+    // last island whose eoff <= e (islands [i0, i1) are contiguous in eoff)
+    uint32_t lo = i0, hi = i1 - 1u;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (d[mid].eoff <= e) lo = mid;
+        else hi = mid - 1u;
+    }
+    return lo;
+}
+
+// phase 0: the velocity draws of the evolve call (2 words per element,
+// pso_gen.cpp:193-200); phase 1: a generation (4 words per element).
+__global__ __launch_bounds__(kBlock) void k_rng(const island_desc *__restrict__ d, mt_state *__restrict__ st,
+                                                uint32_t *__restrict__ words, uint32_t i0, uint32_t phase, uint32_t gen)
+{
+    __shared__ uint32_t ring[kMtRing];
+    const uint32_t k = i0 + blockIdx.x;
+    const island_desc di = d[k];
+    if (phase == 1u && gen >= di.gens) return;
+    const uint64_t count = static_cast<uint64_t>(di.pop) * di.nx * (phase == 0u ? 2u : 4u);
+    mt_generate(&st[k], ring, words + di.woff, count);
+}
+
+// Start of an evolve call (pso_gen.cpp:183-200): V = U(minv, maxv) per
+// element, and X = lbX = the population.  The population is X/fit before the
+// first call and lbX/lbfit after one (pop.set_xf(i, lbX, lbfit)).
+__global__ __launch_bounds__(kBlock) void k_vinit(const island_desc *__restrict__ d, uint32_t i0, uint32_t i1,
+                                                  uint64_t e0, uint64_t total, const uint32_t *__restrict__ words,
+                                                  double *__restrict__ X, double *__restrict__ V,
+                                                  double *__restrict__ lbX, double *__restrict__ fit,
+                                                  double *__restrict__ lbfit, uint32_t draw_v, uint32_t from_lb)
+{
+    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= total) return;
+    const uint64_t e = e0 + t;
+    const uint32_t k = find_island(d, i0, i1, e);
+    const island_desc di = d[k];
+    const uint64_t m = e - di.eoff;
+    const uint32_t j = static_cast<uint32_t>(m % di.nx);
+    if (draw_v) {
+        double lb, ub;
+        bounds_of(di.kind, j, lb, ub);
+        const double vwidth = (ub - lb) * di.max_vel; // pso_gen.cpp:177-181
+        const double minv = -1. * vwidth, maxv = vwidth;
+        const uint32_t *w = words + di.woff + 2u * m;
+        V[di.xoff + m] = mt_uniform(w[0], w[1], minv, maxv);
+    }
+    const uint64_t p = m / di.nx;
+    if (from_lb) {
+        X[di.xoff + m] = lbX[di.xoff + m];
+        if (j == 0u) fit[di.foff + p] = lbfit[di.foff + p];
+    } else {
+        lbX[di.xoff + m] = X[di.xoff + m];
+        if (j == 0u) lbfit[di.foff + p] = fit[di.foff + p];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_update(const island_desc *__restrict__ d, uint32_t i0, uint32_t i1,
+                                                   uint64_t e0, uint64_t total, const uint32_t *__restrict__ words,
+                                                   double *__restrict__ X, double *__restrict__ V,
+                                                   const double *__restrict__ lbX, const double *__restrict__ lbfit,
+                                                   uint32_t gen)
+{
+    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= total) return;
+    const uint64_t e = e0 + t;
+    const uint32_t k = find_island(d, i0, i1, e);
+    const island_desc di = d[k];
+    if (gen >= di.gens) return;
+    const uint64_t m = e - di.eoff;
+    const uint32_t p = static_cast<uint32_t>(m / di.nx);
+    const uint32_t dd = static_cast<uint32_t>(m - static_cast<uint64_t>(p) * di.nx);
+    const uint32_t S = di.pop, R = di.radius;
+    const double *lbf = lbfit + di.foff;
+    // lbest ring (pso_gen.cpp:679-699): radius..1 to the left, then 1..radius
+    // to the right; the later neighbour wins ties (:615-617)
+    uint32_t bn = p < R ? p - R + S : p - R;
+    for (uint32_t j = R - 1u; j >= 1u; --j) {
+        const uint32_t q = p < j ? p - j + S : p - j;
+        if (le_f(lbf[q], lbf[bn])) bn = q;
+    }
+    for (uint32_t j = 1u; j <= R; ++j) {
+        const uint32_t q = p + j >= S ? p + j - S : p + j;
+        if (le_f(lbf[q], lbf[bn])) bn = q;
+    }
+    const uint32_t *w = words + di.woff + 4u * m;
+    const double r1 = mt_canonical(w[0], w[1]);
+    const double r2 = mt_canonical(w[2], w[3]);
+    const uint64_t xi = di.xoff + m;
+    const double x = X[xi];
+    const double best = lbX[di.xoff + static_cast<uint64_t>(bn) * di.nx + dd];
+    double v = di.omega * (V[xi] + di.eta1 * r1 * (lbX[xi] - x) + di.eta2 * r2 * (best - x));
+    double lb, ub;
+    bounds_of(di.kind, dd, lb, ub);
+    const double vwidth = (ub - lb) * di.max_vel;
+    const double minv = -1. * vwidth, maxv = vwidth;
+    if (v > maxv) {
+        v = maxv;
+    } else if (v < minv) {
+        v = minv;
+    }
+    double nxv = x + v;
+    if (nxv < lb) {
+        nxv = lb;
+        v = 0.;
+    } else if (nxv > ub) {
+        nxv = ub;
+        v = 0.;
+    }
+    X[xi] = nxv;
+    V[xi] = v;
+}
+
+struct eval_ctx {
+    double *X, *lbX, *fit, *lbfit, *trace;
+    uint32_t gen;
+};
+
+// Memory update of individual p by the L lanes of its group (lane r copies
+// elements r, r+L, ...).
+__device__ __forceinline__ void remember(const island_desc &di, const eval_ctx &c, uint32_t p, double f, uint32_t r,
+                                         uint32_t L)
+{
+    const uint64_t row = di.xoff + static_cast<uint64_t>(p) * di.nx;
+    if (le_f(f, c.lbfit[di.foff + p])) {
+        for (uint32_t i = r; i < di.nx; i += L) c.lbX[row + i] = c.X[row + i];
+        if (r == 0u) c.lbfit[di.foff + p] = f;
+    }
+    if (r == 0u) {
+        c.fit[di.foff + p] = f;
+        if (c.trace) c.trace[di.toff + static_cast<uint64_t>(c.gen) * di.pop + p] = f;
+    }
+}
+
+template <int KIND, int L>
+__device__ __forceinline__ void eval_separable(const island_desc &di, const tile &tl, const eval_ctx &c)
+{
+    const uint32_t g = threadIdx.x / L, r = threadIdx.x % L;
+    if (g * L >= kBlock) return;
+    const uint32_t gg = g < tl.count ? g : tl.count - 1u;
+    const uint32_t p = tl.first + gg;
+    const double *x = c.X + di.xoff + static_cast<uint64_t>(p) * di.nx;
+    const double f = (di.nx % 2u == 0u) ? separable_eval<KIND, L, true>(x, di.nx, static_cast<int>(r))
+                                        : separable_eval<KIND, L, false>(x, di.nx, static_cast<int>(r));
+    if (g < tl.count) remember(di, c, p, f, r, L);
+}
+
+template <int KIND>
+__device__ __forceinline__ void eval_sep_dispatch(const island_desc &di, const tile &tl, const eval_ctx &c)
+{
+    switch (di.lanes) {
+        case 64:
+            eval_separable<KIND, 64>(di, tl, c);
+            return;
+        case 16:
+            eval_separable<KIND, 16>(di, tl, c);
+            return;
+        default:
+            eval_separable<KIND, 4>(di, tl, c);
+    }
+}
+
+template <int L>
+__device__ __forceinline__ void eval_lj(const island_desc &di, const tile &tl, const eval_ctx &c, double *lds)
+{
+    const uint32_t atoms = di.dim;
+    lj_stage(c.X + di.xoff + static_cast<uint64_t>(tl.first) * di.nx, tl.count, atoms, lds);
+    __syncthreads();
+    const uint32_t g = threadIdx.x / L, r = threadIdx.x % L;
+    if (g >= lj_group_count(atoms)) return;
+    const uint32_t gg = g < tl.count ? g : tl.count - 1u;
+    const double f = lj_energy<L>(lds + gg * 3u * atoms, atoms, static_cast<int>(r));
+    if (g < tl.count) remember(di, c, tl.first + gg, f, r, L);
+}
+
+__global__ __launch_bounds__(kBlock) void k_eval(const island_desc *__restrict__ d, const tile *__restrict__ tiles,
+                                                 uint32_t t0, eval_ctx c)
+{
+    extern __shared__ double lds[];
+    const tile tl = tiles[t0 + blockIdx.x];
+    const island_desc di = d[tl.island];
+    if (c.gen >= di.gens) return;
+    switch (di.kind) {
+        case ROSENBROCK:
+            if (threadIdx.x < tl.count) {
+                const uint32_t p = tl.first + threadIdx.x;
+                const double f = rosenbrock_eval(c.X + di.xoff + static_cast<uint64_t>(p) * di.nx, di.nx);
+                remember(di, c, p, f, 0u, 1u);
+            }
+            return;
+        case RASTRIGIN:
+            eval_sep_dispatch<RASTRIGIN>(di, tl, c);
+            return;
+        case ACKLEY:
+            eval_sep_dispatch<ACKLEY>(di, tl, c);
+            return;
+        case SCHWEFEL:
+            eval_sep_dispatch<SCHWEFEL>(di, tl, c);
+            return;
+        case GRIEWANK:
+            eval_sep_dispatch<GRIEWANK>(di, tl, c);
+            return;
+        case LENNARD_JONES:
+            switch (di.lanes) {
+                case 16:
+                    eval_lj<16>(di, tl, c, lds);
+                    return;
+                case 8:
+                    eval_lj<8>(di, tl, c, lds);
+                    return;
+                case 4:
+                    eval_lj<4>(di, tl, c, lds);
+                    return;
+                case 2:
+                    eval_lj<2>(di, tl, c, lds);
+                    return;
+                default:
+                    eval_lj<1>(di, tl, c, lds);
+                    return;
+            }
+    }
+}
+
+#define PGH_TRY(expr)                                                                                                \
+    do {                                                                                                             \
+        const pgh_status pgh_st_ = (expr);                                                                           \
+        if (pgh_st_ != PGH_OK) return pgh_st_;                                                                       \
+    } while (0)
+
+uint64_t align_up(uint64_t v, uint64_t a)
+{
+    return (v + a - 1u) / a * a;
+}
+
+} // namespace
+} // namespace pgh
+
+using namespace pgh;
+
+struct pgh_archi {
+    std::vector<island_desc> h_desc;
+    std::vector<tile> h_tiles;
+    std::vector<uint32_t> tile_begin; // per island, into h_tiles (+ end sentinel)
+    uint64_t total_elems = 0, total_words = 0, total_trace = 0;
+    uint32_t max_gens = 0;
+    size_t lds_bytes = 0;
+    bool trace = false;
+    bool v_valid = false;
+    bool evolved = false; // the population lives in lbX/lbfit once evolved
+    std::vector<uint32_t> memory;
+    dev_buf desc, tiles, X, V, lbX, fit, lbfit, words, states, trace_buf;
+    hipStream_t own = nullptr;
+    hipStream_t last = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool graph_draw_v = false, graph_from_lb = false;
+
+    ~pgh_archi()
+    {
+        if (last) (void)hipStreamSynchronize(last);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (own) (void)hipStreamDestroy(own);
+    }
+};
+
+namespace
+{
+
+eval_ctx make_ctx(pgh_archi *a, uint32_t gen)
+{
+    eval_ctx c;
+    c.X = static_cast<double *>(a->X.ptr);
+    c.lbX = static_cast<double *>(a->lbX.ptr);
+    c.fit = static_cast<double *>(a->fit.ptr);
+    c.lbfit = static_cast<double *>(a->lbfit.ptr);
+    c.trace = a->trace ? static_cast<double *>(a->trace_buf.ptr) : nullptr;
+    c.gen = gen;
+    return c;
+}
+
+uint32_t blocks_for(uint64_t count)
+{
+    return static_cast<uint32_t>((count + kBlock - 1u) / kBlock);
+}
+
+// Launches the evolve call for islands [i0, i1) on stream s.
+pgh_status launch_range(pgh_archi *a, uint32_t i0, uint32_t i1, bool draw_v, bool from_lb, hipStream_t s)
+{
+    const auto *d = static_cast<const island_desc *>(a->desc.ptr);
+    auto *st = static_cast<mt_state *>(a->states.ptr);
+    auto *w = static_cast<uint32_t *>(a->words.ptr);
+    auto *X = static_cast<double *>(a->X.ptr);
+    auto *V = static_cast<double *>(a->V.ptr);
+    auto *lbX = static_cast<double *>(a->lbX.ptr);
+    auto *fit = static_cast<double *>(a->fit.ptr);
+    auto *lbfit = static_cast<double *>(a->lbfit.ptr);
+    const uint64_t e0 = a->h_desc[i0].eoff;
+    const uint64_t e1 = i1 < a->h_desc.size() ? a->h_desc[i1].eoff : a->total_elems;
+    const uint64_t total = e1 - e0;
+    const uint32_t t0 = a->tile_begin[i0], nt = a->tile_begin[i1] - t0;
+    const uint32_t nisl = i1 - i0;
+    uint32_t gens = 0;
+    for (uint32_t k = i0; k < i1; ++k) gens = std::max(gens, a->h_desc[k].gens);
+    if (draw_v) {
+        k_rng<<<nisl, kBlock, 0, s>>>(d, st, w, i0, 0u, 0u);
+        PGH_HIP(hipGetLastError());
+    }
+    k_vinit<<<blocks_for(total), kBlock, 0, s>>>(d, i0, i1, e0, total, w, X, V, lbX, fit, lbfit, draw_v ? 1u : 0u,
+                                                 from_lb ? 1u : 0u);
+    PGH_HIP(hipGetLastError());
+    for (uint32_t g = 0; g < gens; ++g) {
+        k_rng<<<nisl, kBlock, 0, s>>>(d, st, w, i0, 1u, g);
+        k_update<<<blocks_for(total), kBlock, 0, s>>>(d, i0, i1, e0, total, w, X, V, lbX, lbfit, g);
+        k_eval<<<nt, kBlock, a->lds_bytes, s>>>(d, static_cast<const tile *>(a->tiles.ptr), t0, make_ctx(a, g));
+        PGH_HIP(hipGetLastError());
+    }
+    return PGH_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+pgh_status pgh_archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_t n_islands,
+                            const double *const *X0, const double *const *F0)
+{
+    if (!out || !islands || !X0 || !F0) return fail(PGH_INVALID_ARGUMENT, "null argument");
+    if (n_islands == 0u) return fail(PGH_INVALID_ARGUMENT, "an archipelago needs at least one island");
+    auto a = std::make_unique<pgh_archi>();
+    uint64_t xoff = 0, foff = 0, woff = 0, toff = 0, eoff = 0;
+    a->tile_begin.push_back(0);
+    for (uint64_t k = 0; k < n_islands; ++k) {
+        const pgh_island_spec &sp = islands[k];
+        prob p;
+        PGH_TRY(resolve(&sp.udp, p));
+        if (p.kind == PGH_MO_SPHERE) return fail(PGH_UNSUPPORTED, "pso_gen needs a single-objective problem");
+        const pgh_pso_params &q = sp.algo;
+        // pso_gen.cpp:83-112 parameter checks
+        if (q.omega < 0. || q.omega > 1.) {
+            return fail(PGH_INVALID_ARGUMENT, "The particles' inertia (or the constriction factor) must be in the "
+                                              "[0,1] range");
+        }
+        if (q.eta1 < 0. || q.eta2 < 0. || q.eta1 > 4. || q.eta2 > 4.) {
+            return fail(PGH_INVALID_ARGUMENT, "The eta parameters must be in the [0,4] range");
+        }
+        if (q.max_vel <= 0. || q.max_vel > 1.) {
+            return fail(PGH_INVALID_ARGUMENT, "The maximum particle velocity (as a fraction of the bounds) should be "
+                                              "in the (0,1] range");
+        }
+        if (q.variant < 1u || q.variant > 6u) {
+            return fail(PGH_INVALID_ARGUMENT, "The PSO variant must be in [1,6], while a value of "
+                                                  + std::to_string(q.variant) + " was detected");
+        }
+        if (q.neighb_type < 1u || q.neighb_type > 4u) {
+            return fail(PGH_INVALID_ARGUMENT, "The swarm topology variant must be in [1,4], while a value of "
+                                                  + std::to_string(q.neighb_type) + " was detected");
+        }
+        if (q.neighb_param < 1u) {
+            return fail(PGH_INVALID_ARGUMENT, "The neighborhood parameter must be in (0, inf), while a value of "
+                                                  + std::to_string(q.neighb_param) + " was detected");
+        }
+        if (q.variant != 5u || q.neighb_type != 2u) {
+            return fail(PGH_UNSUPPORTED, "the device pso_gen implements variant 5 with the lbest topology");
+        }
+        if (sp.pop == 0u) return fail(PGH_INVALID_ARGUMENT, "pso_gen does not work on an empty population");
+        const uint32_t radius = q.neighb_param / 2u;
+        if (radius == 0u || sp.pop < radius || sp.pop > (uint64_t(1) << 24)) {
+            return fail(PGH_UNSUPPORTED, "population size incompatible with the lbest radius");
+        }
+        island_desc di{};
+        di.kind = p.kind;
+        di.dim = p.dim;
+        di.nx = p.nx;
+        di.pop = static_cast<uint32_t>(sp.pop);
+        di.gens = q.gen;
+        di.radius = radius;
+        di.omega = q.omega;
+        di.eta1 = q.eta1;
+        di.eta2 = q.eta2;
+        di.max_vel = q.max_vel;
+        const uint64_t elems = sp.pop * p.nx;
+        di.xoff = xoff;
+        di.foff = foff;
+        di.woff = woff;
+        di.toff = toff;
+        di.eoff = eoff;
+        xoff += align_up(elems, 32u); // 256-byte aligned rows blocks
+        foff += align_up(sp.pop, 32u);
+        woff += align_up(elems * 4u, 64u);
+        toff += static_cast<uint64_t>(q.gen) * sp.pop;
+        eoff += elems;
+        a->max_gens = std::max(a->max_gens, q.gen);
+        a->memory.push_back(q.memory);
+        // evaluation tiles
+        uint32_t per = 256u;
+        if (p.kind == PGH_LENNARD_JONES) {
+            di.lanes = static_cast<uint32_t>(lj_lanes(p.dim));
+            per = lj_group_count(p.dim);
+            if (per == 0u) return fail(PGH_UNSUPPORTED, "lennard_jones cluster too large for one workgroup");
+            a->lds_bytes = std::max<size_t>(a->lds_bytes, static_cast<size_t>(per) * 3u * p.dim * sizeof(double));
+        } else if (p.kind != PGH_ROSENBROCK) {
+            di.lanes = static_cast<uint32_t>(sep_lanes(p.nx));
+            per = 256u / di.lanes;
+        } else {
+            di.lanes = 1u;
+        }
+        for (uint64_t f = 0; f < sp.pop; f += per) {
+            a->h_tiles.push_back(tile{static_cast<uint32_t>(k), static_cast<uint32_t>(f),
+                                      static_cast<uint32_t>(std::min<uint64_t>(per, sp.pop - f)), 0u});
+        }
+        a->tile_begin.push_back(static_cast<uint32_t>(a->h_tiles.size()));
+        a->h_desc.push_back(di);
+    }
+    a->total_elems = eoff;
+    a->total_words = woff;
+    a->total_trace = toff;
+    PGH_HIP(hipStreamCreateWithFlags(&a->own, hipStreamNonBlocking));
+    PGH_TRY(a->desc.reserve(a->h_desc.size() * sizeof(island_desc)));
+    PGH_TRY(a->tiles.reserve(a->h_tiles.size() * sizeof(tile)));
+    PGH_TRY(a->X.reserve(xoff * 8u));
+    PGH_TRY(a->V.reserve(xoff * 8u));
+    PGH_TRY(a->lbX.reserve(xoff * 8u));
+    PGH_TRY(a->fit.reserve(foff * 8u));
+    PGH_TRY(a->lbfit.reserve(foff * 8u));
+    PGH_TRY(a->words.reserve(std::max<uint64_t>(woff, 1u) * 4u));
+    PGH_TRY(a->states.reserve(n_islands * sizeof(mt_state)));
+    PGH_HIP(hipMemcpy(a->desc.ptr, a->h_desc.data(), a->h_desc.size() * sizeof(island_desc), hipMemcpyHostToDevice));
+    PGH_HIP(hipMemcpy(a->tiles.ptr, a->h_tiles.data(), a->h_tiles.size() * sizeof(tile), hipMemcpyHostToDevice));
+    std::vector<mt_state> states(n_islands);
+    for (uint64_t k = 0; k < n_islands; ++k) {
+        mt_seed_host(islands[k].algo.seed, states[k]);
+        const island_desc &di = a->h_desc[k];
+        PGH_HIP(hipMemcpy(static_cast<double *>(a->X.ptr) + di.xoff, X0[k],
+                          static_cast<uint64_t>(di.pop) * di.nx * 8u, hipMemcpyHostToDevice));
+        PGH_HIP(hipMemcpy(static_cast<double *>(a->fit.ptr) + di.foff, F0[k], static_cast<uint64_t>(di.pop) * 8u,
+                          hipMemcpyHostToDevice));
+    }
+    PGH_HIP(hipMemcpy(a->states.ptr, states.data(), n_islands * sizeof(mt_state), hipMemcpyHostToDevice));
+    *out = a.release();
+    return PGH_OK;
+}
+
+pgh_status pgh_archi_destroy(pgh_archi *a)
+{
+    delete a;
+    return PGH_OK;
+}
+
+pgh_status pgh_archi_set_trace(pgh_archi *a, int on)
+{
+    if (!a) return fail(PGH_INVALID_ARGUMENT, "null archipelago");
+    if (on && !a->trace_buf.ptr) PGH_TRY(a->trace_buf.reserve(std::max<uint64_t>(a->total_trace, 1u) * 8u));
+    if (a->trace != (on != 0)) {
+        // the captured graph bakes the trace pointer in
+        if (a->exec) (void)hipGraphExecDestroy(a->exec);
+        if (a->graph) (void)hipGraphDestroy(a->graph);
+        a->exec = nullptr;
+        a->graph = nullptr;
+    }
+    a->trace = on != 0;
+    return PGH_OK;
+}
+
+pgh_status pgh_archi_evolve(pgh_archi *a, int mode, void *stream)
+{
+    if (!a) return fail(PGH_INVALID_ARGUMENT, "null archipelago");
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : a->own;
+    // velocities are re-drawn unless memory is set and they exist
+    // (pso_gen.cpp:193-200); islands share the memory flag of island 0
+    const bool draw_v = !(a->memory[0] && a->v_valid);
+    const bool from_lb = a->evolved;
+    const uint32_t n = static_cast<uint32_t>(a->h_desc.size());
+    if (mode == PGH_ARCHI_PER_ISLAND) {
+        for (uint32_t k = 0; k < n; ++k) PGH_TRY(launch_range(a, k, k + 1u, draw_v, from_lb, s));
+    } else if (mode == PGH_ARCHI_BATCHED) {
+        if (!a->exec || a->graph_draw_v != draw_v || a->graph_from_lb != from_lb) {
+            if (a->exec) (void)hipGraphExecDestroy(a->exec);
+            if (a->graph) (void)hipGraphDestroy(a->graph);
+            a->exec = nullptr;
+            a->graph = nullptr;
+            PGH_HIP(hipStreamBeginCapture(a->own, hipStreamCaptureModeThreadLocal));
+            const pgh_status st = launch_range(a, 0u, n, draw_v, from_lb, a->own);
+            hipGraph_t g = nullptr;
+            const hipError_t ce = hipStreamEndCapture(a->own, &g);
+            if (st != PGH_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return st;
+            }
+            if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
+            a->graph = g;
+            PGH_HIP(hipGraphInstantiate(&a->exec, a->graph, nullptr, nullptr, 0));
+            a->graph_draw_v = draw_v;
+            a->graph_from_lb = from_lb;
+        }
+        PGH_HIP(hipGraphLaunch(a->exec, s));
+    } else {
+        return fail(PGH_INVALID_ARGUMENT, "unknown archipelago mode " + std::to_string(mode));
+    }
+    a->v_valid = true;
+    a->evolved = true;
+    a->last = s;
+    return PGH_OK;
+}
+
+pgh_status pgh_archi_get(pgh_archi *a, uint64_t k, double *X, double *F)
+{
+    if (!a || k >= a->h_desc.size()) return fail(PGH_INVALID_ARGUMENT, "no such island");
+    if (a->last) PGH_HIP(hipStreamSynchronize(a->last));
+    const island_desc &di = a->h_desc[k];
+    // pop.set_xf(i, lbX[i], lbfit[i]), pso_gen.cpp:526-528
+    const double *xs = static_cast<const double *>(a->evolved ? a->lbX.ptr : a->X.ptr);
+    const double *fs = static_cast<const double *>(a->evolved ? a->lbfit.ptr : a->fit.ptr);
+    if (X) {
+        PGH_HIP(hipMemcpy(X, xs + di.xoff, static_cast<uint64_t>(di.pop) * di.nx * 8u, hipMemcpyDeviceToHost));
+    }
+    if (F) {
+        PGH_HIP(hipMemcpy(F, fs + di.foff, static_cast<uint64_t>(di.pop) * 8u, hipMemcpyDeviceToHost));
+    }
+    return PGH_OK;
+}
+
+pgh_status pgh_archi_get_trace(pgh_archi *a, uint64_t k, double *trace_F)
+{
+    if (!a || k >= a->h_desc.size()) return fail(PGH_INVALID_ARGUMENT, "no such island");
+    if (!a->trace) return fail(PGH_INVALID_ARGUMENT, "tracing is off");
+    if (a->last) PGH_HIP(hipStreamSynchronize(a->last));
+    const island_desc &di = a->h_desc[k];
+    PGH_HIP(hipMemcpy(trace_F, static_cast<double *>(a->trace_buf.ptr) + di.toff,
+                      static_cast<uint64_t>(di.gens) * di.pop * 8u, hipMemcpyDeviceToHost));
+    return PGH_OK;
+}
+
+pgh_status pgh_pso_gen_evolve_host(const pgh_udp *udp, const pgh_pso_params *prm, double *X, double *F, uint64_t pop,
+                                   uint64_t nx)
+{
+    if (!udp || !prm || !X || !F) return fail(PGH_INVALID_ARGUMENT, "null argument");
+    prob p;
+    PGH_TRY(resolve(udp, p));
+    if (nx != p.nx) return fail(PGH_INVALID_ARGUMENT, "population dimension does not match the problem");
+    pgh_island_spec sp{};
+    sp.udp = *udp;
+    sp.pop = pop;
+    sp.algo = *prm;
+    pgh_archi *a = nullptr;
+    const double *xs[1] = {X};
+    const double *fs[1] = {F};
+    PGH_TRY(pgh_archi_create(&a, &sp, 1u, xs, fs));
+    std::unique_ptr<pgh_archi> guard(a);
+    PGH_TRY(pgh_archi_evolve(a, PGH_ARCHI_BATCHED, nullptr));
+    return pgh_archi_get(a, 0u, X, F);
+}
+
+} // extern "C"

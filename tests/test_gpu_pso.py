@@ -1,0 +1,110 @@
+"""pso_gen on the device (single island and uneven archipelagos).
+
+Rosenbrock islands reproduce the oracle's trajectory bit for bit (its fitness
+is bit-exact, so every comparison of pso_gen.cpp:443-449 and :615-617 goes the
+same way).  Batched and per-island launches are bit-identical on every island
+and every problem.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import pagmo2_amd as P
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def test_plain_method_rosenbrock_bit_exact():
+    """Rosenbrock n=10, population 64 from seed 42, pso_gen(100 generations,
+    seed 42) with default parameters: every generation's evaluated fitness and
+    the final population equal the oracle's."""
+    X0, F0 = O.population_init(P.ROSENBROCK, 10, 64, 42)
+    Xo, Fo, tX, tF = O.pso_gen_evolve(P.ROSENBROCK, 10, X0, F0, 100, 42, trace=True)
+    a = P.Archipelago([dict(kind=P.ROSENBROCK, dim=10, X0=X0, F0=F0, algo=P.pso_params(gen=100, seed=42))],
+                      trace=True)
+    a.evolve()
+    Xg, Fg = a.get(0)
+    assert np.array_equal(a.trace(0), tF)
+    assert np.array_equal(Xg, Xo)
+    assert np.array_equal(Fg, Fo)
+    a.close()
+
+
+def test_host_entry_point():
+    X0, F0 = O.population_init(P.ROSENBROCK, 5, 20, 3)
+    Xo, Fo = O.pso_gen_evolve(P.ROSENBROCK, 5, X0, F0, 25, 9)
+    Xg, Fg = P.pso_gen_evolve_host(P.ROSENBROCK, 5, X0, F0, P.pso_params(gen=25, seed=9))
+    assert np.array_equal(Xg, Xo) and np.array_equal(Fg, Fo)
+
+
+def test_two_evolve_calls_continue_the_stream():
+    """A second evolve call re-draws velocities from the same engine and
+    starts from the evolved population (pso_gen.cpp:193-200, :526-528)."""
+    X0, F0 = O.population_init(P.ROSENBROCK, 6, 30, 5)
+    a = P.Archipelago([dict(kind=P.ROSENBROCK, dim=6, X0=X0, F0=F0, algo=P.pso_params(gen=7, seed=11))])
+    a.evolve()
+    a.evolve(P.ARCHI_PER_ISLAND)
+    Xg, Fg = a.get(0)
+    a.close()
+    # the oracle's pso_gen keeps one engine across calls
+    Xo, Fo = O.pso_gen_evolve(P.ROSENBROCK, 6, X0, F0, 7, 11, calls=2)
+    assert np.array_equal(Xg, Xo) and np.array_equal(Fg, Fo)
+
+
+def uneven_islands(n_islands, seed, max_pop=300, max_dim=60, gens=8):
+    rng = np.random.default_rng(seed)
+    kinds = [P.ROSENBROCK, P.RASTRIGIN, P.LENNARD_JONES]
+    out = []
+    for k in range(n_islands):
+        kind = kinds[k % 3]
+        pop = int(round(np.exp(rng.uniform(np.log(8), np.log(max_pop)))))
+        dim = int(round(np.exp(rng.uniform(np.log(10), np.log(max_dim)))))
+        if kind == P.LENNARD_JONES:
+            dim = max(3, round((dim + 6) / 3))  # atoms
+        X0, F0 = O.population_init(kind, dim, pop, 42 + k)
+        out.append(dict(kind=kind, dim=dim, X0=X0, F0=F0, algo=P.pso_params(gen=gens, seed=42 + k)))
+    return out
+
+
+def test_batched_equals_per_island():
+    isl = uneven_islands(9, 1)
+    a = P.Archipelago(isl, trace=True)
+    b = P.Archipelago(isl, trace=True)
+    a.evolve(P.ARCHI_BATCHED)
+    b.evolve(P.ARCHI_PER_ISLAND)
+    for k in range(len(isl)):
+        xa, fa = a.get(k)
+        xb, fb = b.get(k)
+        assert np.array_equal(xa, xb) and np.array_equal(fa, fb), k
+        assert np.array_equal(a.trace(k), b.trace(k)), k
+    a.close()
+    b.close()
+
+
+def test_islands_against_oracle():
+    """Positions depend on fitness only through comparisons, so a few
+    generations reproduce the oracle's positions exactly; fitness values agree
+    to the kernel's tolerance (bit-exact for rosenbrock)."""
+    isl = uneven_islands(6, 2, gens=5)
+    a = P.Archipelago(isl)
+    a.evolve()
+    for k, d in enumerate(isl):
+        xg, fg = a.get(k)
+        xo, fo = O.pso_gen_evolve(d["kind"], d["dim"], d["X0"], d["F0"], 5, 42 + k)
+        assert np.array_equal(xg, xo), k
+        if d["kind"] == P.ROSENBROCK:
+            assert np.array_equal(fg, fo), k
+        elif d["kind"] == P.RASTRIGIN:
+            np.testing.assert_allclose(fg, fo, rtol=1e-12, atol=0)
+        else:
+            np.testing.assert_allclose(fg, fo, rtol=1e-9, atol=1e-9)
+    a.close()
+
+
+def test_unsupported_variant():
+    X0, F0 = O.population_init(P.ROSENBROCK, 4, 10, 1)
+    with pytest.raises(P.PghError) as e:
+        P.Archipelago([dict(kind=P.ROSENBROCK, dim=4, X0=X0, F0=F0, algo=P.pso_params(variant=1))])
+    assert e.value.status == P.UNSUPPORTED
