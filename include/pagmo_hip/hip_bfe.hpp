@@ -1,0 +1,216 @@
+// hip_bfe.hpp -- the MI355X batch fitness evaluator as a pagmo UDBFE.
+//
+//     pagmo::pso_gen algo{100u};
+//     algo.set_bfe(pagmo::bfe{pagmo_hip::hip_bfe{}});
+//
+// hip_bfe is a user-defined batch fitness evaluator in the sense of
+// include/pagmo/bfe.hpp:68-108: default/copy/move constructible and callable as
+// vector_double(const problem &, const vector_double &).  It is stateless
+// (like thread_bfe, src/batch_evaluators/thread_bfe.cpp:147-150) and never
+// touches the problem's counters: the pagmo::bfe wrapper increments fevals
+// (src/bfe.cpp:102-107).
+//
+// Problems with a device kernel (rosenbrock, rastrigin, ackley, schwefel,
+// griewank, lennard_jones, found with problem::extract<T>(),
+// include/pagmo/problem.hpp:1104-1147) are evaluated by libpagmo_hip.so.  Any
+// other problem goes through default_bfe's dispatch
+// (src/batch_evaluators/default_bfe.cpp:53-68): member_bfe if it has
+// batch_fitness(), else thread_bfe if its thread safety allows it, else the
+// same std::invalid_argument.  A supported problem with no usable device is an
+// error, never a silent CPU evaluation.
+//
+// Header-only; link with -lpagmo_hip.  Besides the UDBFE the header offers the
+// batched gradient (problem::gradient, src/problem.cpp:456-467, for a whole
+// population) and fast_non_dominated_sorting
+// (src/utils/multi_objective.cpp:200-257) on the device.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include <pagmo/batch_evaluators/member_bfe.hpp>
+#include <pagmo/batch_evaluators/thread_bfe.hpp>
+#include <pagmo/exceptions.hpp>
+#include <pagmo/problem.hpp>
+#include <pagmo/problems/ackley.hpp>
+#include <pagmo/problems/griewank.hpp>
+#include <pagmo/problems/lennard_jones.hpp>
+#include <pagmo/problems/rastrigin.hpp>
+#include <pagmo/problems/rosenbrock.hpp>
+#include <pagmo/problems/schwefel.hpp>
+#include <pagmo/threading.hpp>
+#include <pagmo/types.hpp>
+#include <pagmo/utils/multi_objective.hpp>
+
+#include "pgh.h"
+
+namespace pagmo_hip
+{
+
+namespace detail
+{
+
+// Throws the exception type of the reference for a failed status.
+[[noreturn]] inline void raise(pgh_status st)
+{
+    const std::string msg = pgh_last_error();
+    switch (st) {
+        case PGH_INVALID_ARGUMENT:
+            pagmo_throw(std::invalid_argument, msg);
+        case PGH_OVERFLOW:
+            pagmo_throw(std::overflow_error, msg);
+        default:
+            pagmo_throw(std::runtime_error, "libpagmo_hip: " + msg);
+    }
+}
+
+inline void check(pgh_status st)
+{
+    if (st != PGH_OK) raise(st);
+}
+
+} // namespace detail
+
+// The device description of p, if a kernel exists for it.
+inline bool describe(const pagmo::problem &p, pgh_udp &u)
+{
+    u = pgh_udp{};
+    if (const auto *r = p.extract<pagmo::rosenbrock>()) {
+        u.kind = PGH_ROSENBROCK;
+        u.dim = r->m_dim;
+    } else if (p.extract<pagmo::rastrigin>()) {
+        u.kind = PGH_RASTRIGIN;
+        u.dim = p.get_nx();
+    } else if (p.extract<pagmo::ackley>()) {
+        u.kind = PGH_ACKLEY;
+        u.dim = p.get_nx();
+    } else if (p.extract<pagmo::schwefel>()) {
+        u.kind = PGH_SCHWEFEL;
+        u.dim = p.get_nx();
+    } else if (p.extract<pagmo::griewank>()) {
+        u.kind = PGH_GRIEWANK;
+        u.dim = p.get_nx();
+    } else if (p.extract<pagmo::lennard_jones>()) {
+        // m_atoms is private (lennard_jones.hpp:87); nx = 3*atoms - 6
+        u.kind = PGH_LENNARD_JONES;
+        u.dim = (p.get_nx() + 6u) / 3u;
+    } else {
+        return false;
+    }
+    u.nobj = 1;
+    return pgh_supported(&u) == 1;
+}
+
+class hip_bfe
+{
+public:
+    pagmo::vector_double operator()(const pagmo::problem &p, const pagmo::vector_double &dvs) const
+    {
+        pgh_udp u;
+        if (!describe(p, u)) {
+            return fallback(p, dvs);
+        }
+        const auto nx = p.get_nx(), nf = p.get_nf();
+        pagmo::vector_double out(nx ? dvs.size() / nx * nf : 0u);
+        detail::check(pgh_bfe_eval_host(&u, dvs.data(), dvs.size(), out.data(), out.size()));
+        return out;
+    }
+    std::string get_name() const
+    {
+        return "HIP batch fitness evaluator (MI355X, gfx950)";
+    }
+    std::string get_extra_info() const
+    {
+        return std::string("\tDevice available: ") + (pgh_device_available() ? "yes" : "no")
+               + "\n\tC ABI version: " + std::to_string(pgh_abi_version());
+    }
+    pagmo::thread_safety get_thread_safety() const
+    {
+        // every host thread gets its own stream and staging buffers
+        return pagmo::thread_safety::basic;
+    }
+    template <typename Archive>
+    void serialize(Archive &, unsigned)
+    {
+    }
+
+private:
+    // default_bfe's heuristic (src/batch_evaluators/default_bfe.cpp:53-68).
+    static pagmo::vector_double fallback(const pagmo::problem &p, const pagmo::vector_double &dvs)
+    {
+        if (p.has_batch_fitness()) {
+            return pagmo::member_bfe{}(p, dvs);
+        }
+        if (p.get_thread_safety() >= pagmo::thread_safety::basic) {
+            return pagmo::thread_bfe{}(p, dvs);
+        }
+        pagmo_throw(std::invalid_argument,
+                    "Cannot execute fitness evaluations in batch mode for a problem of type '" + p.get_name()
+                        + "': the problem does not implement the batch_fitness() member function, and its thread "
+                          "safety level is not sufficient to run a thread-based batch fitness evaluation "
+                          "implementation");
+    }
+};
+
+// Gradients of a whole population: n_dvs x nx (the dense pattern of
+// problem.cpp:132-140 with nf == 1), row-major.  rosenbrock and rastrigin
+// follow their UDP's gradient(); lennard_jones uses the library's adjoint
+// kernel.  Like problem::gradient for a UDP without one, other problems throw.
+inline pagmo::vector_double batch_gradient(const pagmo::problem &p, const pagmo::vector_double &dvs)
+{
+    pgh_udp u;
+    if (!describe(p, u)) {
+        pagmo_throw(std::invalid_argument, "batch_gradient: no device gradient for the problem '" + p.get_name() + "'");
+    }
+    pagmo::vector_double g(dvs.size());
+    detail::check(pgh_batch_gradient_host(&u, dvs.data(), dvs.size(), g.data(), g.size(), nullptr));
+    return g;
+}
+
+// fast_non_dominated_sorting on the device, same return value and exceptions.
+inline pagmo::fnds_return_type fast_non_dominated_sorting(const std::vector<pagmo::vector_double> &points)
+{
+    const auto n = points.size();
+    const auto m = n ? points[0].size() : 0u;
+    std::vector<double> flat;
+    flat.reserve(n * m);
+    for (const auto &v : points) {
+        if (v.size() != m) {
+            pagmo_throw(std::invalid_argument, "Different number of objectives found in input fitnesses: "
+                                                   + std::to_string(m) + " and " + std::to_string(v.size())
+                                                   + ". I cannot define dominance");
+        }
+        flat.insert(flat.end(), v.begin(), v.end());
+    }
+    pgh_fnds_ws *ws = nullptr;
+    detail::check(pgh_fnds_create(&ws));
+    struct guard {
+        pgh_fnds_ws *w;
+        ~guard()
+        {
+            pgh_fnds_destroy(w);
+        }
+    } g{ws};
+    detail::check(pgh_fnds_run_host(ws, flat.empty() ? nullptr : flat.data(), n, m));
+    uint64_t n_fronts = 0, n_dom = 0;
+    detail::check(pgh_fnds_sizes(ws, &n_fronts, &n_dom));
+    std::vector<uint64_t> dom_count(n), rank(n), fo(n_fronts + 1u), fi(n), doff(n + 1u), di(n_dom);
+    detail::check(pgh_fnds_copy(ws, dom_count.data(), rank.data(), fo.data(), fi.data(), doff.data(),
+                                di.empty() ? nullptr : di.data(), 1));
+    std::vector<std::vector<pagmo::pop_size_t>> fronts(n_fronts), dom_list(n);
+    for (uint64_t f = 0; f < n_fronts; ++f) {
+        fronts[f].assign(fi.begin() + static_cast<std::ptrdiff_t>(fo[f]), fi.begin() + static_cast<std::ptrdiff_t>(fo[f + 1u]));
+    }
+    for (uint64_t i = 0; i < n; ++i) {
+        dom_list[i].assign(di.begin() + static_cast<std::ptrdiff_t>(doff[i]),
+                           di.begin() + static_cast<std::ptrdiff_t>(doff[i + 1u]));
+    }
+    return std::make_tuple(std::move(fronts), std::move(dom_list),
+                           std::vector<pagmo::pop_size_t>(dom_count.begin(), dom_count.end()),
+                           std::vector<pagmo::pop_size_t>(rank.begin(), rank.end()));
+}
+
+} // namespace pagmo_hip

@@ -119,6 +119,9 @@ pgh_status pgh_fnds_destroy(pgh_fnds_ws *ws);
 /* INVALID_ARGUMENT when n < 2, with the reference's message
  * (multi_objective.cpp:203-207). */
 pgh_status pgh_fnds_run(pgh_fnds_ws *ws, const double *objs, uint64_t n, uint64_t m, void *stream);
+/* Host-buffer form: objs (n x m, host memory) is staged into the workspace and
+ * sorted on a stream the workspace owns. */
+pgh_status pgh_fnds_run_host(pgh_fnds_ws *ws, const double *objs, uint64_t n, uint64_t m);
 pgh_status pgh_fnds_sizes(pgh_fnds_ws *ws, uint64_t *n_fronts, uint64_t *n_dom_items);
 /* Any output pointer may be NULL.  to_host != 0: destinations are host
  * memory (synchronous); 0: device memory (async on the run's stream). */

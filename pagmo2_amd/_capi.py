@@ -23,7 +23,8 @@ ARCHI_BATCHED, ARCHI_PER_ISLAND = 0, 1
 EXPORTS = [
     "pgh_last_error", "pgh_abi_version", "pgh_device_available", "pgh_problem_dims", "pgh_supported",
     "pgh_bfe_eval", "pgh_bfe_eval_host", "pgh_batch_gradient", "pgh_batch_gradient_host",
-    "pgh_fnds_create", "pgh_fnds_destroy", "pgh_fnds_run", "pgh_fnds_sizes", "pgh_fnds_copy",
+    "pgh_fnds_create", "pgh_fnds_destroy", "pgh_fnds_run", "pgh_fnds_run_host", "pgh_fnds_sizes",
+    "pgh_fnds_copy",
     "pgh_mt19937_generate", "pgh_batch_random_dv",
     "pgh_archi_create", "pgh_archi_destroy", "pgh_archi_evolve", "pgh_archi_get", "pgh_archi_set_trace",
     "pgh_archi_get_trace", "pgh_pso_gen_evolve_host",
@@ -86,6 +87,7 @@ def _declare(L):
         "pgh_fnds_create": [C.POINTER(vp)],
         "pgh_fnds_destroy": [vp],
         "pgh_fnds_run": [vp, vp, u64, u64, vp],
+        "pgh_fnds_run_host": [vp, vp, u64, u64],
         "pgh_fnds_sizes": [vp, C.POINTER(u64), C.POINTER(u64)],
         "pgh_fnds_copy": [vp, vp, vp, vp, vp, vp, vp, ci],
         "pgh_mt19937_generate": [u32, u64, u64, vp, vp],
@@ -194,6 +196,11 @@ class Fnds:
     def run(self, objs, stream=None):
         n, m = objs.shape
         check(lib().pgh_fnds_run(self.h, objs.data_ptr(), n, m, _stream(stream)))
+
+    def run_host(self, objs: np.ndarray):
+        objs = np.ascontiguousarray(objs, np.float64)
+        n, m = objs.shape
+        check(lib().pgh_fnds_run_host(self.h, objs.ctypes.data if objs.size else None, n, m))
 
     def sizes(self):
         a, b = C.c_uint64(), C.c_uint64()

@@ -14,6 +14,8 @@
 
 #include <cstdint>
 
+#include "trig_dev.hpp"
+
 namespace pgh
 {
 namespace dev
@@ -144,11 +146,11 @@ struct sep_acc {
     {
         if constexpr (KIND == RASTRIGIN) {
             // rastrigin.cpp:67
-            a += xi * xi - 10. * cos(kOmega * xi);
+            a += xi * xi - 10. * cos_any(kOmega * xi);
         } else if constexpr (KIND == ACKLEY) {
             // ackley.cpp:69-70
             a += xi * xi;
-            b += cos(kOmega * xi);
+            b += cos_any(kOmega * xi);
         } else if constexpr (KIND == SCHWEFEL) {
             // schwefel.cpp:64
             a += xi * sin(sqrt(fabs(xi)));
@@ -163,22 +165,46 @@ struct sep_acc {
 };
 
 template <int KIND, int L, bool ALIGNED>
+__device__ __forceinline__ void separable_pair(sep_acc<KIND> &acc, const double *__restrict__ x, uint32_t nx,
+                                               uint32_t c)
+{
+    const uint32_t i = 2u * c;
+    if constexpr (ALIGNED) {
+        const double2 v = *reinterpret_cast<const double2 *>(x + i);
+        acc.add(v.x, i);
+        acc.add(v.y, i + 1u);
+    } else {
+        acc.add(x[i], i);
+        if (i + 1u < nx) acc.add(x[i + 1u], i + 1u);
+    }
+}
+
+template <int KIND, int L, bool ALIGNED>
 __device__ __forceinline__ double separable_eval(const double *__restrict__ x, uint32_t nx, int r)
 {
     sep_acc<KIND> acc;
     acc.init();
     const uint32_t npairs = (nx + 1u) >> 1;
-    for (uint32_t c = static_cast<uint32_t>(r); c < npairs; c += L) {
-        const uint32_t i = 2u * c;
-        if constexpr (ALIGNED) {
-            const double2 v = *reinterpret_cast<const double2 *>(x + i);
-            acc.add(v.x, i);
-            acc.add(v.y, i + 1u);
-        } else {
-            acc.add(x[i], i);
-            if (i + 1u < nx) acc.add(x[i + 1u], i + 1u);
+    uint32_t c = static_cast<uint32_t>(r);
+    if constexpr (ALIGNED) {
+        // four 16-byte loads in flight per lane; the pairs are still added in
+        // increasing c
+        for (; c + 3u * L < npairs; c += 4u * L) {
+            const double2 v0 = *reinterpret_cast<const double2 *>(x + 2u * c);
+            const double2 v1 = *reinterpret_cast<const double2 *>(x + 2u * (c + L));
+            const double2 v2 = *reinterpret_cast<const double2 *>(x + 2u * (c + 2u * L));
+            const double2 v3 = *reinterpret_cast<const double2 *>(x + 2u * (c + 3u * L));
+            acc.add(v0.x, 2u * c);
+            acc.add(v0.y, 2u * c + 1u);
+            acc.add(v1.x, 2u * (c + L));
+            acc.add(v1.y, 2u * (c + L) + 1u);
+            acc.add(v2.x, 2u * (c + 2u * L));
+            acc.add(v2.y, 2u * (c + 2u * L) + 1u);
+            acc.add(v3.x, 2u * (c + 3u * L));
+            acc.add(v3.y, 2u * (c + 3u * L) + 1u);
         }
     }
+    for (; c < npairs; c += L) separable_pair<KIND, L, ALIGNED>(acc, x, nx, c);
     const double n = static_cast<double>(nx);
     if constexpr (KIND == RASTRIGIN) {
         // rastrigin.cpp:69
@@ -232,7 +258,7 @@ __device__ __forceinline__ double rosenbrock_grad(const double *__restrict__ x, 
 // rastrigin.cpp:103.
 __device__ __forceinline__ double rastrigin_grad(double xi)
 {
-    return 2 * xi + 10.0 * kOmega * sin(kOmega * xi);
+    return 2 * xi + 10.0 * kOmega * sin_any(kOmega * xi);
 }
 
 // ---------------------------------------------------------------------------

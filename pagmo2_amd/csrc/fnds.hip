@@ -24,6 +24,7 @@
 //               loop appends them.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
 
@@ -31,6 +32,8 @@
 
 struct pgh_fnds_ws {
     hipStream_t stream = nullptr;
+    hipStream_t own = nullptr; // created by pgh_fnds_run_host
+    pgh::dev_buf staged;
     uint64_t n = 0, m = 0, words = 0;
     pgh::dev_buf keys, D, cnt32, ndom32, key32, dom_count, rank, front_items, front_offsets, dom_offsets, dom_items,
         scalars;
@@ -118,20 +121,37 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const int64_t *__restrict__ ke
     if (dominates) atomicAdd(&ndom[i], dominates);
 }
 
-// Block-wide exclusive scan of one value per thread (kPeelBlock threads).
+__device__ __forceinline__ uint64_t shfl_up_u64(uint64_t v, uint32_t d)
+{
+    const uint32_t lo = __shfl_up(static_cast<uint32_t>(v), d), hi = __shfl_up(static_cast<uint32_t>(v >> 32), d);
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// Block-wide exclusive scan of one value per thread: wavefront scans, then a
+// scan of the wavefront totals.  sh holds at least blockDim.x / 64 values.
 __device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t v, uint64_t *sh, uint64_t &total)
 {
-    const uint32_t t = threadIdx.x;
-    sh[t] = v;
-    __syncthreads();
-    for (uint32_t o = 1; o < kPeelBlock; o <<= 1) {
-        const uint64_t add = t >= o ? sh[t - o] : 0u;
-        __syncthreads();
-        sh[t] += add;
-        __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    uint64_t x = v;
+#pragma unroll
+    for (uint32_t d = 1; d < 64u; d <<= 1) {
+        const uint64_t y = shfl_up_u64(x, d);
+        if (lane >= d) x += y;
     }
-    total = sh[kPeelBlock - 1];
-    const uint64_t excl = sh[t] - v;
+    if (lane == 63u) sh[wid] = x;
+    __syncthreads();
+    if (wid == 0) {
+        uint64_t t = lane < nw ? sh[lane] : 0u;
+#pragma unroll
+        for (uint32_t d = 1; d < 64u; d <<= 1) {
+            const uint64_t y = shfl_up_u64(t, d);
+            if (lane >= d) t += y;
+        }
+        if (lane < nw) sh[lane] = t;
+    }
+    __syncthreads();
+    total = sh[nw - 1u];
+    const uint64_t excl = (wid ? sh[wid - 1u] : 0u) + x - v;
     __syncthreads();
     return excl;
 }
@@ -160,22 +180,130 @@ __global__ __launch_bounds__(kPeelBlock) void k_scan(const uint32_t *__restrict_
     }
 }
 
-// dom_items: one lane per row, set bits in increasing column order.
+// dom_items: one wavefront per row; lane l takes words l, l+64, ... of the row
+// (coalesced), and a wavefront scan of the popcounts places every set bit,
+// so each row's items come out in increasing column order.
 __global__ __launch_bounds__(kBlock) void k_emit(const uint64_t *__restrict__ D, uint32_t n, uint32_t words,
                                                  const uint64_t *__restrict__ offsets, uint64_t *__restrict__ items)
 {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    uint64_t o = offsets[i];
-    const uint64_t *row = D + static_cast<uint64_t>(i) * words;
-    for (uint32_t w = 0; w < words; ++w) {
-        uint64_t word = row[w];
+    const uint32_t row = blockIdx.x * (kBlock / 64u) + threadIdx.x / 64u;
+    const uint32_t lane = threadIdx.x & 63u;
+    if (row >= n) return; // whole wavefronts retire together
+    uint64_t o = offsets[row];
+    const uint64_t *r = D + static_cast<uint64_t>(row) * words;
+    for (uint32_t w0 = 0; w0 < words; w0 += 64u) {
+        const uint32_t w = w0 + lane;
+        uint64_t word = w < words ? r[w] : 0u;
+        const uint32_t c = static_cast<uint32_t>(__popcll(word));
+        uint32_t x = c;
+#pragma unroll
+        for (uint32_t d = 1; d < 64u; d <<= 1) {
+            const uint32_t y = __shfl_up(x, d);
+            if (lane >= d) x += y;
+        }
+        const uint32_t total = __shfl(x, 63);
+        uint64_t pos = o + (x - c);
         while (word) {
-            const uint32_t b = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(word)) - 1);
-            items[o++] = static_cast<uint64_t>(w) * 64u + b;
+            items[pos++] = static_cast<uint64_t>(w) * 64u +
+                           static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(word)) - 1);
             word &= word - 1u;
         }
+        o += total;
     }
+}
+
+// The front peel with the counters in LDS (n <= 65535 and 8n bytes of LDS).
+// Level r spreads the (member, word) pairs of front r over the workgroup in
+// order; key[j] = ((r+1) << 16) | p is the largest position p of a member of
+// front r dominating j, tagged with the level so that values left by earlier
+// levels never win.
+__global__ __launch_bounds__(kPeelBlock) void k_peel_lds(const uint64_t *__restrict__ D, uint32_t n, uint32_t words,
+                                                         const uint32_t *__restrict__ dom_count,
+                                                         uint64_t *__restrict__ front_items,
+                                                         uint64_t *__restrict__ front_offsets,
+                                                         uint64_t *__restrict__ rank, uint64_t *__restrict__ scalars)
+{
+    extern __shared__ uint32_t lds_u32[];
+    uint32_t *cnt = lds_u32;
+    uint32_t *key = lds_u32 + n;
+    __shared__ uint64_t sh[kPeelBlock / 64u];
+    const uint32_t t = threadIdx.x;
+    for (uint32_t j = t; j < n; j += kPeelBlock) {
+        cnt[j] = dom_count[j];
+        key[j] = 0u;
+    }
+    __syncthreads();
+    uint64_t total;
+    {
+        // F0: ascending indices with dom_count == 0 (multi_objective.cpp:228-233)
+        const uint32_t per = (n + kPeelBlock - 1u) / kPeelBlock;
+        const uint32_t b = min(t * per, n), e = min(b + per, n);
+        uint64_t c = 0;
+        for (uint32_t i = b; i < e; ++i) c += cnt[i] == 0u;
+        uint64_t o = block_exclusive_scan(c, sh, total);
+        for (uint32_t i = b; i < e; ++i) {
+            if (cnt[i] == 0u) {
+                front_items[o++] = i;
+                rank[i] = 0;
+            }
+        }
+        if (t == 0) {
+            front_offsets[0] = 0;
+            front_offsets[1] = total;
+        }
+    }
+    uint64_t s = 0, e = total;
+    uint32_t level = 0;
+    __syncthreads();
+    while (e > s) {
+        const uint64_t items = (e - s) * words;
+        const uint64_t per = (items + kPeelBlock - 1u) / kPeelBlock;
+        const uint64_t ib = min(static_cast<uint64_t>(t) * per, items), ie = min(ib + per, items);
+        const uint32_t tag = (level + 1u) << 16;
+        // pass 1: decrement the counters of everything front r dominates
+        for (uint64_t it = ib; it < ie; ++it) {
+            const uint32_t p = static_cast<uint32_t>(it / words), w = static_cast<uint32_t>(it - uint64_t(p) * words);
+            uint64_t word = D[front_items[s + p] * words + w];
+            while (word) {
+                const uint32_t j = w * 64u + static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(word)) - 1);
+                atomicSub(&cnt[j], 1u);
+                atomicMax(&key[j], tag | p);
+                word &= word - 1u;
+            }
+        }
+        __syncthreads();
+        // pass 2: count, in (member, word, bit) order, the elements whose last
+        // dominator is this member; pass 3 writes them at their positions
+        uint64_t c = 0;
+        for (uint64_t it = ib; it < ie; ++it) {
+            const uint32_t p = static_cast<uint32_t>(it / words), w = static_cast<uint32_t>(it - uint64_t(p) * words);
+            uint64_t word = D[front_items[s + p] * words + w];
+            while (word) {
+                const uint32_t j = w * 64u + static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(word)) - 1);
+                c += (cnt[j] == 0u && key[j] == (tag | p)) ? 1u : 0u;
+                word &= word - 1u;
+            }
+        }
+        uint64_t o = e + block_exclusive_scan(c, sh, total);
+        for (uint64_t it = ib; it < ie; ++it) {
+            const uint32_t p = static_cast<uint32_t>(it / words), w = static_cast<uint32_t>(it - uint64_t(p) * words);
+            uint64_t word = D[front_items[s + p] * words + w];
+            while (word) {
+                const uint32_t j = w * 64u + static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(word)) - 1);
+                if (cnt[j] == 0u && key[j] == (tag | p)) {
+                    front_items[o++] = j;
+                    rank[j] = level + 1u;
+                }
+                word &= word - 1u;
+            }
+        }
+        ++level;
+        if (t == 0 && total) front_offsets[level + 1u] = e + total;
+        __syncthreads();
+        s = e;
+        e = e + total;
+    }
+    if (t == 0) scalars[0] = level;
 }
 
 __device__ __forceinline__ uint32_t ld_relaxed(const uint32_t *p)
@@ -187,6 +315,8 @@ __device__ __forceinline__ uint32_t ld_relaxed(const uint32_t *p)
 // memory and are only touched with atomics (they bypass the non-coherent L1).
 // key[j] is written once, at the level where cnt[j] reaches zero: 1 + the
 // largest position, in the current front, of a member dominating j.
+// The front peel for large n: counters in global memory, touched only with
+// atomics (which bypass the non-coherent L1).
 __global__ __launch_bounds__(kPeelBlock) void k_peel(const uint64_t *__restrict__ D, uint32_t n, uint32_t words,
                                                      uint32_t *__restrict__ cnt, uint32_t *__restrict__ key,
                                                      uint64_t *__restrict__ front_items,
@@ -195,7 +325,7 @@ __global__ __launch_bounds__(kPeelBlock) void k_peel(const uint64_t *__restrict_
 {
     __shared__ uint64_t sh[kPeelBlock];
     const uint32_t t = threadIdx.x;
-    // F0: ascending indices with dom_count == 0 (multi_objective.cpp:228-233)
+    // F0: ascending indices with dom_count == 0
     {
         const uint32_t per = (n + kPeelBlock - 1u) / kPeelBlock;
         const uint32_t b = t * per, e = min(b + per, n);
@@ -299,6 +429,28 @@ uint32_t grid_for(uint64_t count)
     return static_cast<uint32_t>((count + kBlock - 1u) / kBlock);
 }
 
+// Dynamic LDS a workgroup of k_peel_lds may use on the current device (the
+// static scan scratch aside), with the kernel's limit raised to it once.
+size_t max_dynamic_lds()
+{
+    static thread_local int dev_cached = -1;
+    static thread_local size_t bytes = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (dev != dev_cached) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 65536;
+        bytes = v > 1024 ? static_cast<size_t>(v) - 1024u : 0u;
+        if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_peel_lds), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                static_cast<int>(bytes)) != hipSuccess) {
+            (void)hipGetLastError();
+            bytes = 65536u - 1024u;
+        }
+        dev_cached = dev;
+    }
+    return bytes;
+}
+
 #define PGH_TRY(expr)                                                                                                \
     do {                                                                                                             \
         const pgh_status pgh_st_ = (expr);                                                                           \
@@ -322,8 +474,28 @@ pgh_status pgh_fnds_create(pgh_fnds_ws **ws)
 pgh_status pgh_fnds_destroy(pgh_fnds_ws *ws)
 {
     if (ws && ws->stream) (void)hipStreamSynchronize(ws->stream);
+    if (ws && ws->own) (void)hipStreamDestroy(ws->own);
     delete ws;
     return PGH_OK;
+}
+
+pgh_status pgh_fnds_run_host(pgh_fnds_ws *ws, const double *objs, uint64_t n, uint64_t m)
+{
+    if (!ws) return fail(PGH_INVALID_ARGUMENT, "null workspace");
+    if (n < 2u) {
+        ws->sized = false;
+        return fail(PGH_INVALID_ARGUMENT, "At least two points are needed for fast_non_dominated_sorting: "
+                                              + std::to_string(n) + " detected.");
+    }
+    if (!ws->own) {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(PGH_NO_DEVICE, "no HIP device is available");
+        PGH_HIP(hipStreamCreateWithFlags(&ws->own, hipStreamNonBlocking));
+    }
+    const uint64_t bytes = n * m * sizeof(double);
+    PGH_TRY(ws->staged.reserve(std::max<uint64_t>(bytes, 8u)));
+    if (bytes) PGH_HIP(hipMemcpyAsync(ws->staged.ptr, objs, bytes, hipMemcpyHostToDevice, ws->own));
+    return pgh_fnds_run(ws, static_cast<const double *>(ws->staged.ptr), n, m, ws->own);
 }
 
 pgh_status pgh_fnds_run(pgh_fnds_ws *ws, const double *objs, uint64_t n, uint64_t m, void *stream)
@@ -395,9 +567,16 @@ pgh_status pgh_fnds_run(pgh_fnds_ws *ws, const double *objs, uint64_t n, uint64_
     auto *scalars = static_cast<uint64_t *>(ws->scalars.ptr);
     k_scan<<<1, kPeelBlock, 0, s>>>(ndom, cnt, n32, dom_offsets, static_cast<uint64_t *>(ws->dom_count.ptr), scalars);
     PGH_HIP(hipGetLastError());
-    k_peel<<<1, kPeelBlock, 0, s>>>(D, n32, w32, cnt, key, static_cast<uint64_t *>(ws->front_items.ptr),
-                                    static_cast<uint64_t *>(ws->front_offsets.ptr),
-                                    static_cast<uint64_t *>(ws->rank.ptr), scalars);
+    const size_t peel_lds = 8u * static_cast<size_t>(n);
+    if (n < 65536u && peel_lds <= max_dynamic_lds()) {
+        k_peel_lds<<<1, kPeelBlock, peel_lds, s>>>(D, n32, w32, cnt, static_cast<uint64_t *>(ws->front_items.ptr),
+                                                   static_cast<uint64_t *>(ws->front_offsets.ptr),
+                                                   static_cast<uint64_t *>(ws->rank.ptr), scalars);
+    } else {
+        k_peel<<<1, kPeelBlock, 0, s>>>(D, n32, w32, cnt, key, static_cast<uint64_t *>(ws->front_items.ptr),
+                                        static_cast<uint64_t *>(ws->front_offsets.ptr),
+                                        static_cast<uint64_t *>(ws->rank.ptr), scalars);
+    }
     PGH_HIP(hipGetLastError());
     return PGH_OK;
 }
@@ -411,7 +590,7 @@ pgh_status pgh_fnds_sizes(pgh_fnds_ws *ws, uint64_t *n_fronts, uint64_t *n_dom_i
         // dom_items is written once its size is known
         const uint64_t total = ws->h_sizes[1];
         PGH_TRY(ws->dom_items.reserve(std::max<uint64_t>(total, 1u) * 8u));
-        k_emit<<<grid_for(ws->n), kBlock, 0, ws->stream>>>(
+        k_emit<<<static_cast<uint32_t>((ws->n + 3u) / 4u), kBlock, 0, ws->stream>>>(
             static_cast<const uint64_t *>(ws->D.ptr), static_cast<uint32_t>(ws->n), static_cast<uint32_t>(ws->words),
             static_cast<const uint64_t *>(ws->dom_offsets.ptr), static_cast<uint64_t *>(ws->dom_items.ptr));
         PGH_HIP(hipGetLastError());

@@ -1,0 +1,239 @@
+// test_hip_bfe.cpp -- hip_bfe as a drop-in pagmo UDBFE.
+//
+//   test_hip_bfe cpu   contract checks that need no GPU (type requirements,
+//                      fallback to the CPU evaluators, validation messages,
+//                      the loud failure of a supported problem without device)
+//   test_hip_bfe gpu   parity with thread_bfe through pagmo::bfe, fevals
+//                      accounting, pso_gen with set_bfe (tests/pso_gen.cpp:
+//                      302-318 style), batch_gradient, fast_non_dominated_sorting
+//
+// Built against pagmo's headers when they exist, and otherwise against the
+// stand-in in tests/cpp/pagmo_compat.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <type_traits>
+
+#include <pagmo/algorithms/pso_gen.hpp>
+#include <pagmo/population.hpp>
+
+#include "pagmo_hip/hip_bfe.hpp"
+
+using namespace pagmo;
+using pagmo_hip::hip_bfe;
+
+static int g_fail = 0;
+#define CHECK(cond)                                                                                                  \
+    do {                                                                                                             \
+        if (!(cond)) {                                                                                               \
+            std::printf("FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond);                                           \
+            ++g_fail;                                                                                                \
+        }                                                                                                            \
+    } while (0)
+
+template <typename E, typename F>
+static std::string throws(F &&f)
+{
+    try {
+        f();
+    } catch (const E &e) {
+        return e.what();
+    } catch (...) {
+        return "<other exception>";
+    }
+    return "<no exception>";
+}
+
+static vector_double random_dvs(const problem &p, std::size_t n, unsigned seed)
+{
+    std::mt19937 e(seed);
+    const auto b = p.get_bounds();
+    vector_double out(n * p.get_nx());
+    for (std::size_t i = 0; i < out.size(); ++i) {
+        const auto j = i % p.get_nx();
+        out[i] = std::uniform_real_distribution<double>(b.first[j], b.second[j])(e);
+    }
+    return out;
+}
+
+// A user problem with no device kernel (thread safety basic by default).
+struct user_udp {
+    vector_double fitness(const vector_double &x) const
+    {
+        return {x[0] * x[0] + 3. * x[1]};
+    }
+    std::pair<vector_double, vector_double> get_bounds() const
+    {
+        return {{-1., -1.}, {1., 1.}};
+    }
+};
+
+// A problem that forbids concurrent use (tests/thread_bfe.cpp:99-116).
+struct unsafe_udp {
+    vector_double fitness(const vector_double &) const
+    {
+        return {1.};
+    }
+    std::pair<vector_double, vector_double> get_bounds() const
+    {
+        return {{0.}, {1.}};
+    }
+    thread_safety get_thread_safety() const
+    {
+        return thread_safety::none;
+    }
+    std::string get_name() const
+    {
+        return "unsafe_prob";
+    }
+};
+
+static void cpu_tests()
+{
+    // UDBFE requirements (bfe.hpp:97-108)
+    static_assert(std::is_default_constructible<hip_bfe>::value, "");
+    static_assert(std::is_copy_constructible<hip_bfe>::value, "");
+    static_assert(std::is_move_constructible<hip_bfe>::value, "");
+    static_assert(std::is_destructible<hip_bfe>::value, "");
+    CHECK(hip_bfe{}.get_thread_safety() == thread_safety::basic);
+    CHECK(hip_bfe{}.get_name().find("HIP") != std::string::npos);
+
+    // problems without a kernel go through default_bfe's dispatch
+    problem up{user_udp{}};
+    pgh_udp u;
+    CHECK(!pagmo_hip::describe(up, u));
+    const auto dvs = random_dvs(up, 100, 1);
+    bfe b{hip_bfe{}};
+    const auto f = b(up, dvs);
+    CHECK(f.size() == 100u);
+    bool same = true;
+    for (std::size_t i = 0; i < 100; ++i) {
+        same = same && f[i] == user_udp{}.fitness({dvs[2 * i], dvs[2 * i + 1]})[0];
+    }
+    CHECK(same);
+    // fevals: the problem's per-call counter through thread_bfe's copies is not
+    // shared; the wrapper adds n_dvs once (bfe.cpp:102-107)
+    CHECK(up.get_fevals() == 100u);
+
+    problem unsafe{unsafe_udp{}};
+    const auto msg = throws<std::invalid_argument>([&] { hip_bfe{}(unsafe, vector_double(4, 0.5)); });
+    CHECK(msg.find("Cannot execute fitness evaluations in batch mode for a problem of type 'unsafe_prob'")
+          != std::string::npos);
+
+    // supported problems are recognised from the UDP type
+    CHECK(pagmo_hip::describe(problem{rosenbrock{10u}}, u) && u.kind == PGH_ROSENBROCK && u.dim == 10u);
+    CHECK(pagmo_hip::describe(problem{lennard_jones{48u}}, u) && u.kind == PGH_LENNARD_JONES && u.dim == 48u);
+    CHECK(pagmo_hip::describe(problem{ackley{7u}}, u) && u.kind == PGH_ACKLEY && u.dim == 7u);
+
+    // the library validates the batch like bfe_check_input_dvs (bfe_impl.cpp:63-88)
+    const auto m2 = throws<std::invalid_argument>([&] { hip_bfe{}(problem{rosenbrock{3u}}, vector_double(7, 0.)); });
+    CHECK(m2 == "Invalid argument for a batch fitness evaluation: the length of the vector representing the decision "
+                "vectors, 7, is not an exact multiple of the dimension of the problem, 3");
+
+    if (!pgh_device_available()) {
+        // a supported problem with no device fails loudly
+        const auto m3 = throws<std::runtime_error>([&] { hip_bfe{}(problem{rosenbrock{3u}}, vector_double(6, 0.)); });
+        CHECK(m3.find("libpagmo_hip") != std::string::npos);
+    }
+}
+
+static double rel(double a, double b)
+{
+    return std::abs(a - b) / std::max(std::abs(b), 1e-300);
+}
+
+template <typename UDP>
+static void parity(UDP udp, std::size_t n, bool exact, double tol)
+{
+    problem p{udp};
+    const auto dvs = random_dvs(p, n, 7);
+    bfe hb{hip_bfe{}}, tb{thread_bfe{}};
+    const auto a = hb(p, dvs);
+    const auto b = tb(p, dvs);
+    CHECK(a.size() == b.size());
+    double worst = 0.;
+    for (std::size_t i = 0; i < a.size(); ++i) {
+        if (exact) {
+            CHECK(a[i] == b[i]);
+        } else {
+            worst = std::max(worst, rel(a[i], b[i]));
+        }
+    }
+    if (!exact && !(worst <= tol)) {
+        std::printf("FAILED parity %s: worst rel %.3e\n", p.get_name().c_str(), worst);
+        ++g_fail;
+    }
+    // the wrapper counted both calls, each once (tests/thread_bfe.cpp:66-96)
+    CHECK(p.get_fevals() == 2u * n);
+}
+
+static void gpu_tests()
+{
+    if (!pgh_device_available()) {
+        std::printf("FAILED: no gfx950 device\n");
+        ++g_fail;
+        return;
+    }
+    parity(rosenbrock{10u}, 1000, true, 0.);
+    parity(rosenbrock{501u}, 300, true, 0.);
+    parity(rastrigin{1000u}, 512, false, 1e-12);
+    parity(ackley{1000u}, 512, false, 1e-12);
+    parity(schwefel{33u}, 700, false, 1e-12);
+    parity(griewank{100u}, 700, false, 1e-12);
+    parity(lennard_jones{5u}, 300, false, 1e-9);
+
+    // pso_gen with bfe equals pso_gen without (tests/pso_gen.cpp:302-318), and
+    // hip_bfe reproduces thread_bfe's trajectory bit for bit on rosenbrock
+    {
+        population pop{problem{rosenbrock{10u}}, 64u, 42u};
+        pso_gen a1{100u, 0.7298, 2.05, 2.05, 0.5, 5u, 2u, 4u, false, 42u};
+        pso_gen a2{100u, 0.7298, 2.05, 2.05, 0.5, 5u, 2u, 4u, false, 42u};
+        pso_gen a3{100u, 0.7298, 2.05, 2.05, 0.5, 5u, 2u, 4u, false, 42u};
+        a1.set_bfe(bfe{hip_bfe{}});
+        a2.set_bfe(bfe{thread_bfe{}});
+        const auto p1 = a1.evolve(pop), p2 = a2.evolve(pop), p3 = a3.evolve(pop);
+        CHECK(p1.get_f() == p2.get_f());
+        CHECK(p1.get_x() == p2.get_x());
+        CHECK(p1.get_f() == p3.get_f());
+    }
+
+    // batched gradient: rosenbrock bit-exact with its UDP gradient
+    {
+        problem p{rosenbrock{7u}};
+        const auto dvs = random_dvs(p, 50, 3);
+        const auto g = pagmo_hip::batch_gradient(p, dvs);
+        bool same = true;
+        for (std::size_t i = 0; i < 50; ++i) {
+            const auto gi = p.gradient(vector_double(dvs.begin() + static_cast<std::ptrdiff_t>(7 * i),
+                                                     dvs.begin() + static_cast<std::ptrdiff_t>(7 * i + 7)));
+            for (std::size_t j = 0; j < 7; ++j) same = same && gi[j] == g[7 * i + j];
+        }
+        CHECK(same);
+    }
+
+    // fast_non_dominated_sorting: the reference's known answer and a tied random set
+    {
+        std::vector<vector_double> ex = {{0, 7}, {1, 5}, {2, 3}, {4, 2}, {7, 1}, {10, 0},
+                                         {2, 6}, {4, 4}, {10, 2}, {6, 6}, {9, 5}};
+        CHECK(pagmo_hip::fast_non_dominated_sorting(ex) == fast_non_dominated_sorting(ex));
+        std::mt19937 e(5);
+        std::vector<vector_double> pts(2000, vector_double(3));
+        for (auto &v : pts)
+            for (auto &x : v) x = static_cast<double>(e() % 7u);
+        CHECK(pagmo_hip::fast_non_dominated_sorting(pts) == fast_non_dominated_sorting(pts));
+        const auto m = throws<std::invalid_argument>(
+            [&] { pagmo_hip::fast_non_dominated_sorting(std::vector<vector_double>{{1., 2.}}); });
+        CHECK(m == "At least two points are needed for fast_non_dominated_sorting: 1 detected.");
+    }
+}
+
+int main(int argc, char **argv)
+{
+    const bool gpu = argc > 1 && std::strcmp(argv[1], "gpu") == 0;
+    cpu_tests();
+    if (gpu) gpu_tests();
+    std::printf("%s: %d failure(s)\n", gpu ? "gpu" : "cpu", g_fail);
+    return g_fail ? 1 : 0;
+}
