@@ -30,6 +30,19 @@ __global__ __launch_bounds__(kBlock) void k_separable(const double *__restrict__
     if (ind < n && r == 0) fvs[ind] = f;
 }
 
+// Recomputes, with the math library's cosine and the reference's sequential
+// order, the individuals that k_separable marked (an argument outside the
+// fast reduction's range).  One lane per individual: the scan of the marks is
+// a single coalesced pass.
+template <int KIND, bool ALIGNED>
+__global__ __launch_bounds__(kBlock) void k_separable_fix(const double *__restrict__ dvs, uint64_t n, uint32_t nx,
+                                                          double *__restrict__ fvs)
+{
+    const uint64_t ind = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (ind >= n || !is_recompute_mark(fvs[ind])) return;
+    fvs[ind] = separable_eval_exact<KIND, 1, ALIGNED>(dvs + ind * nx, nx, 0);
+}
+
 // ---- rosenbrock: one lane per individual ------------------------------------
 __global__ __launch_bounds__(kBlock) void k_rosenbrock(const double *__restrict__ dvs, uint64_t n, uint32_t nx,
                                                        double *__restrict__ fvs)
@@ -53,7 +66,7 @@ __global__ __launch_bounds__(kBlock) void k_mo_sphere(const double *__restrict__
 }
 
 // ---- lennard_jones: G individuals per workgroup staged in LDS --------------
-template <int L>
+template <int L, uint32_t APL>
 __global__ __launch_bounds__(kBlock) void k_lj(const double *__restrict__ dvs, uint64_t n, uint32_t atoms,
                                                uint32_t G, double *__restrict__ fvs)
 {
@@ -66,11 +79,11 @@ __global__ __launch_bounds__(kBlock) void k_lj(const double *__restrict__ dvs, u
     const uint32_t g = threadIdx.x / L;
     const int r = static_cast<int>(threadIdx.x % L);
     const uint32_t gg = g < count ? g : count - 1u;
-    const double e = lj_energy<L>(lds + gg * 3u * atoms, atoms, r);
+    const double e = lj_energy<L, APL>(lds + gg * 3u * atoms, atoms, r);
     if (g < count && r == 0) fvs[ind0 + g] = e;
 }
 
-template <int L>
+template <int L, uint32_t APL>
 __global__ __launch_bounds__(kBlock) void k_lj_grad(const double *__restrict__ dvs, uint64_t n, uint32_t atoms,
                                                     uint32_t G, double *__restrict__ grad, double *__restrict__ fvs)
 {
@@ -83,7 +96,7 @@ __global__ __launch_bounds__(kBlock) void k_lj_grad(const double *__restrict__ d
     const uint32_t g = threadIdx.x / L;
     const int r = static_cast<int>(threadIdx.x % L);
     if (g >= count) return; // whole groups retire together: no shuffle partner is lost
-    const double e = lj_energy_grad<L>(lds + g * 3u * atoms, atoms, r, grad + (ind0 + g) * nx);
+    const double e = lj_energy_grad<L, APL>(lds + g * 3u * atoms, atoms, r, grad + (ind0 + g) * nx);
     if (fvs && r == 0) fvs[ind0 + g] = e;
 }
 
@@ -112,10 +125,15 @@ pgh_status launch_sep_l(const double *dvs, uint64_t n, uint32_t nx, double *fvs,
     constexpr uint64_t G = kBlock / L;
     const uint64_t blocks = (n + G - 1u) / G;
     const bool aligned = (nx % 2u == 0u) && (reinterpret_cast<uintptr_t>(dvs) % 16u == 0u);
+    const dim3 grid(static_cast<uint32_t>(blocks));
+    const dim3 fix_grid(static_cast<uint32_t>((n + kBlock - 1u) / kBlock));
+    constexpr bool fixable = (KIND == RASTRIGIN || KIND == ACKLEY);
     if (aligned) {
-        k_separable<KIND, L, true><<<dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, s>>>(dvs, n, nx, fvs);
+        k_separable<KIND, L, true><<<grid, dim3(kBlock), 0, s>>>(dvs, n, nx, fvs);
+        if (fixable) k_separable_fix<KIND, true><<<fix_grid, dim3(kBlock), 0, s>>>(dvs, n, nx, fvs);
     } else {
-        k_separable<KIND, L, false><<<dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, s>>>(dvs, n, nx, fvs);
+        k_separable<KIND, L, false><<<grid, dim3(kBlock), 0, s>>>(dvs, n, nx, fvs);
+        if (fixable) k_separable_fix<KIND, false><<<fix_grid, dim3(kBlock), 0, s>>>(dvs, n, nx, fvs);
     }
     PGH_HIP(hipGetLastError());
     return PGH_OK;
@@ -134,19 +152,37 @@ pgh_status launch_sep(const double *dvs, uint64_t n, uint32_t nx, double *fvs, h
     }
 }
 
-template <int L>
-pgh_status launch_lj_l(const double *dvs, uint64_t n, uint32_t atoms, double *fvs, double *grad, hipStream_t s)
+template <int L, uint32_t APL>
+pgh_status launch_lj_la(const double *dvs, uint64_t n, uint32_t atoms, double *fvs, double *grad, hipStream_t s)
 {
     const uint32_t G = lj_group_count(atoms);
     const uint64_t blocks = (n + G - 1u) / G;
     const size_t lds = static_cast<size_t>(G) * 3u * atoms * sizeof(double);
     if (grad) {
-        k_lj_grad<L><<<dim3(static_cast<uint32_t>(blocks)), dim3(G * L), lds, s>>>(dvs, n, atoms, G, grad, fvs);
+        k_lj_grad<L, APL><<<dim3(static_cast<uint32_t>(blocks)), dim3(G * L), lds, s>>>(dvs, n, atoms, G, grad,
+                                                                                        fvs);
     } else {
-        k_lj<L><<<dim3(static_cast<uint32_t>(blocks)), dim3(G * L), lds, s>>>(dvs, n, atoms, G, fvs);
+        k_lj<L, APL><<<dim3(static_cast<uint32_t>(blocks)), dim3(G * L), lds, s>>>(dvs, n, atoms, G, fvs);
     }
     PGH_HIP(hipGetLastError());
     return PGH_OK;
+}
+
+template <int L>
+pgh_status launch_lj_l(const double *dvs, uint64_t n, uint32_t atoms, double *fvs, double *grad, hipStream_t s)
+{
+    switch (lj_apl(atoms, L)) {
+        case 1:
+            return launch_lj_la<L, 1>(dvs, n, atoms, fvs, grad, s);
+        case 2:
+            return launch_lj_la<L, 2>(dvs, n, atoms, fvs, grad, s);
+        case 3:
+            return launch_lj_la<L, 3>(dvs, n, atoms, fvs, grad, s);
+        case 4:
+            return launch_lj_la<L, 4>(dvs, n, atoms, fvs, grad, s);
+        default:
+            return launch_lj_la<L, 0>(dvs, n, atoms, fvs, grad, s);
+    }
 }
 
 pgh_status launch_lj(const double *dvs, uint64_t n, uint32_t atoms, double *fvs, double *grad, hipStream_t s)

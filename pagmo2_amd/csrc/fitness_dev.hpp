@@ -134,23 +134,36 @@ __device__ __forceinline__ int group_max(int v)
 // and 2c+1) and accumulates them in increasing order; the L partial results
 // are then combined by group_sum/group_prod.  ALIGNED selects 16-byte loads
 // (even nx and a 16-byte aligned row) without changing that order.
-template <int KIND>
+// FAST: rastrigin and ackley take the cosine from cos_pi_reduced and only
+// record whether an argument fell outside its range (|y| >= 2^20, inf, NaN);
+// the caller then evaluates the whole individual again with FAST = false.
+template <int KIND, bool FAST = false>
 struct sep_acc {
     double a = 0., b = 0.;
+    bool out_of_range = false;
     __device__ __forceinline__ void init()
     {
         a = 0.;
         b = (KIND == GRIEWANK) ? 1. : 0.;
     }
+    __device__ __forceinline__ double cosine(double y)
+    {
+        if constexpr (FAST) {
+            out_of_range |= !(fabs(y) < 0x1p20);
+            return cos_pi_reduced(y);
+        } else {
+            return cos_any(y);
+        }
+    }
     __device__ __forceinline__ void add(double xi, uint32_t i)
     {
         if constexpr (KIND == RASTRIGIN) {
             // rastrigin.cpp:67
-            a += xi * xi - 10. * cos_any(kOmega * xi);
+            a += xi * xi - 10. * cosine(kOmega * xi);
         } else if constexpr (KIND == ACKLEY) {
             // ackley.cpp:69-70
             a += xi * xi;
-            b += cos_any(kOmega * xi);
+            b += cosine(kOmega * xi);
         } else if constexpr (KIND == SCHWEFEL) {
             // schwefel.cpp:64
             a += xi * sin(sqrt(fabs(xi)));
@@ -164,8 +177,8 @@ struct sep_acc {
     }
 };
 
-template <int KIND, int L, bool ALIGNED>
-__device__ __forceinline__ void separable_pair(sep_acc<KIND> &acc, const double *__restrict__ x, uint32_t nx,
+template <int KIND, int L, bool ALIGNED, bool FAST>
+__device__ __forceinline__ void separable_pair(sep_acc<KIND, FAST> &acc, const double *__restrict__ x, uint32_t nx,
                                                uint32_t c)
 {
     const uint32_t i = 2u * c;
@@ -179,14 +192,25 @@ __device__ __forceinline__ void separable_pair(sep_acc<KIND> &acc, const double 
     }
 }
 
-template <int KIND, int L, bool ALIGNED>
-__device__ __forceinline__ double separable_eval(const double *__restrict__ x, uint32_t nx, int r)
+template <int KIND, int L, bool ALIGNED, bool FAST, int UNROLL = 1>
+__device__ __forceinline__ double separable_eval_impl(const double *__restrict__ x, uint32_t nx, int r,
+                                                      bool &out_of_range)
 {
-    sep_acc<KIND> acc;
+    sep_acc<KIND, FAST> acc;
     acc.init();
     const uint32_t npairs = (nx + 1u) >> 1;
     uint32_t c = static_cast<uint32_t>(r);
-    if constexpr (ALIGNED) {
+    if constexpr (ALIGNED && UNROLL == 2) {
+        for (; c + L < npairs; c += 2u * L) {
+            const double2 v0 = *reinterpret_cast<const double2 *>(x + 2u * c);
+            const double2 v1 = *reinterpret_cast<const double2 *>(x + 2u * (c + L));
+            acc.add(v0.x, 2u * c);
+            acc.add(v0.y, 2u * c + 1u);
+            acc.add(v1.x, 2u * (c + L));
+            acc.add(v1.y, 2u * (c + L) + 1u);
+        }
+    }
+    if constexpr (ALIGNED && UNROLL == 4) {
         // four 16-byte loads in flight per lane; the pairs are still added in
         // increasing c
         for (; c + 3u * L < npairs; c += 4u * L) {
@@ -204,7 +228,8 @@ __device__ __forceinline__ double separable_eval(const double *__restrict__ x, u
             acc.add(v3.y, 2u * (c + 3u * L) + 1u);
         }
     }
-    for (; c < npairs; c += L) separable_pair<KIND, L, ALIGNED>(acc, x, nx, c);
+    for (; c < npairs; c += L) separable_pair<KIND, L, ALIGNED, FAST>(acc, x, nx, c);
+    out_of_range = acc.out_of_range;
     const double n = static_cast<double>(nx);
     if constexpr (KIND == RASTRIGIN) {
         // rastrigin.cpp:69
@@ -224,6 +249,42 @@ __device__ __forceinline__ double separable_eval(const double *__restrict__ x, u
         const double p = group_prod<L>(acc.b);
         return s / 4000. - p + 1.;
     }
+}
+
+// A signalling NaN: arithmetic never produces one, so it can mark a result
+// that must be recomputed.
+constexpr unsigned long long kRecomputeBits = 0x7ff44f5253464958ull;
+
+__device__ __forceinline__ bool is_recompute_mark(double f)
+{
+    return static_cast<unsigned long long>(__double_as_longlong(f)) == kRecomputeBits;
+}
+
+// Fitness of one individual by the L lanes of its group (lane r); the same
+// value is returned to every lane.  The per-element operations and the
+// reduction tree depend on nx and L only.  For rastrigin and ackley the
+// cosines come from cos_pi_reduced; an individual with an argument outside its
+// range (|y| >= 2^20, inf, NaN) gets the recompute mark instead, and
+// separable_eval_exact produces its value (kept out of this routine so that
+// its registers do not weigh on the streaming kernel).
+template <int KIND, int L, bool ALIGNED>
+__device__ __forceinline__ double separable_eval(const double *__restrict__ x, uint32_t nx, int r)
+{
+    constexpr bool fast = (KIND == RASTRIGIN || KIND == ACKLEY);
+    bool bad = false;
+    const double f = separable_eval_impl<KIND, L, ALIGNED, fast>(x, nx, r, bad);
+    if constexpr (fast) {
+        if (group_max<L>(bad ? 1 : 0)) return __longlong_as_double(static_cast<long long>(kRecomputeBits));
+    }
+    return f;
+}
+
+// The same individual with the math library's cosine for every element.
+template <int KIND, int L, bool ALIGNED>
+__device__ __forceinline__ double separable_eval_exact(const double *__restrict__ x, uint32_t nx, int r)
+{
+    bool unused;
+    return separable_eval_impl<KIND, L, ALIGNED, false>(x, nx, r, unused);
 }
 
 // ---------------------------------------------------------------------------
@@ -323,116 +384,220 @@ __device__ __forceinline__ uint32_t lj_row_start(uint32_t i, uint32_t atoms)
     return i * atoms - (i * (i + 1u)) / 2u;
 }
 
-// Combines per-lane partial sums with the reference's dist == 0 semantics:
-// a zero distance overwrites the accumulator with DBL_MAX and later finite
-// terms cannot bring it back, so the result is +inf after the final *4,
-// unless a NaN term follows the last zero pair (then NaN).
-template <int L>
-__device__ __forceinline__ double lj_finish(double acc, int zlast, int nlast)
+// Index, in the reference's loop order, of the pair {a, b}.
+__device__ __forceinline__ int lj_pair_index(uint32_t a, uint32_t b, uint32_t atoms)
 {
-    acc = group_sum<L>(acc);
-    zlast = group_max<L>(zlast);
-    nlast = group_max<L>(nlast);
-    if (zlast >= 0) {
-        return nlast > zlast ? __builtin_nan("") : __builtin_inf();
-    }
-    return 4 * acc;
+    const uint32_t i = a < b ? a : b, j = a < b ? b : a;
+    return static_cast<int>(lj_row_start(i, atoms) + (j - i - 1u));
 }
 
-// Energy of the individual staged at X/Y/Z (= X + atoms, X + 2*atoms); lane r
-// of L takes a contiguous slice of the pair sequence.
+// 1 / d for the sixth power of a distance: v_rcp_f64 refined by two Newton
+// steps, and the IEEE division where the refinement is not valid (d zero,
+// subnormal, infinite or NaN).
+__device__ __forceinline__ double lj_recip(double d)
+{
+    if (d > 0x1p-1000 && d < 0x1p1000) {
+        double r = __builtin_amdgcn_rcp(d);
+        r = fma(r, fma(-d, r, 1.0), r);
+        return fma(r, fma(-d, r, 1.0), r);
+    }
+    return 1.0 / d;
+}
+
+// Cyclic pair schedule of an individual's atoms over the L lanes of its group:
+// lane r owns atoms a = r, r+L, ...; step k (1 <= k <= A/2) pairs each owned
+// atom a with b = (a + k) mod A, and when A is even the last step keeps only
+// a < A/2, so every unordered pair is visited exactly once.  Within a step the
+// lanes of a group read consecutive atoms.
+struct lj_cycle {
+    uint32_t atoms, half;
+    __device__ __forceinline__ explicit lj_cycle(uint32_t a) : atoms(a), half(a / 2u) {}
+    __device__ __forceinline__ bool counts(uint32_t a, uint32_t k) const
+    {
+        return 2u * k < atoms || a < half; // k == A/2 with A even: one of the two visits
+    }
+    __device__ __forceinline__ uint32_t partner(uint32_t a, uint32_t k) const
+    {
+        const uint32_t b = a + k;
+        return b >= atoms ? b - atoms : b;
+    }
+};
+
+// The reference's dist == 0 rule (lennard_jones.cpp:81-83,89): the result is
+// +inf unless a NaN term comes after the last zero-distance pair in the
+// reference's pair order, then NaN.  Only reached when some pair of the
+// individual has zero distance; recomputes the group's terms to find the NaNs.
 template <int L>
+__device__ __noinline__ double lj_zero_rule(const double *__restrict__ X, uint32_t atoms, int r, int zlast)
+{
+    const double *__restrict__ Y = X + atoms;
+    const double *__restrict__ Z = X + 2u * atoms;
+    const lj_cycle cyc(atoms);
+    int nlast = -1;
+    for (uint32_t k = 1u; k <= cyc.half; ++k) {
+        for (uint32_t a = static_cast<uint32_t>(r); a < atoms; a += L) {
+            if (!cyc.counts(a, k)) continue;
+            const uint32_t b = cyc.partner(a, k);
+            const double dx = X[a] - X[b], dy = Y[a] - Y[b], dz = Z[a] - Z[b];
+            const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+            if (r2 == 0.0) continue;
+            const double s = lj_recip(r2 * r2 * r2);
+            const double t = fma(s, s, -s);
+            if (t != t) nlast = max(nlast, lj_pair_index(a, b, atoms));
+        }
+    }
+    nlast = group_max<L>(nlast);
+    return nlast > zlast ? __builtin_nan("") : __builtin_inf();
+}
+
+// Lane r of a group owns atoms a_q = r + q L.  APL > 0: exactly APL owned
+// slots (the ceiling of atoms / L), all held in registers; APL == 0: any
+// count, taken four slots at a time.  Owned slots beyond the last atom are
+// masked.  Processing a lane's slots together gives each step of the
+// schedule several independent chains of arithmetic.
+__host__ __device__ constexpr uint32_t lj_apl(uint32_t atoms, int L)
+{
+    const uint32_t apl = (atoms + static_cast<uint32_t>(L) - 1u) / static_cast<uint32_t>(L);
+    return apl <= 4u ? apl : 0u;
+}
+
+template <int L, uint32_t NS>
+struct lj_slots {
+    uint32_t a[NS];
+    bool own[NS];
+    double x[NS], y[NS], z[NS];
+    __device__ __forceinline__ void load(const double *__restrict__ X, const double *__restrict__ Y,
+                                         const double *__restrict__ Z, uint32_t a0, uint32_t atoms)
+    {
+#pragma unroll
+        for (uint32_t q = 0; q < NS; ++q) {
+            a[q] = a0 + q * L;
+            own[q] = a[q] < atoms;
+            if (!own[q]) a[q] = a0;
+            x[q] = X[a[q]];
+            y[q] = Y[a[q]];
+            z[q] = Z[a[q]];
+        }
+    }
+};
+
+template <int L, uint32_t NS>
+__device__ __forceinline__ void lj_energy_slots(const double *__restrict__ X, const double *__restrict__ Y,
+                                                const double *__restrict__ Z, uint32_t atoms, uint32_t a0,
+                                                double &acc, int &zlast)
+{
+    const lj_cycle cyc(atoms);
+    lj_slots<L, NS> sl;
+    sl.load(X, Y, Z, a0, atoms);
+    for (uint32_t k = 1u; k <= cyc.half; ++k) {
+#pragma unroll
+        for (uint32_t q = 0; q < NS; ++q) {
+            const uint32_t b = cyc.partner(sl.a[q], k);
+            const double dx = sl.x[q] - X[b], dy = sl.y[q] - Y[b], dz = sl.z[q] - Z[b];
+            const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+            const bool valid = sl.own[q] && cyc.counts(sl.a[q], k);
+            const double s = lj_recip(r2 * r2 * r2);
+            const double t = fma(s, s, -s);
+            acc += (valid && r2 != 0.0) ? t : 0.0;
+            if (valid && r2 == 0.0) zlast = max(zlast, lj_pair_index(sl.a[q], b, atoms));
+        }
+    }
+}
+
+// Energy of the individual staged at X (Y = X + A, Z = X + 2A) by the L lanes
+// of a group (lane r): sum of 4 (s^2 - s), s = r^-6, over the cyclic
+// schedule, then the group's butterfly sum.
+template <int L, uint32_t APL>
 __device__ __forceinline__ double lj_energy(const double *__restrict__ X, uint32_t atoms, int r)
 {
     const double *__restrict__ Y = X + atoms;
     const double *__restrict__ Z = X + 2u * atoms;
-    const uint32_t P = atoms * (atoms - 1u) / 2u;
-    const uint32_t k0 = static_cast<uint32_t>((static_cast<uint64_t>(P) * r) / L);
-    const uint32_t k1 = static_cast<uint32_t>((static_cast<uint64_t>(P) * (r + 1)) / L);
-    uint32_t i = 0u, rem = k0;
-    while (i + 1u < atoms && rem >= atoms - 1u - i) {
-        rem -= atoms - 1u - i;
-        ++i;
-    }
-    uint32_t j = i + 1u + rem;
     double acc = 0.;
-    int zlast = -1, nlast = -1;
-    double xi = X[i], yi = Y[i], zi = Z[i];
-    for (uint32_t k = k0; k < k1; ++k) {
-        const double dx = xi - X[j], dy = yi - Y[j], dz = zi - Z[j];
-        const double dist = dx * dx + dy * dy + dz * dz;
-        if (dist == 0.0) {
-            zlast = static_cast<int>(k);
-        } else {
-            const double inv = 1.0 / dist;
-            const double sixth = inv * inv * inv;
-            const double t = sixth * sixth - sixth;
-            if (t != t) nlast = static_cast<int>(k);
-            acc += t;
-        }
-        if (++j == atoms) {
-            ++i;
-            j = i + 1u;
-            if (i < atoms) {
-                xi = X[i];
-                yi = Y[i];
-                zi = Z[i];
-            }
+    int zlast = -1;
+    if constexpr (APL > 0) {
+        lj_energy_slots<L, APL>(X, Y, Z, atoms, static_cast<uint32_t>(r), acc, zlast);
+    } else {
+        for (uint32_t a0 = static_cast<uint32_t>(r); a0 < atoms; a0 += 4u * L) {
+            lj_energy_slots<L, 4>(X, Y, Z, atoms, a0, acc, zlast);
         }
     }
-    return lj_finish<L>(acc, zlast, nlast);
+    acc = group_sum<L>(acc);
+    zlast = group_max<L>(zlast);
+    if (zlast >= 0) return lj_zero_rule<L>(X, atoms, r, zlast);
+    return 4 * acc;
 }
 
-// Energy and gradient: lane r owns atoms a = r, r+L, ...; each owned atom
-// sums the force of every other atom (dE/dp_a = sum_j (p_a - p_j) *
-// (24 s - 48 s^2) / r^2, s = r^-6) and the pairs (a, j>a) of the energy.
-// The gradient row (nx doubles) is written to g; returns the energy.
-template <int L>
+template <int L, uint32_t NS>
+__device__ __forceinline__ void lj_grad_slots(const double *__restrict__ X, const double *__restrict__ Y,
+                                              const double *__restrict__ Z, uint32_t atoms, uint32_t a0,
+                                              double &acc, int &zlast, double *__restrict__ g)
+{
+    const lj_cycle cyc(atoms);
+    lj_slots<L, NS> sl;
+    sl.load(X, Y, Z, a0, atoms);
+    double gx[NS], gy[NS], gz[NS];
+#pragma unroll
+    for (uint32_t q = 0; q < NS; ++q) gx[q] = gy[q] = gz[q] = 0.;
+    for (uint32_t k = 1u; k < atoms; ++k) {
+#pragma unroll
+        for (uint32_t q = 0; q < NS; ++q) {
+            const uint32_t b = cyc.partner(sl.a[q], k);
+            const double dx = sl.x[q] - X[b], dy = sl.y[q] - Y[b], dz = sl.z[q] - Z[b];
+            const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+            const double s = lj_recip(r2 * r2 * r2);
+            const double inv_r2 = (s * r2) * r2;
+            const double coef = (fma(-48., s, 24.) * s) * inv_r2;
+            gx[q] = fma(coef, dx, gx[q]);
+            gy[q] = fma(coef, dy, gy[q]);
+            gz[q] = fma(coef, dz, gz[q]);
+            const bool valid = sl.own[q] && k <= cyc.half && cyc.counts(sl.a[q], k);
+            const double t = fma(s, s, -s);
+            acc += (valid && r2 != 0.0) ? t : 0.0;
+            if (valid && r2 == 0.0) zlast = max(zlast, lj_pair_index(sl.a[q], b, atoms));
+        }
+    }
+#pragma unroll
+    for (uint32_t q = 0; q < NS; ++q) {
+        if (!sl.own[q]) continue;
+        const uint32_t a = sl.a[q];
+        if (a == 1u) {
+            g[0] = gz[q];
+        } else if (a == 2u) {
+            g[1] = gy[q];
+            g[2] = gz[q];
+        } else if (a >= 3u) {
+            const uint32_t o = 3u * (a - 2u);
+            g[o] = gx[q];
+            g[o + 1u] = gy[q];
+            g[o + 2u] = gz[q];
+        }
+    }
+}
+
+// Energy and gradient: lane r owns atoms a = r, r+L, ... and visits every
+// other atom b = (a + k) mod A, k = 1..A-1, accumulating the force
+// dE/dp_a = sum_b (p_a - p_b) (24 s - 48 s^2) / r^2 (s = r^-6; 1/r^2 = s r^4)
+// and, on the steps the cyclic schedule counts, the energy terms.  The
+// gradient row (nx doubles) is written to g; returns the energy.
+template <int L, uint32_t APL>
 __device__ __forceinline__ double lj_energy_grad(const double *__restrict__ X, uint32_t atoms, int r,
                                                  double *__restrict__ g)
 {
     const double *__restrict__ Y = X + atoms;
     const double *__restrict__ Z = X + 2u * atoms;
     double acc = 0.;
-    int zlast = -1, nlast = -1;
-    for (uint32_t a = static_cast<uint32_t>(r); a < atoms; a += L) {
-        const double xa = X[a], ya = Y[a], za = Z[a];
-        double gx = 0., gy = 0., gz = 0.;
-        const uint32_t base = lj_row_start(a, atoms) - a - 1u;
-        for (uint32_t j = 0u; j < atoms; ++j) {
-            if (j == a) continue;
-            const double dx = xa - X[j], dy = ya - Y[j], dz = za - Z[j];
-            const double dist = dx * dx + dy * dy + dz * dz;
-            const double inv = 1.0 / dist;
-            const double sixth = inv * inv * inv;
-            const double coef = (24. * sixth - 48. * sixth * sixth) * inv;
-            gx += coef * dx;
-            gy += coef * dy;
-            gz += coef * dz;
-            if (j > a) {
-                const int k = static_cast<int>(base + j);
-                if (dist == 0.0) {
-                    zlast = max(zlast, k);
-                } else {
-                    const double t = sixth * sixth - sixth;
-                    if (t != t) nlast = max(nlast, k);
-                    acc += t;
-                }
-            }
-        }
-        if (a == 1u) {
-            g[0] = gz;
-        } else if (a == 2u) {
-            g[1] = gy;
-            g[2] = gz;
-        } else if (a >= 3u) {
-            const uint32_t o = 3u * (a - 2u);
-            g[o] = gx;
-            g[o + 1u] = gy;
-            g[o + 2u] = gz;
+    int zlast = -1;
+    if constexpr (APL > 0) {
+        lj_grad_slots<L, APL>(X, Y, Z, atoms, static_cast<uint32_t>(r), acc, zlast, g);
+    } else {
+        for (uint32_t a0 = static_cast<uint32_t>(r); a0 < atoms; a0 += 4u * L) {
+            lj_grad_slots<L, 4>(X, Y, Z, atoms, a0, acc, zlast, g);
         }
     }
-    return lj_finish<L>(acc, zlast, nlast);
+    acc = group_sum<L>(acc);
+    zlast = group_max<L>(zlast);
+    if (zlast >= 0) return lj_zero_rule<L>(X, atoms, r, zlast);
+    return 4 * acc;
 }
 
 } // namespace dev

@@ -212,6 +212,28 @@ __global__ __launch_bounds__(kBlock) void k_emit(const uint64_t *__restrict__ D,
     }
 }
 
+// A thread's next (member, word) items of a level, loaded together so that
+// several global loads are in flight before any bit is processed.
+constexpr uint32_t kPeelBatch = 8u;
+struct peel_batch {
+    uint64_t word[kPeelBatch];
+    uint32_t p[kPeelBatch], w[kPeelBatch];
+    __device__ __forceinline__ void load(const uint64_t *__restrict__ D, const uint64_t *__restrict__ fr,
+                                         uint32_t words, uint64_t it0, uint64_t ie)
+    {
+#pragma unroll
+        for (uint32_t q = 0; q < kPeelBatch; ++q) {
+            const uint64_t it = it0 + q;
+            const bool ok = it < ie;
+            const uint64_t itc = ok ? it : it0;
+            p[q] = static_cast<uint32_t>(itc / words);
+            w[q] = static_cast<uint32_t>(itc - static_cast<uint64_t>(p[q]) * words);
+            const uint64_t v = D[fr[p[q]] * words + w[q]];
+            word[q] = ok ? v : 0u;
+        }
+    }
+};
+
 // The front peel with the counters in LDS (n <= 65535 and 8n bytes of LDS).
 // Level r spreads the (member, word) pairs of front r over the workgroup in
 // order; key[j] = ((r+1) << 16) | p is the largest position p of a member of
@@ -260,41 +282,57 @@ __global__ __launch_bounds__(kPeelBlock) void k_peel_lds(const uint64_t *__restr
         const uint64_t per = (items + kPeelBlock - 1u) / kPeelBlock;
         const uint64_t ib = min(static_cast<uint64_t>(t) * per, items), ie = min(ib + per, items);
         const uint32_t tag = (level + 1u) << 16;
+        const uint64_t *fr = front_items + s;
         // pass 1: decrement the counters of everything front r dominates
-        for (uint64_t it = ib; it < ie; ++it) {
-            const uint32_t p = static_cast<uint32_t>(it / words), w = static_cast<uint32_t>(it - uint64_t(p) * words);
-            uint64_t word = D[front_items[s + p] * words + w];
-            while (word) {
-                const uint32_t j = w * 64u + static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(word)) - 1);
-                atomicSub(&cnt[j], 1u);
-                atomicMax(&key[j], tag | p);
-                word &= word - 1u;
+        for (uint64_t it0 = ib; it0 < ie; it0 += kPeelBatch) {
+            peel_batch b;
+            b.load(D, fr, words, it0, ie);
+#pragma unroll
+            for (uint32_t q = 0; q < kPeelBatch; ++q) {
+                uint64_t word = b.word[q];
+                while (word) {
+                    const uint32_t j =
+                        b.w[q] * 64u + static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(word)) - 1);
+                    atomicSub(&cnt[j], 1u);
+                    atomicMax(&key[j], tag | b.p[q]);
+                    word &= word - 1u;
+                }
             }
         }
         __syncthreads();
         // pass 2: count, in (member, word, bit) order, the elements whose last
         // dominator is this member; pass 3 writes them at their positions
         uint64_t c = 0;
-        for (uint64_t it = ib; it < ie; ++it) {
-            const uint32_t p = static_cast<uint32_t>(it / words), w = static_cast<uint32_t>(it - uint64_t(p) * words);
-            uint64_t word = D[front_items[s + p] * words + w];
-            while (word) {
-                const uint32_t j = w * 64u + static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(word)) - 1);
-                c += (cnt[j] == 0u && key[j] == (tag | p)) ? 1u : 0u;
-                word &= word - 1u;
+        for (uint64_t it0 = ib; it0 < ie; it0 += kPeelBatch) {
+            peel_batch b;
+            b.load(D, fr, words, it0, ie);
+#pragma unroll
+            for (uint32_t q = 0; q < kPeelBatch; ++q) {
+                uint64_t word = b.word[q];
+                while (word) {
+                    const uint32_t j =
+                        b.w[q] * 64u + static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(word)) - 1);
+                    c += (cnt[j] == 0u && key[j] == (tag | b.p[q])) ? 1u : 0u;
+                    word &= word - 1u;
+                }
             }
         }
         uint64_t o = e + block_exclusive_scan(c, sh, total);
-        for (uint64_t it = ib; it < ie; ++it) {
-            const uint32_t p = static_cast<uint32_t>(it / words), w = static_cast<uint32_t>(it - uint64_t(p) * words);
-            uint64_t word = D[front_items[s + p] * words + w];
-            while (word) {
-                const uint32_t j = w * 64u + static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(word)) - 1);
-                if (cnt[j] == 0u && key[j] == (tag | p)) {
-                    front_items[o++] = j;
-                    rank[j] = level + 1u;
+        for (uint64_t it0 = ib; it0 < ie; it0 += kPeelBatch) {
+            peel_batch b;
+            b.load(D, fr, words, it0, ie);
+#pragma unroll
+            for (uint32_t q = 0; q < kPeelBatch; ++q) {
+                uint64_t word = b.word[q];
+                while (word) {
+                    const uint32_t j =
+                        b.w[q] * 64u + static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(word)) - 1);
+                    if (cnt[j] == 0u && key[j] == (tag | b.p[q])) {
+                        front_items[o++] = j;
+                        rank[j] = level + 1u;
+                    }
+                    word &= word - 1u;
                 }
-                word &= word - 1u;
             }
         }
         ++level;
@@ -311,10 +349,6 @@ __device__ __forceinline__ uint32_t ld_relaxed(const uint32_t *p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// The front peel, one workgroup.  cnt (dom_count copy) and key live in global
-// memory and are only touched with atomics (they bypass the non-coherent L1).
-// key[j] is written once, at the level where cnt[j] reaches zero: 1 + the
-// largest position, in the current front, of a member dominating j.
 // The front peel for large n: counters in global memory, touched only with
 // atomics (which bypass the non-coherent L1).
 __global__ __launch_bounds__(kPeelBlock) void k_peel(const uint64_t *__restrict__ D, uint32_t n, uint32_t words,

@@ -208,8 +208,12 @@ __device__ __forceinline__ void eval_separable(const island_desc &di, const tile
     const uint32_t gg = g < tl.count ? g : tl.count - 1u;
     const uint32_t p = tl.first + gg;
     const double *x = c.X + di.xoff + static_cast<uint64_t>(p) * di.nx;
-    const double f = (di.nx % 2u == 0u) ? separable_eval<KIND, L, true>(x, di.nx, static_cast<int>(r))
-                                        : separable_eval<KIND, L, false>(x, di.nx, static_cast<int>(r));
+    double f = (di.nx % 2u == 0u) ? separable_eval<KIND, L, true>(x, di.nx, static_cast<int>(r))
+                                  : separable_eval<KIND, L, false>(x, di.nx, static_cast<int>(r));
+    if (is_recompute_mark(f)) { // group-uniform; an argument beyond the fast cosine's range
+        f = (di.nx % 2u == 0u) ? separable_eval_exact<KIND, L, true>(x, di.nx, static_cast<int>(r))
+                               : separable_eval_exact<KIND, L, false>(x, di.nx, static_cast<int>(r));
+    }
     if (g < tl.count) remember(di, c, p, f, r, L);
 }
 
@@ -228,8 +232,8 @@ __device__ __forceinline__ void eval_sep_dispatch(const island_desc &di, const t
     }
 }
 
-template <int L>
-__device__ __forceinline__ void eval_lj(const island_desc &di, const tile &tl, const eval_ctx &c, double *lds)
+template <int L, uint32_t APL>
+__device__ __forceinline__ void eval_lj_a(const island_desc &di, const tile &tl, const eval_ctx &c, double *lds)
 {
     const uint32_t atoms = di.dim;
     lj_stage(c.X + di.xoff + static_cast<uint64_t>(tl.first) * di.nx, tl.count, atoms, lds);
@@ -237,8 +241,29 @@ __device__ __forceinline__ void eval_lj(const island_desc &di, const tile &tl, c
     const uint32_t g = threadIdx.x / L, r = threadIdx.x % L;
     if (g >= lj_group_count(atoms)) return;
     const uint32_t gg = g < tl.count ? g : tl.count - 1u;
-    const double f = lj_energy<L>(lds + gg * 3u * atoms, atoms, static_cast<int>(r));
+    const double f = lj_energy<L, APL>(lds + gg * 3u * atoms, atoms, static_cast<int>(r));
     if (g < tl.count) remember(di, c, tl.first + gg, f, r, L);
+}
+
+template <int L>
+__device__ __forceinline__ void eval_lj(const island_desc &di, const tile &tl, const eval_ctx &c, double *lds)
+{
+    switch (lj_apl(di.dim, L)) {
+        case 1:
+            eval_lj_a<L, 1>(di, tl, c, lds);
+            return;
+        case 2:
+            eval_lj_a<L, 2>(di, tl, c, lds);
+            return;
+        case 3:
+            eval_lj_a<L, 3>(di, tl, c, lds);
+            return;
+        case 4:
+            eval_lj_a<L, 4>(di, tl, c, lds);
+            return;
+        default:
+            eval_lj_a<L, 0>(di, tl, c, lds);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_eval(const island_desc *__restrict__ d, const tile *__restrict__ tiles,

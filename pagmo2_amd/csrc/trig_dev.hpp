@@ -30,8 +30,8 @@ __host__ __device__ __forceinline__ double reduce_pi(double y, double &k)
 
 __host__ __device__ __forceinline__ double parity_sign(double k, double v)
 {
-    // k is an integer-valued double with |k| < 2^52
-    return (static_cast<long long>(k) & 1) ? -v : v;
+    // k is an integer-valued double with |k| < 2^19 (|y| < 2^20)
+    return (static_cast<int>(k) & 1) ? -v : v;
 }
 
 __host__ __device__ __forceinline__ double cos_reduced(double r)

@@ -69,6 +69,23 @@ def test_separable_rel_1e12(kind, dim, n):
     np.testing.assert_allclose(g, r, rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("kind", [P.RASTRIGIN, P.ACKLEY])
+def test_separable_out_of_range_arguments(kind):
+    """Arguments beyond the fast cosine's range (|2 pi x| >= 2^20), inf and NaN
+    are legal inputs (bounds are never checked, problem.cpp:948-949); those
+    rows are recomputed with the math library's cosine."""
+    x = inputs(kind, 300, 12, 9)
+    x[1, 17] = 3.0e5
+    x[2, 0] = -7.5e9
+    x[3, 299] = 1.0e300
+    x[4, 5] = np.inf
+    x[5, 6] = np.nan
+    x[6, 7] = 166886.0  # just below 2^20 / (2 pi): stays on the fast path
+    g = gpu_bfe(kind, 300, x)
+    r = O.bfe(kind, 300, x, threads=4)
+    np.testing.assert_allclose(g, r, rtol=1e-12, atol=0, equal_nan=True)
+
+
 def test_separable_unaligned_rows_same_order():
     # an odd row offset selects the unaligned load path; the reduction order
     # (and so every bit of the result) must not change
