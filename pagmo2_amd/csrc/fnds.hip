@@ -15,13 +15,14 @@
 //   k_emit      dom_list[i] = set bits of row i in increasing j: exactly the
 //               reference's push_back order (multi_objective.cpp:215-227 appends
 //               j < i while scanning row i, then larger indices in scan order)
-//   k_peel      the front peel (multi_objective.cpp:228-254) in one workgroup:
-//               F0 ascending; then per level, decrement the counters of the
-//               rows dominated by the current front, mark the elements whose
-//               counter reached zero with the position (in the current front)
-//               of their last dominator, and emit them ordered by (that
-//               position, index) -- the order in which the reference's nested
-//               loop appends them.
+//   k_front0 +  the front peel (multi_objective.cpp:228-254), n <= 32768:
+//   k_peel_grid F0 ascending; then per level, one pass over the whole grid
+//               decrements the counters of the rows dominated by the current
+//               front and records for each the position (in the current front)
+//               of its last dominator; the elements whose counter reached zero
+//               are ordered by (that position, index) -- the order in which the
+//               reference's nested loop appends them -- in one workgroup.
+//   k_peel      the same peel in one workgroup for larger n.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,7 +38,8 @@ struct pgh_fnds_ws {
     uint64_t n = 0, m = 0, words = 0;
     pgh::dev_buf keys, D, cnt32, ndom32, key32, dom_count, rank, front_items, front_offsets, dom_offsets, dom_items,
         scalars;
-    uint64_t h_sizes[2] = {0, 0}; // n_fronts, n_dom_items
+    pgh::dev_buf comp, ctl, lvl;
+    uint64_t h_sizes[3] = {0, 0, 0}; // n_fronts, n_dom_items, peel barrier timeout
     bool sized = false;
 };
 
@@ -212,138 +214,6 @@ __global__ __launch_bounds__(kBlock) void k_emit(const uint64_t *__restrict__ D,
     }
 }
 
-// A thread's next (member, word) items of a level, loaded together so that
-// several global loads are in flight before any bit is processed.
-constexpr uint32_t kPeelBatch = 8u;
-struct peel_batch {
-    uint64_t word[kPeelBatch];
-    uint32_t p[kPeelBatch], w[kPeelBatch];
-    __device__ __forceinline__ void load(const uint64_t *__restrict__ D, const uint64_t *__restrict__ fr,
-                                         uint32_t words, uint64_t it0, uint64_t ie)
-    {
-#pragma unroll
-        for (uint32_t q = 0; q < kPeelBatch; ++q) {
-            const uint64_t it = it0 + q;
-            const bool ok = it < ie;
-            const uint64_t itc = ok ? it : it0;
-            p[q] = static_cast<uint32_t>(itc / words);
-            w[q] = static_cast<uint32_t>(itc - static_cast<uint64_t>(p[q]) * words);
-            const uint64_t v = D[fr[p[q]] * words + w[q]];
-            word[q] = ok ? v : 0u;
-        }
-    }
-};
-
-// The front peel with the counters in LDS (n <= 65535 and 8n bytes of LDS).
-// Level r spreads the (member, word) pairs of front r over the workgroup in
-// order; key[j] = ((r+1) << 16) | p is the largest position p of a member of
-// front r dominating j, tagged with the level so that values left by earlier
-// levels never win.
-__global__ __launch_bounds__(kPeelBlock) void k_peel_lds(const uint64_t *__restrict__ D, uint32_t n, uint32_t words,
-                                                         const uint32_t *__restrict__ dom_count,
-                                                         uint64_t *__restrict__ front_items,
-                                                         uint64_t *__restrict__ front_offsets,
-                                                         uint64_t *__restrict__ rank, uint64_t *__restrict__ scalars)
-{
-    extern __shared__ uint32_t lds_u32[];
-    uint32_t *cnt = lds_u32;
-    uint32_t *key = lds_u32 + n;
-    __shared__ uint64_t sh[kPeelBlock / 64u];
-    const uint32_t t = threadIdx.x;
-    for (uint32_t j = t; j < n; j += kPeelBlock) {
-        cnt[j] = dom_count[j];
-        key[j] = 0u;
-    }
-    __syncthreads();
-    uint64_t total;
-    {
-        // F0: ascending indices with dom_count == 0 (multi_objective.cpp:228-233)
-        const uint32_t per = (n + kPeelBlock - 1u) / kPeelBlock;
-        const uint32_t b = min(t * per, n), e = min(b + per, n);
-        uint64_t c = 0;
-        for (uint32_t i = b; i < e; ++i) c += cnt[i] == 0u;
-        uint64_t o = block_exclusive_scan(c, sh, total);
-        for (uint32_t i = b; i < e; ++i) {
-            if (cnt[i] == 0u) {
-                front_items[o++] = i;
-                rank[i] = 0;
-            }
-        }
-        if (t == 0) {
-            front_offsets[0] = 0;
-            front_offsets[1] = total;
-        }
-    }
-    uint64_t s = 0, e = total;
-    uint32_t level = 0;
-    __syncthreads();
-    while (e > s) {
-        const uint64_t items = (e - s) * words;
-        const uint64_t per = (items + kPeelBlock - 1u) / kPeelBlock;
-        const uint64_t ib = min(static_cast<uint64_t>(t) * per, items), ie = min(ib + per, items);
-        const uint32_t tag = (level + 1u) << 16;
-        const uint64_t *fr = front_items + s;
-        // pass 1: decrement the counters of everything front r dominates
-        for (uint64_t it0 = ib; it0 < ie; it0 += kPeelBatch) {
-            peel_batch b;
-            b.load(D, fr, words, it0, ie);
-#pragma unroll
-            for (uint32_t q = 0; q < kPeelBatch; ++q) {
-                uint64_t word = b.word[q];
-                while (word) {
-                    const uint32_t j =
-                        b.w[q] * 64u + static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(word)) - 1);
-                    atomicSub(&cnt[j], 1u);
-                    atomicMax(&key[j], tag | b.p[q]);
-                    word &= word - 1u;
-                }
-            }
-        }
-        __syncthreads();
-        // pass 2: count, in (member, word, bit) order, the elements whose last
-        // dominator is this member; pass 3 writes them at their positions
-        uint64_t c = 0;
-        for (uint64_t it0 = ib; it0 < ie; it0 += kPeelBatch) {
-            peel_batch b;
-            b.load(D, fr, words, it0, ie);
-#pragma unroll
-            for (uint32_t q = 0; q < kPeelBatch; ++q) {
-                uint64_t word = b.word[q];
-                while (word) {
-                    const uint32_t j =
-                        b.w[q] * 64u + static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(word)) - 1);
-                    c += (cnt[j] == 0u && key[j] == (tag | b.p[q])) ? 1u : 0u;
-                    word &= word - 1u;
-                }
-            }
-        }
-        uint64_t o = e + block_exclusive_scan(c, sh, total);
-        for (uint64_t it0 = ib; it0 < ie; it0 += kPeelBatch) {
-            peel_batch b;
-            b.load(D, fr, words, it0, ie);
-#pragma unroll
-            for (uint32_t q = 0; q < kPeelBatch; ++q) {
-                uint64_t word = b.word[q];
-                while (word) {
-                    const uint32_t j =
-                        b.w[q] * 64u + static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(word)) - 1);
-                    if (cnt[j] == 0u && key[j] == (tag | b.p[q])) {
-                        front_items[o++] = j;
-                        rank[j] = level + 1u;
-                    }
-                    word &= word - 1u;
-                }
-            }
-        }
-        ++level;
-        if (t == 0 && total) front_offsets[level + 1u] = e + total;
-        __syncthreads();
-        s = e;
-        e = e + total;
-    }
-    if (t == 0) scalars[0] = level;
-}
-
 __device__ __forceinline__ uint32_t ld_relaxed(const uint32_t *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -458,31 +328,236 @@ __global__ __launch_bounds__(kPeelBlock) void k_peel(const uint64_t *__restrict_
     if (t == 0) scalars[0] = level;
 }
 
+
+// ---- grid-wide peel (n <= 32768) --------------------------------------------
+// Level L turns front L into front L+1.  Its (member, word) items are claimed
+// in chunks by any workgroup through claim[L]; each set bit decrements the
+// counter of the dominated element and raises its key to ((L+1) << 16 |
+// position of the member), and the decrement that reaches zero appends the
+// element to a completion list.  done[L] counts processed items: the
+// workgroup whose addition completes the level orders the list by (position,
+// index) -- the order of the reference's nested loop -- writes front L+1 and
+// publishes its end offset.  No workgroup ever waits for another one to
+// arrive, only for a level to be published by a workgroup that is running, so
+// the kernel needs no co-residency: workgroups that start late join the
+// current level or find the work done.  Every wait is still bounded (a spin
+// limit sets a flag the host reports) so that no wave can loop forever.
+// Control words: ctl[0] completion count of the current level, ctl[1] number
+// of published fronts (front_offsets[0..ctl[1]] are final), ctl[2] set once
+// the last front is known.  lvl[2L] = claim[L], lvl[2L+1] = done[L].
+constexpr uint32_t kGridBlock = 1024;
+constexpr uint32_t kGridMaxN = 32768;
+constexpr uint32_t kChunk = 2048;         // items per claim
+constexpr uint32_t kSpinLimit = 1u << 26; // ~2 s of s_sleep(1) polls
+
+__device__ __forceinline__ uint64_t ld_relaxed64(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_release(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// F0 in ascending order (multi_objective.cpp:228-233) and the control words.
+__global__ __launch_bounds__(kPeelBlock) void k_front0(const uint32_t *__restrict__ dom_count, uint32_t n,
+                                                       uint64_t *__restrict__ front_items,
+                                                       uint64_t *__restrict__ front_offsets,
+                                                       uint64_t *__restrict__ rank, uint32_t *__restrict__ ctl)
+{
+    __shared__ uint64_t sh[kPeelBlock / 64u];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (n + kPeelBlock - 1u) / kPeelBlock;
+    const uint32_t b = min(t * per, n), e = min(b + per, n);
+    uint64_t c = 0;
+    for (uint32_t i = b; i < e; ++i) c += dom_count[i] == 0u;
+    uint64_t total;
+    uint64_t o = block_exclusive_scan(c, sh, total);
+    for (uint32_t i = b; i < e; ++i) {
+        if (dom_count[i] == 0u) {
+            front_items[o++] = i;
+            rank[i] = 0;
+        }
+    }
+    if (t == 0) {
+        front_offsets[0] = 0;
+        front_offsets[1] = total;
+        ctl[0] = 0;
+        ctl[1] = 1;
+        ctl[2] = 0;
+    }
+}
+
+// Bitonic sort of sh[0..P) (P a power of two) by one workgroup.
+__device__ __forceinline__ void block_bitonic_sort(uint32_t *sh, uint32_t P)
+{
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
+                const uint32_t l = i ^ j;
+                if (l > i) {
+                    const uint32_t a = sh[i], b = sh[l];
+                    const bool up = (i & k) == 0;
+                    if ((a > b) == up) {
+                        sh[i] = b;
+                        sh[l] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Front L+1 from the m completed elements of level L (the workgroup that
+// completed the level, after acquiring every other workgroup's writes).
+__device__ void publish_front(uint32_t L, uint32_t e, uint32_t n, uint32_t *sort_lds,
+                              const uint32_t *comp, const uint32_t *key, uint64_t *front_items,
+                              uint64_t *front_offsets, uint64_t *rank, uint32_t *ctl, uint64_t *scalars)
+{
+    const uint32_t t = threadIdx.x;
+    const uint32_t m = min(ld_relaxed(&ctl[0]), n - e);
+    // sort keys (position << 16 | index): index < n <= 32768 and position <
+    // front size <= 32768, so the keys are distinct
+    auto sort_key = [&](uint32_t i) {
+        const uint32_t j = min(ld_relaxed(&comp[i]), n - 1u);
+        return ((ld_relaxed(&key[j]) & 0xffffu) << 16) | j;
+    };
+    if (m <= kGridBlock) {
+        // rank of each key among the m keys: one pass, no stages
+        if (t < m) sort_lds[t] = sort_key(t);
+        __syncthreads();
+        if (t < m) {
+            const uint32_t v = sort_lds[t];
+            uint32_t r = 0;
+            for (uint32_t i = 0; i < m; ++i) r += sort_lds[i] < v;
+            front_items[e + r] = v & 0xffffu;
+            rank[v & 0xffffu] = L + 1u;
+        }
+    } else {
+        uint32_t P = 1;
+        while (P < m) P <<= 1;
+        for (uint32_t i = t; i < P; i += blockDim.x) sort_lds[i] = i < m ? sort_key(i) : 0xffffffffu;
+        __syncthreads();
+        block_bitonic_sort(sort_lds, P);
+        for (uint32_t i = t; i < m; i += blockDim.x) {
+            const uint32_t j = sort_lds[i] & 0xffffu;
+            front_items[e + i] = j;
+            rank[j] = L + 1u;
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        if (m) {
+            front_offsets[L + 2u] = e + m;
+            ctl[0] = 0;
+            st_release(&ctl[1], L + 2u);
+        } else {
+            scalars[0] = L + 1u; // number of fronts
+            st_release(&ctl[2], 1u);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kGridBlock) void k_peel_grid(const uint64_t *__restrict__ D, uint32_t n, uint32_t words,
+                                                          uint32_t *__restrict__ cnt, uint32_t *__restrict__ key,
+                                                          uint32_t *__restrict__ comp,
+                                                          uint64_t *__restrict__ front_items,
+                                                          uint64_t *__restrict__ front_offsets,
+                                                          uint64_t *__restrict__ rank, uint32_t *__restrict__ ctl,
+                                                          uint32_t *__restrict__ lvl, uint64_t *__restrict__ scalars)
+{
+    extern __shared__ uint32_t sort_lds[];
+    __shared__ uint32_t sh_word[2];
+    const uint32_t t = threadIdx.x;
+    for (uint32_t L = 0; L < n; ++L) {
+        // wait until front L is published (or the last front is known)
+        if (t == 0) {
+            uint32_t state = 0, spins = 0;
+            for (;;) {
+                if (ld_relaxed(&ctl[1]) > L) {
+                    state = 1;
+                    break;
+                }
+                if (ld_relaxed(&ctl[2])) break;
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins == kSpinLimit) {
+                    atomicOr(reinterpret_cast<unsigned long long *>(&scalars[2]), 1ull);
+                    break;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            sh_word[0] = state;
+        }
+        __syncthreads();
+        if (!sh_word[0]) return;
+        // n <= 32768: at most 2^15 * 2^9 (member, word) items per level
+        const uint32_t s = static_cast<uint32_t>(ld_relaxed64(&front_offsets[L]));
+        const uint32_t e = static_cast<uint32_t>(ld_relaxed64(&front_offsets[L + 1u]));
+        // items: 16-bit pieces of the rows of front L (four per word), so
+        // that a level spreads over more lanes; each lane issues the atomics
+        // of up to four set bits before it waits for their results
+        const uint32_t pieces = 4u * words;
+        const uint32_t items = (e - s) * pieces;
+        const uint32_t tag = (L + 1u) << 16;
+        for (;;) {
+            __syncthreads();
+            if (t == 0) sh_word[1] = atomicAdd(&lvl[2u * L], kChunk);
+            __syncthreads();
+            const uint32_t c0 = sh_word[1];
+            if (c0 >= items) break;
+            const uint32_t c1 = min(c0 + kChunk, items);
+            for (uint32_t it = c0 + t; it < c1; it += kGridBlock) {
+                const uint32_t p = it / pieces, q = it - p * pieces, w = q >> 2;
+                uint32_t bits = static_cast<uint32_t>(D[ld_relaxed64(&front_items[s + p]) * words + w] >> (16u * (q & 3u)))
+                                & 0xffffu;
+                const uint32_t base = 16u * q;
+                while (bits) {
+                    uint32_t j[4], old[4];
+                    bool on[4];
+#pragma unroll
+                    for (uint32_t b = 0; b < 4u; ++b) {
+                        on[b] = bits != 0u;
+                        j[b] = base + static_cast<uint32_t>(__ffs(bits) - 1);
+                        bits &= bits - 1u;
+                    }
+#pragma unroll
+                    for (uint32_t b = 0; b < 4u; ++b) {
+                        if (on[b]) {
+                            atomicMax(&key[j[b]], tag | p);
+                            old[b] = atomicSub(&cnt[j[b]], 1u);
+                        }
+                    }
+#pragma unroll
+                    for (uint32_t b = 0; b < 4u; ++b) {
+                        if (on[b] && old[b] == 1u) {
+                            const uint32_t slot = atomicAdd(&ctl[0], 1u);
+                            if (slot < n) comp[slot] = j[b];
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            if (t == 0) {
+                const uint32_t done = c1 - c0;
+                const uint32_t before =
+                    __hip_atomic_fetch_add(&lvl[2u * L + 1u], done, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t last = before + done == items;
+                if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                sh_word[0] = last;
+            }
+            __syncthreads();
+            if (sh_word[0]) {
+                publish_front(L, e, n, sort_lds, comp, key, front_items, front_offsets, rank, ctl, scalars);
+                break;
+            }
+        }
+    }
+}
+
 uint32_t grid_for(uint64_t count)
 {
     return static_cast<uint32_t>((count + kBlock - 1u) / kBlock);
-}
-
-// Dynamic LDS a workgroup of k_peel_lds may use on the current device (the
-// static scan scratch aside), with the kernel's limit raised to it once.
-size_t max_dynamic_lds()
-{
-    static thread_local int dev_cached = -1;
-    static thread_local size_t bytes = 0;
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 0;
-    if (dev != dev_cached) {
-        int v = 0;
-        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 65536;
-        bytes = v > 1024 ? static_cast<size_t>(v) - 1024u : 0u;
-        if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_peel_lds), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                static_cast<int>(bytes)) != hipSuccess) {
-            (void)hipGetLastError();
-            bytes = 65536u - 1024u;
-        }
-        dev_cached = dev;
-    }
-    return bytes;
 }
 
 #define PGH_TRY(expr)                                                                                                \
@@ -490,6 +565,45 @@ size_t max_dynamic_lds()
         const pgh_status pgh_st_ = (expr);                                                                           \
         if (pgh_st_ != PGH_OK) return pgh_st_;                                                                       \
     } while (0)
+
+// Launches k_front0 and the grid-wide peel.
+pgh_status launch_grid_peel(pgh_fnds_ws *ws, hipStream_t s, uint32_t n32, uint32_t w32)
+{
+    static thread_local int dev_cached = -1;
+    static thread_local uint32_t cus = 0;
+    int dev = 0;
+    PGH_HIP(hipGetDevice(&dev));
+    if (dev != dev_cached) {
+        int v = 0;
+        PGH_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev));
+        // the sort of a level may use up to 128 KiB of dynamic LDS
+        PGH_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_peel_grid),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 4 * static_cast<int>(kGridMaxN)));
+        cus = static_cast<uint32_t>(std::max(v, 1));
+        dev_cached = dev;
+    }
+    uint32_t P = 1;
+    while (P < n32) P <<= 1;
+    const size_t lds = static_cast<size_t>(std::max(P, kGridBlock)) * 4u;
+    // one workgroup per 512 elements, at least 8 and at most one per CU: the
+    // items of a level rarely fill more
+    const uint32_t blocks = std::min(cus, std::max(8u, n32 / 512u));
+    auto *cnt = static_cast<uint32_t *>(ws->cnt32.ptr);
+    auto *key = static_cast<uint32_t *>(ws->key32.ptr);
+    auto *comp = static_cast<uint32_t *>(ws->comp.ptr);
+    auto *ctl = static_cast<uint32_t *>(ws->ctl.ptr);
+    auto *lvl = static_cast<uint32_t *>(ws->lvl.ptr);
+    auto *fi = static_cast<uint64_t *>(ws->front_items.ptr);
+    auto *fo = static_cast<uint64_t *>(ws->front_offsets.ptr);
+    auto *rk = static_cast<uint64_t *>(ws->rank.ptr);
+    auto *sc = static_cast<uint64_t *>(ws->scalars.ptr);
+    k_fill_u32<<<grid_for(2u * n32), kBlock, 0, s>>>(lvl, 2u * n32, 0u);
+    k_front0<<<1, kPeelBlock, 0, s>>>(cnt, n32, fi, fo, rk, ctl);
+    k_peel_grid<<<blocks, kGridBlock, lds, s>>>(static_cast<const uint64_t *>(ws->D.ptr), n32, w32, cnt, key, comp,
+                                                 fi, fo, rk, ctl, lvl, sc);
+    PGH_HIP(hipGetLastError());
+    return PGH_OK;
+}
 
 } // namespace
 } // namespace pgh
@@ -559,7 +673,7 @@ pgh_status pgh_fnds_run(pgh_fnds_ws *ws, const double *objs, uint64_t n, uint64_
     PGH_TRY(ws->front_items.reserve(n * 8u));
     PGH_TRY(ws->front_offsets.reserve((n + 1u) * 8u));
     PGH_TRY(ws->dom_offsets.reserve((n + 1u) * 8u));
-    PGH_TRY(ws->scalars.reserve(16u));
+    PGH_TRY(ws->scalars.reserve(32u));
     auto *keys = static_cast<int64_t *>(ws->keys.ptr);
     auto *D = static_cast<uint64_t *>(ws->D.ptr);
     auto *cnt = static_cast<uint32_t *>(ws->cnt32.ptr);
@@ -573,6 +687,7 @@ pgh_status pgh_fnds_run(pgh_fnds_ws *ws, const double *objs, uint64_t n, uint64_
     k_fill_u32<<<grid_for(n), kBlock, 0, s>>>(cnt, n, 0u);
     k_fill_u32<<<grid_for(n), kBlock, 0, s>>>(ndom, n, 0u);
     k_fill_u32<<<grid_for(n), kBlock, 0, s>>>(key, n, 0u);
+    k_fill_u32<<<1, kBlock, 0, s>>>(static_cast<uint32_t *>(ws->scalars.ptr) + 4, 2u, 0u); // timeout flag
     PGH_HIP(hipGetLastError());
     // columns per workgroup: a multiple of 64 whose keys fit 32 KiB of LDS
     uint32_t jc = 1024u;
@@ -601,11 +716,11 @@ pgh_status pgh_fnds_run(pgh_fnds_ws *ws, const double *objs, uint64_t n, uint64_
     auto *scalars = static_cast<uint64_t *>(ws->scalars.ptr);
     k_scan<<<1, kPeelBlock, 0, s>>>(ndom, cnt, n32, dom_offsets, static_cast<uint64_t *>(ws->dom_count.ptr), scalars);
     PGH_HIP(hipGetLastError());
-    const size_t peel_lds = 8u * static_cast<size_t>(n);
-    if (n < 65536u && peel_lds <= max_dynamic_lds()) {
-        k_peel_lds<<<1, kPeelBlock, peel_lds, s>>>(D, n32, w32, cnt, static_cast<uint64_t *>(ws->front_items.ptr),
-                                                   static_cast<uint64_t *>(ws->front_offsets.ptr),
-                                                   static_cast<uint64_t *>(ws->rank.ptr), scalars);
+    if (n <= kGridMaxN) {
+        PGH_TRY(ws->comp.reserve(n * 4u));
+        PGH_TRY(ws->ctl.reserve(64u));
+        PGH_TRY(ws->lvl.reserve(n * 8u));
+        PGH_TRY(launch_grid_peel(ws, s, n32, w32));
     } else {
         k_peel<<<1, kPeelBlock, 0, s>>>(D, n32, w32, cnt, key, static_cast<uint64_t *>(ws->front_items.ptr),
                                         static_cast<uint64_t *>(ws->front_offsets.ptr),
@@ -619,8 +734,9 @@ pgh_status pgh_fnds_sizes(pgh_fnds_ws *ws, uint64_t *n_fronts, uint64_t *n_dom_i
 {
     if (!ws || !ws->n) return fail(PGH_INVALID_ARGUMENT, "fnds workspace has no result");
     if (!ws->sized) {
-        PGH_HIP(hipMemcpyAsync(ws->h_sizes, ws->scalars.ptr, 16u, hipMemcpyDeviceToHost, ws->stream));
+        PGH_HIP(hipMemcpyAsync(ws->h_sizes, ws->scalars.ptr, 24u, hipMemcpyDeviceToHost, ws->stream));
         PGH_HIP(hipStreamSynchronize(ws->stream));
+        if (ws->h_sizes[2]) return fail(PGH_DEVICE_ERROR, "fast_non_dominated_sorting: a grid barrier timed out");
         // dom_items is written once its size is known
         const uint64_t total = ws->h_sizes[1];
         PGH_TRY(ws->dom_items.reserve(std::max<uint64_t>(total, 1u) * 8u));

@@ -112,3 +112,33 @@ def test_ranking_workload_duplicates():
     dup = n // 10
     objs[n - dup:] = objs[rng.integers(0, n - dup, dup)]
     assert_same(objs)
+
+
+def layered(n, layers, delta, seed):
+    """n points on `layers` parallel anti-diagonals x + y = c_L (c_L = L * delta),
+    rows shuffled: a point dominates the points of later layers whose x lies
+    within (L' - L) * delta above its own, so fronts are wide and each element
+    has few dominators at varied positions."""
+    rng = np.random.default_rng(seed)
+    lay = np.arange(n) % layers
+    x = rng.uniform(0, 1, n)
+    objs = np.stack([x, lay * delta - x], axis=1)
+    return objs[rng.permutation(n)]
+
+
+def test_cooperative_peel_at_its_size_limit():
+    # n = 32768 is the largest size the grid-wide peel takes (16-bit positions
+    # and indices in its sort keys); one level completes thousands of elements
+    assert_same(layered(32768, 8, 0.002, 5))
+
+
+def test_cooperative_peel_one_dominator_of_all():
+    # F1 = every other point, all completed by the one member of F0
+    x = np.linspace(0, 1, 20000)
+    objs = np.stack([x, 1 - x], axis=1)
+    objs[7777] = (-1.0, -1.0)
+    assert_same(objs)
+
+
+def test_single_workgroup_peel_beyond_cooperative_limit():
+    assert_same(layered(40000, 20, 0.0005, 6))
