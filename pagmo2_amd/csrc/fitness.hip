@@ -79,7 +79,7 @@ __global__ __launch_bounds__(kBlock) void k_lj(const double *__restrict__ dvs, u
     const uint32_t g = threadIdx.x / L;
     const int r = static_cast<int>(threadIdx.x % L);
     const uint32_t gg = g < count ? g : count - 1u;
-    const double e = lj_energy<L, APL>(lds + gg * 3u * atoms, atoms, r);
+    const double e = lj_energy<L, APL>(lds + gg * lj_stride(atoms), atoms, r);
     if (g < count && r == 0) fvs[ind0 + g] = e;
 }
 
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(kBlock) void k_lj_grad(const double *__restrict__ d
     const uint32_t g = threadIdx.x / L;
     const int r = static_cast<int>(threadIdx.x % L);
     if (g >= count) return; // whole groups retire together: no shuffle partner is lost
-    const double e = lj_energy_grad<L, APL>(lds + g * 3u * atoms, atoms, r, grad + (ind0 + g) * nx);
+    const double e = lj_energy_grad<L, APL>(lds + g * lj_stride(atoms), atoms, r, grad + (ind0 + g) * nx);
     if (fvs && r == 0) fvs[ind0 + g] = e;
 }
 
@@ -157,7 +157,7 @@ pgh_status launch_lj_la(const double *dvs, uint64_t n, uint32_t atoms, double *f
 {
     const uint32_t G = lj_group_count(atoms);
     const uint64_t blocks = (n + G - 1u) / G;
-    const size_t lds = static_cast<size_t>(G) * 3u * atoms * sizeof(double);
+    const size_t lds = static_cast<size_t>(G) * lj_stride(atoms) * sizeof(double);
     if (grad) {
         k_lj_grad<L, APL><<<dim3(static_cast<uint32_t>(blocks)), dim3(G * L), lds, s>>>(dvs, n, atoms, G, grad,
                                                                                         fvs);

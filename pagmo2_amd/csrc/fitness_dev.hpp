@@ -88,14 +88,22 @@ __host__ __device__ constexpr int lj_lanes(uint32_t atoms)
     return want >= 16u ? 16 : (want >= 8u ? 8 : (want >= 4u ? 4 : (want >= 2u ? 2 : 1)));
 }
 
-// Individuals per workgroup for Lennard-Jones: the LDS holds 3*atoms doubles
-// per individual and a workgroup uses at most 64 KiB.
+// Doubles of LDS per staged Lennard-Jones individual: the atoms' (x, y, z)
+// twice over (see lj_stage).
+__host__ __device__ constexpr uint32_t lj_stride(uint32_t atoms)
+{
+    return 6u * atoms;
+}
+
+// Individuals per workgroup for Lennard-Jones: a workgroup uses at most 64 KiB
+// of LDS.
 __host__ __device__ constexpr uint32_t lj_group_count(uint32_t atoms)
 {
     const uint32_t L = static_cast<uint32_t>(lj_lanes(atoms));
+    const uint32_t bytes = 8u * lj_stride(atoms);
     uint32_t g = 256u / L;
-    while (g > 1u && g * 24u * atoms > 65536u) g >>= 1;
-    return (g * 24u * atoms > 65536u) ? 0u : g;
+    while (g > 1u && g * bytes > 65536u) g >>= 1;
+    return (g * bytes > 65536u) ? 0u : g;
 }
 
 // ---------------------------------------------------------------------------
@@ -134,36 +142,38 @@ __device__ __forceinline__ int group_max(int v)
 // and 2c+1) and accumulates them in increasing order; the L partial results
 // are then combined by group_sum/group_prod.  ALIGNED selects 16-byte loads
 // (even nx and a 16-byte aligned row) without changing that order.
-// FAST: rastrigin and ackley take the cosine from cos_pi_reduced and only
-// record whether an argument fell outside its range (|y| >= 2^20, inf, NaN);
-// the caller then evaluates the whole individual again with FAST = false.
+// FAST (rastrigin and ackley): the cosines come from cos_2pi_small, valid for
+// |x| < 2^10; the caller checks at the end, from the lane's sum of squares,
+// that every element was in that range (else the individual is evaluated
+// again with FAST = false, the reference's expression with an accurate
+// cosine).
 template <int KIND, bool FAST = false>
 struct sep_acc {
     double a = 0., b = 0.;
-    bool out_of_range = false;
     __device__ __forceinline__ void init()
     {
         a = 0.;
         b = (KIND == GRIEWANK) ? 1. : 0.;
     }
-    __device__ __forceinline__ double cosine(double y)
-    {
-        if constexpr (FAST) {
-            out_of_range |= !(fabs(y) < 0x1p20);
-            return cos_pi_reduced(y);
-        } else {
-            return cos_any(y);
-        }
-    }
     __device__ __forceinline__ void add(double xi, uint32_t i)
     {
         if constexpr (KIND == RASTRIGIN) {
-            // rastrigin.cpp:67
-            a += xi * xi - 10. * cosine(kOmega * xi);
+            if constexpr (FAST) {
+                a = fma(xi, xi, a);
+                a -= cos_2pi_small<true>(xi);
+            } else {
+                // rastrigin.cpp:67
+                a += xi * xi - 10. * cos_any(kOmega * xi);
+            }
         } else if constexpr (KIND == ACKLEY) {
-            // ackley.cpp:69-70
-            a += xi * xi;
-            b += cosine(kOmega * xi);
+            if constexpr (FAST) {
+                a = fma(xi, xi, a);
+                b += cos_2pi_small<false>(xi);
+            } else {
+                // ackley.cpp:69-70
+                a += xi * xi;
+                b += cos_any(kOmega * xi);
+            }
         } else if constexpr (KIND == SCHWEFEL) {
             // schwefel.cpp:64
             a += xi * sin(sqrt(fabs(xi)));
@@ -174,6 +184,17 @@ struct sep_acc {
             b *= cos(xi / sqrt(static_cast<double>(i) + 1.0));
         }
         (void)i;
+    }
+    // FAST: true when every element added had |x| < 2^10, known from the sum
+    // of squares (< 2^20); rastrigin's a holds sum(x^2 - 10 cos), at least
+    // sum(x^2) - 10 * elements.  NaN and inf fail the test.
+    __device__ __forceinline__ bool in_fast_range(uint32_t elements) const
+    {
+        if constexpr (KIND == RASTRIGIN) {
+            return a + 10. * static_cast<double>(elements) < 0x1p20;
+        } else {
+            return a < 0x1p20;
+        }
     }
 };
 
@@ -192,7 +213,7 @@ __device__ __forceinline__ void separable_pair(sep_acc<KIND, FAST> &acc, const d
     }
 }
 
-template <int KIND, int L, bool ALIGNED, bool FAST, int UNROLL = 1>
+template <int KIND, int L, bool ALIGNED, bool FAST>
 __device__ __forceinline__ double separable_eval_impl(const double *__restrict__ x, uint32_t nx, int r,
                                                       bool &out_of_range)
 {
@@ -200,36 +221,27 @@ __device__ __forceinline__ double separable_eval_impl(const double *__restrict__
     acc.init();
     const uint32_t npairs = (nx + 1u) >> 1;
     uint32_t c = static_cast<uint32_t>(r);
-    if constexpr (ALIGNED && UNROLL == 2) {
-        for (; c + L < npairs; c += 2u * L) {
-            const double2 v0 = *reinterpret_cast<const double2 *>(x + 2u * c);
-            const double2 v1 = *reinterpret_cast<const double2 *>(x + 2u * (c + L));
-            acc.add(v0.x, 2u * c);
-            acc.add(v0.y, 2u * c + 1u);
-            acc.add(v1.x, 2u * (c + L));
-            acc.add(v1.y, 2u * (c + L) + 1u);
+    if constexpr (ALIGNED) {
+        // the next pair's load is issued before the current pair is used
+        if (c < npairs) {
+            double2 v = *reinterpret_cast<const double2 *>(x + 2u * c);
+            for (; c + L < npairs; c += L) {
+                const double2 w = *reinterpret_cast<const double2 *>(x + 2u * (c + L));
+                acc.add(v.x, 2u * c);
+                acc.add(v.y, 2u * c + 1u);
+                v = w;
+            }
+            acc.add(v.x, 2u * c);
+            acc.add(v.y, 2u * c + 1u);
         }
+    } else {
+        for (; c < npairs; c += L) separable_pair<KIND, L, ALIGNED, FAST>(acc, x, nx, c);
     }
-    if constexpr (ALIGNED && UNROLL == 4) {
-        // four 16-byte loads in flight per lane; the pairs are still added in
-        // increasing c
-        for (; c + 3u * L < npairs; c += 4u * L) {
-            const double2 v0 = *reinterpret_cast<const double2 *>(x + 2u * c);
-            const double2 v1 = *reinterpret_cast<const double2 *>(x + 2u * (c + L));
-            const double2 v2 = *reinterpret_cast<const double2 *>(x + 2u * (c + 2u * L));
-            const double2 v3 = *reinterpret_cast<const double2 *>(x + 2u * (c + 3u * L));
-            acc.add(v0.x, 2u * c);
-            acc.add(v0.y, 2u * c + 1u);
-            acc.add(v1.x, 2u * (c + L));
-            acc.add(v1.y, 2u * (c + L) + 1u);
-            acc.add(v2.x, 2u * (c + 2u * L));
-            acc.add(v2.y, 2u * (c + 2u * L) + 1u);
-            acc.add(v3.x, 2u * (c + 3u * L));
-            acc.add(v3.y, 2u * (c + 3u * L) + 1u);
-        }
+    if constexpr (FAST) {
+        out_of_range = !acc.in_fast_range(2u * ((npairs + L - 1u) / L));
+    } else {
+        out_of_range = false;
     }
-    for (; c < npairs; c += L) separable_pair<KIND, L, ALIGNED, FAST>(acc, x, nx, c);
-    out_of_range = acc.out_of_range;
     const double n = static_cast<double>(nx);
     if constexpr (KIND == RASTRIGIN) {
         // rastrigin.cpp:69
@@ -263,8 +275,8 @@ __device__ __forceinline__ bool is_recompute_mark(double f)
 // Fitness of one individual by the L lanes of its group (lane r); the same
 // value is returned to every lane.  The per-element operations and the
 // reduction tree depend on nx and L only.  For rastrigin and ackley the
-// cosines come from cos_pi_reduced; an individual with an argument outside its
-// range (|y| >= 2^20, inf, NaN) gets the recompute mark instead, and
+// cosines come from cos_2pi_small; an individual with an element outside its
+// range (|x| >= 2^10, inf, NaN) gets the recompute mark instead, and
 // separable_eval_exact produces its value (kept out of this routine so that
 // its registers do not weigh on the streaming kernel).
 template <int KIND, int L, bool ALIGNED>
@@ -355,26 +367,34 @@ __device__ __forceinline__ void lj_slot(uint32_t e, uint32_t &atom, uint32_t &co
     }
 }
 
-// Stages `count` individuals (rows of nx = 3*atoms-6) into LDS as per-
-// individual structure-of-arrays: [x0..x_{A-1}, y0.., z0..].  Called by all
-// threads of the workgroup; a __syncthreads() must follow.
+// Stages `count` individuals (rows of nx = 3*atoms-6) into LDS, lj_stride
+// doubles each: atom a's (x, y, z) at [3a, 3a+3) and again at [3(a+A),
+// 3(a+A)+3), so that a partner a + k (k < A) is read without a wrap and at a
+// constant offset from atom a.  Called by all threads of the workgroup; a
+// __syncthreads() must follow.
 __device__ __forceinline__ void lj_stage(const double *__restrict__ rows, uint32_t count, uint32_t atoms,
                                          double *__restrict__ lds)
 {
     const uint32_t nx = 3u * atoms - 6u;
-    const uint32_t stride = 3u * atoms;
+    const uint32_t stride = lj_stride(atoms);
+    const uint32_t copy = 3u * atoms;
     for (uint32_t t = threadIdx.x; t < count * 6u; t += blockDim.x) {
         // the six fixed coordinates: atom 0 (x,y,z), atom 1 (x,y), atom 2 (x)
         const uint32_t g = t / 6u, f = t % 6u;
         const uint32_t atom = f < 3u ? 0u : (f < 5u ? 1u : 2u);
         const uint32_t coord = f < 3u ? f : (f < 5u ? f - 3u : 0u);
-        lds[g * stride + coord * atoms + atom] = 0.0;
+        double *dst = lds + g * stride + 3u * atom + coord;
+        dst[0] = 0.0;
+        dst[copy] = 0.0;
     }
     for (uint32_t t = threadIdx.x; t < count * nx; t += blockDim.x) {
         const uint32_t g = t / nx, e = t - g * nx;
         uint32_t atom, coord;
         lj_slot(e, atom, coord);
-        lds[g * stride + coord * atoms + atom] = rows[static_cast<uint64_t>(g) * nx + e];
+        const double v = rows[static_cast<uint64_t>(g) * nx + e];
+        double *dst = lds + g * stride + 3u * atom + coord;
+        dst[0] = v;
+        dst[copy] = v;
     }
 }
 
@@ -391,30 +411,39 @@ __device__ __forceinline__ int lj_pair_index(uint32_t a, uint32_t b, uint32_t at
     return static_cast<int>(lj_row_start(i, atoms) + (j - i - 1u));
 }
 
-// 1 / d for the sixth power of a distance: v_rcp_f64 refined by two Newton
-// steps, and the IEEE division where the refinement is not valid (d zero,
-// subnormal, infinite or NaN).
+// 1 / d by v_rcp_f64 refined with two Newton steps: accurate to about an ulp
+// for d in [2^-1000, 2^1000]; outside that range (0, subnormal, huge, inf,
+// NaN) the result is not 1 / d, and callers that can meet such d use lj_recip.
+__device__ __forceinline__ double lj_recip_fast(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    return fma(r, fma(-d, r, 1.0), r);
+}
+
+// 1 / d for any d: lj_recip_fast where it is valid, IEEE division otherwise.
 __device__ __forceinline__ double lj_recip(double d)
 {
-    if (d > 0x1p-1000 && d < 0x1p1000) {
-        double r = __builtin_amdgcn_rcp(d);
-        r = fma(r, fma(-d, r, 1.0), r);
-        return fma(r, fma(-d, r, 1.0), r);
-    }
+    if (d > 0x1p-1000 && d < 0x1p1000) return lj_recip_fast(d);
     return 1.0 / d;
 }
 
 // Cyclic pair schedule of an individual's atoms over the L lanes of its group:
 // lane r owns atoms a = r, r+L, ...; step k (1 <= k <= A/2) pairs each owned
-// atom a with b = (a + k) mod A, and when A is even the last step keeps only
-// a < A/2, so every unordered pair is visited exactly once.  Within a step the
-// lanes of a group read consecutive atoms.
+// atom a with b = a + k (mod A, read from the second copy), and when A is even
+// the last step keeps only a < A/2, so every unordered pair is visited exactly
+// once.  Within a step the lanes of a group read consecutive atoms.
 struct lj_cycle {
     uint32_t atoms, half;
     __device__ __forceinline__ explicit lj_cycle(uint32_t a) : atoms(a), half(a / 2u) {}
     __device__ __forceinline__ bool counts(uint32_t a, uint32_t k) const
     {
         return 2u * k < atoms || a < half; // k == A/2 with A even: one of the two visits
+    }
+    // steps in which every visit counts
+    __device__ __forceinline__ uint32_t full_steps() const
+    {
+        return (atoms & 1u) ? half : half - 1u;
     }
     __device__ __forceinline__ uint32_t partner(uint32_t a, uint32_t k) const
     {
@@ -423,38 +452,122 @@ struct lj_cycle {
     }
 };
 
+// ---- careful paths: any input, the reference's special cases -----------------
+
 // The reference's dist == 0 rule (lennard_jones.cpp:81-83,89): the result is
 // +inf unless a NaN term comes after the last zero-distance pair in the
 // reference's pair order, then NaN.  Only reached when some pair of the
 // individual has zero distance; recomputes the group's terms to find the NaNs.
 template <int L>
-__device__ __noinline__ double lj_zero_rule(const double *__restrict__ X, uint32_t atoms, int r, int zlast)
+__device__ __noinline__ double lj_zero_rule(const double *__restrict__ P, uint32_t atoms, int r, int zlast)
 {
-    const double *__restrict__ Y = X + atoms;
-    const double *__restrict__ Z = X + 2u * atoms;
     const lj_cycle cyc(atoms);
     int nlast = -1;
     for (uint32_t k = 1u; k <= cyc.half; ++k) {
         for (uint32_t a = static_cast<uint32_t>(r); a < atoms; a += L) {
             if (!cyc.counts(a, k)) continue;
-            const uint32_t b = cyc.partner(a, k);
-            const double dx = X[a] - X[b], dy = Y[a] - Y[b], dz = Z[a] - Z[b];
+            const double *pa = P + 3u * a, *pb = pa + 3u * k;
+            const double dx = pa[0] - pb[0], dy = pa[1] - pb[1], dz = pa[2] - pb[2];
             const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
             if (r2 == 0.0) continue;
             const double s = lj_recip(r2 * r2 * r2);
             const double t = fma(s, s, -s);
-            if (t != t) nlast = max(nlast, lj_pair_index(a, b, atoms));
+            if (t != t) nlast = max(nlast, lj_pair_index(a, cyc.partner(a, k), atoms));
         }
     }
     nlast = group_max<L>(nlast);
     return nlast > zlast ? __builtin_nan("") : __builtin_inf();
 }
 
+// Energy with every special case: lj_recip's full range, zero distances
+// (excluded from the sum and recorded for lj_zero_rule) and masked visits.
+template <int L>
+__device__ __noinline__ double lj_energy_careful(const double *__restrict__ P, uint32_t atoms, int r)
+{
+    const lj_cycle cyc(atoms);
+    double acc = 0.;
+    int zlast = -1;
+    for (uint32_t k = 1u; k <= cyc.half; ++k) {
+        for (uint32_t a = static_cast<uint32_t>(r); a < atoms; a += L) {
+            if (!cyc.counts(a, k)) continue;
+            const double *pa = P + 3u * a, *pb = pa + 3u * k;
+            const double dx = pa[0] - pb[0], dy = pa[1] - pb[1], dz = pa[2] - pb[2];
+            const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+            if (r2 == 0.0) {
+                zlast = max(zlast, lj_pair_index(a, cyc.partner(a, k), atoms));
+                continue;
+            }
+            const double s = lj_recip(r2 * r2 * r2);
+            acc += fma(s, s, -s);
+        }
+    }
+    acc = group_sum<L>(acc);
+    zlast = group_max<L>(zlast);
+    if (zlast >= 0) return lj_zero_rule<L>(P, atoms, r, zlast);
+    return 4 * acc;
+}
+
+// Writes the gradient components of atom a (its force f) into the gradient row
+// g (nx doubles) through the encoding of lennard_jones.cpp:132-151: atom 0 is
+// fixed, atom 1 moves along z only, atom 2 in y and z.
+__device__ __forceinline__ void lj_write_grad(uint32_t a, double fx, double fy, double fz, double *__restrict__ g)
+{
+    if (a == 1u) {
+        g[0] = fz;
+    } else if (a == 2u) {
+        g[1] = fy;
+        g[2] = fz;
+    } else if (a >= 3u) {
+        const uint32_t o = 3u * (a - 2u);
+        g[o] = fx;
+        g[o + 1u] = fy;
+        g[o + 2u] = fz;
+    }
+}
+
+// Energy and gradient with every special case (see lj_energy_careful).
+template <int L>
+__device__ __noinline__ double lj_energy_grad_careful(const double *__restrict__ P, uint32_t atoms, int r,
+                                                      double *__restrict__ g)
+{
+    const lj_cycle cyc(atoms);
+    double acc = 0.;
+    int zlast = -1;
+    for (uint32_t a = static_cast<uint32_t>(r); a < atoms; a += L) {
+        const double *pa = P + 3u * a;
+        double fx = 0., fy = 0., fz = 0.;
+        for (uint32_t k = 1u; k < atoms; ++k) {
+            const double *pb = pa + 3u * k;
+            const double dx = pa[0] - pb[0], dy = pa[1] - pb[1], dz = pa[2] - pb[2];
+            const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+            const double s = lj_recip(r2 * r2 * r2);
+            const double inv_r2 = (s * r2) * r2;
+            const double coef = (fma(-48., s, 24.) * s) * inv_r2;
+            fx = fma(coef, dx, fx);
+            fy = fma(coef, dy, fy);
+            fz = fma(coef, dz, fz);
+            if (k <= cyc.half && cyc.counts(a, k)) {
+                if (r2 == 0.0) {
+                    zlast = max(zlast, lj_pair_index(a, cyc.partner(a, k), atoms));
+                } else {
+                    acc += fma(s, s, -s);
+                }
+            }
+        }
+        lj_write_grad(a, fx, fy, fz, g);
+    }
+    acc = group_sum<L>(acc);
+    zlast = group_max<L>(zlast);
+    if (zlast >= 0) return lj_zero_rule<L>(P, atoms, r, zlast);
+    return 4 * acc;
+}
+
+// ---- streaming paths ------------------------------------------------------------
+
 // Lane r of a group owns atoms a_q = r + q L.  APL > 0: exactly APL owned
 // slots (the ceiling of atoms / L), all held in registers; APL == 0: any
-// count, taken four slots at a time.  Owned slots beyond the last atom are
-// masked.  Processing a lane's slots together gives each step of the
-// schedule several independent chains of arithmetic.
+// count, taken four slots at a time.  A slot past the last atom repeats atom r
+// with weight 0, so that no visit needs a mask.
 __host__ __device__ constexpr uint32_t lj_apl(uint32_t atoms, int L)
 {
     const uint32_t apl = (atoms + static_cast<uint32_t>(L) - 1u) / static_cast<uint32_t>(L);
@@ -463,141 +576,152 @@ __host__ __device__ constexpr uint32_t lj_apl(uint32_t atoms, int L)
 
 template <int L, uint32_t NS>
 struct lj_slots {
+    const double *p[NS];
+    double x[NS], y[NS], z[NS], w[NS];
     uint32_t a[NS];
-    bool own[NS];
-    double x[NS], y[NS], z[NS];
-    __device__ __forceinline__ void load(const double *__restrict__ X, const double *__restrict__ Y,
-                                         const double *__restrict__ Z, uint32_t a0, uint32_t atoms)
+    __device__ __forceinline__ void load(const double *__restrict__ P, uint32_t a0, uint32_t r, uint32_t atoms)
     {
 #pragma unroll
         for (uint32_t q = 0; q < NS; ++q) {
             a[q] = a0 + q * L;
-            own[q] = a[q] < atoms;
-            if (!own[q]) a[q] = a0;
-            x[q] = X[a[q]];
-            y[q] = Y[a[q]];
-            z[q] = Z[a[q]];
+            const bool own = a[q] < atoms;
+            p[q] = P + 3u * (own ? a[q] : r);
+            x[q] = p[q][0];
+            y[q] = p[q][1];
+            z[q] = p[q][2];
+            w[q] = own ? 1.0 : 0.0;
         }
     }
 };
 
+// One visit: the pair (owned slot, partner at pb).  r2 feeds the zero check.
+struct lj_pair {
+    double dx, dy, dz, r2, s;
+    __device__ __forceinline__ lj_pair(double x, double y, double z, const double *__restrict__ pb)
+    {
+        dx = x - pb[0];
+        dy = y - pb[1];
+        dz = z - pb[2];
+        r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        s = lj_recip_fast(r2 * r2 * r2);
+    }
+};
+
 template <int L, uint32_t NS>
-__device__ __forceinline__ void lj_energy_slots(const double *__restrict__ X, const double *__restrict__ Y,
-                                                const double *__restrict__ Z, uint32_t atoms, uint32_t a0,
-                                                double &acc, int &zlast)
+__device__ __forceinline__ void lj_energy_slots(const double *__restrict__ P, uint32_t atoms, uint32_t a0, uint32_t r,
+                                                double &acc, double &rmin)
 {
     const lj_cycle cyc(atoms);
     lj_slots<L, NS> sl;
-    sl.load(X, Y, Z, a0, atoms);
-    for (uint32_t k = 1u; k <= cyc.half; ++k) {
+    sl.load(P, a0, r, atoms);
+    const uint32_t full = cyc.full_steps();
+    for (uint32_t k = 1u; k <= full; ++k) {
 #pragma unroll
         for (uint32_t q = 0; q < NS; ++q) {
-            const uint32_t b = cyc.partner(sl.a[q], k);
-            const double dx = sl.x[q] - X[b], dy = sl.y[q] - Y[b], dz = sl.z[q] - Z[b];
-            const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-            const bool valid = sl.own[q] && cyc.counts(sl.a[q], k);
-            const double s = lj_recip(r2 * r2 * r2);
-            const double t = fma(s, s, -s);
-            acc += (valid && r2 != 0.0) ? t : 0.0;
-            if (valid && r2 == 0.0) zlast = max(zlast, lj_pair_index(sl.a[q], b, atoms));
+            const lj_pair v(sl.x[q], sl.y[q], sl.z[q], sl.p[q] + 3u * k);
+            acc = fma(sl.w[q], fma(v.s, v.s, -v.s), acc);
+            rmin = fmin(rmin, v.r2);
+        }
+    }
+    if (full < cyc.half) {
+        // A even, k = A/2: only a < A/2 counts
+#pragma unroll
+        for (uint32_t q = 0; q < NS; ++q) {
+            const lj_pair v(sl.x[q], sl.y[q], sl.z[q], sl.p[q] + 3u * cyc.half);
+            const double w = sl.a[q] < cyc.half ? sl.w[q] : 0.0;
+            acc = fma(w, fma(v.s, v.s, -v.s), acc);
+            rmin = fmin(rmin, v.r2);
         }
     }
 }
 
-// Energy of the individual staged at X (Y = X + A, Z = X + 2A) by the L lanes
-// of a group (lane r): sum of 4 (s^2 - s), s = r^-6, over the cyclic
-// schedule, then the group's butterfly sum.
-template <int L, uint32_t APL>
-__device__ __forceinline__ double lj_energy(const double *__restrict__ X, uint32_t atoms, int r)
+template <int L>
+__device__ __forceinline__ double group_min(double v)
 {
-    const double *__restrict__ Y = X + atoms;
-    const double *__restrict__ Z = X + 2u * atoms;
-    double acc = 0.;
-    int zlast = -1;
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, L));
+    return v;
+}
+
+// Energy of the individual staged at P by the L lanes of a group (lane r):
+// sum of 4 (s^2 - s), s = r^-6, over the cyclic schedule, then the group's
+// butterfly sum.  The streaming loop has no masks and no range checks; a
+// zero or non-finite distance, or a non-finite sum, sends the individual to
+// lj_energy_careful, which handles the reference's special cases.
+template <int L, uint32_t APL>
+__device__ __forceinline__ double lj_energy(const double *__restrict__ P, uint32_t atoms, int r)
+{
+    double acc = 0., rmin = __builtin_inf();
     if constexpr (APL > 0) {
-        lj_energy_slots<L, APL>(X, Y, Z, atoms, static_cast<uint32_t>(r), acc, zlast);
+        lj_energy_slots<L, APL>(P, atoms, static_cast<uint32_t>(r), static_cast<uint32_t>(r), acc, rmin);
     } else {
         for (uint32_t a0 = static_cast<uint32_t>(r); a0 < atoms; a0 += 4u * L) {
-            lj_energy_slots<L, 4>(X, Y, Z, atoms, a0, acc, zlast);
+            lj_energy_slots<L, 4>(P, atoms, a0, static_cast<uint32_t>(r), acc, rmin);
         }
     }
     acc = group_sum<L>(acc);
-    zlast = group_max<L>(zlast);
-    if (zlast >= 0) return lj_zero_rule<L>(X, atoms, r, zlast);
+    rmin = group_min<L>(rmin);
+    // every r2 > 2^-333 keeps r2^3 normal; r2 < 2^333 keeps it finite
+    const bool fast_ok = rmin > 0x1p-333 && acc - acc == 0.0;
+    if (!fast_ok) return lj_energy_careful<L>(P, atoms, r);
     return 4 * acc;
 }
 
 template <int L, uint32_t NS>
-__device__ __forceinline__ void lj_grad_slots(const double *__restrict__ X, const double *__restrict__ Y,
-                                              const double *__restrict__ Z, uint32_t atoms, uint32_t a0,
-                                              double &acc, int &zlast, double *__restrict__ g)
+__device__ __forceinline__ void lj_grad_slots(const double *__restrict__ P, uint32_t atoms, uint32_t a0, uint32_t r,
+                                              double &acc, double &rmin, double &fsum, double *__restrict__ g)
 {
-    const lj_cycle cyc(atoms);
     lj_slots<L, NS> sl;
-    sl.load(X, Y, Z, a0, atoms);
-    double gx[NS], gy[NS], gz[NS];
+    sl.load(P, a0, r, atoms);
+    double fx[NS], fy[NS], fz[NS];
 #pragma unroll
-    for (uint32_t q = 0; q < NS; ++q) gx[q] = gy[q] = gz[q] = 0.;
+    for (uint32_t q = 0; q < NS; ++q) fx[q] = fy[q] = fz[q] = 0.;
     for (uint32_t k = 1u; k < atoms; ++k) {
 #pragma unroll
         for (uint32_t q = 0; q < NS; ++q) {
-            const uint32_t b = cyc.partner(sl.a[q], k);
-            const double dx = sl.x[q] - X[b], dy = sl.y[q] - Y[b], dz = sl.z[q] - Z[b];
-            const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-            const double s = lj_recip(r2 * r2 * r2);
-            const double inv_r2 = (s * r2) * r2;
-            const double coef = (fma(-48., s, 24.) * s) * inv_r2;
-            gx[q] = fma(coef, dx, gx[q]);
-            gy[q] = fma(coef, dy, gy[q]);
-            gz[q] = fma(coef, dz, gz[q]);
-            const bool valid = sl.own[q] && k <= cyc.half && cyc.counts(sl.a[q], k);
-            const double t = fma(s, s, -s);
-            acc += (valid && r2 != 0.0) ? t : 0.0;
-            if (valid && r2 == 0.0) zlast = max(zlast, lj_pair_index(sl.a[q], b, atoms));
+            const lj_pair v(sl.x[q], sl.y[q], sl.z[q], sl.p[q] + 3u * k);
+            const double inv_r2 = (v.s * v.r2) * v.r2;
+            const double coef = (fma(-48., v.s, 24.) * v.s) * inv_r2;
+            fx[q] = fma(coef, v.dx, fx[q]);
+            fy[q] = fma(coef, v.dy, fy[q]);
+            fz[q] = fma(coef, v.dz, fz[q]);
+            // every pair is visited from both of its atoms: half weight each
+            acc = fma(sl.w[q], fma(v.s, v.s, -v.s), acc);
+            rmin = fmin(rmin, v.r2);
         }
     }
 #pragma unroll
     for (uint32_t q = 0; q < NS; ++q) {
-        if (!sl.own[q]) continue;
-        const uint32_t a = sl.a[q];
-        if (a == 1u) {
-            g[0] = gz[q];
-        } else if (a == 2u) {
-            g[1] = gy[q];
-            g[2] = gz[q];
-        } else if (a >= 3u) {
-            const uint32_t o = 3u * (a - 2u);
-            g[o] = gx[q];
-            g[o + 1u] = gy[q];
-            g[o + 2u] = gz[q];
-        }
+        fsum += fabs(fx[q]) + fabs(fy[q]) + fabs(fz[q]); // inf or NaN if a force is
+        if (sl.w[q] != 0.0) lj_write_grad(sl.a[q], fx[q], fy[q], fz[q], g);
     }
 }
 
 // Energy and gradient: lane r owns atoms a = r, r+L, ... and visits every
-// other atom b = (a + k) mod A, k = 1..A-1, accumulating the force
+// other atom a + k, k = 1..A-1, accumulating the force
 // dE/dp_a = sum_b (p_a - p_b) (24 s - 48 s^2) / r^2 (s = r^-6; 1/r^2 = s r^4)
-// and, on the steps the cyclic schedule counts, the energy terms.  The
-// gradient row (nx doubles) is written to g; returns the energy.
+// and the energy terms of all its visits (each pair twice, so the sum is
+// halved).  The gradient row (nx doubles) is written to g; returns the
+// energy.  A zero or non-finite distance, or a non-finite result, sends the
+// individual to lj_energy_grad_careful.
 template <int L, uint32_t APL>
-__device__ __forceinline__ double lj_energy_grad(const double *__restrict__ X, uint32_t atoms, int r,
+__device__ __forceinline__ double lj_energy_grad(const double *__restrict__ P, uint32_t atoms, int r,
                                                  double *__restrict__ g)
 {
-    const double *__restrict__ Y = X + atoms;
-    const double *__restrict__ Z = X + 2u * atoms;
-    double acc = 0.;
-    int zlast = -1;
+    double acc = 0., rmin = __builtin_inf(), fsum = 0.;
     if constexpr (APL > 0) {
-        lj_grad_slots<L, APL>(X, Y, Z, atoms, static_cast<uint32_t>(r), acc, zlast, g);
+        lj_grad_slots<L, APL>(P, atoms, static_cast<uint32_t>(r), static_cast<uint32_t>(r), acc, rmin, fsum, g);
     } else {
         for (uint32_t a0 = static_cast<uint32_t>(r); a0 < atoms; a0 += 4u * L) {
-            lj_grad_slots<L, 4>(X, Y, Z, atoms, a0, acc, zlast, g);
+            lj_grad_slots<L, 4>(P, atoms, a0, static_cast<uint32_t>(r), acc, rmin, fsum, g);
         }
     }
     acc = group_sum<L>(acc);
-    zlast = group_max<L>(zlast);
-    if (zlast >= 0) return lj_zero_rule<L>(X, atoms, r, zlast);
-    return 4 * acc;
+    rmin = group_min<L>(rmin);
+    fsum = group_sum<L>(fsum);
+    const bool fast_ok = rmin > 0x1p-333 && acc - acc == 0.0 && fsum - fsum == 0.0;
+    if (!fast_ok) return lj_energy_grad_careful<L>(P, atoms, r, g);
+    return 2 * acc;
 }
 
 } // namespace dev

@@ -241,7 +241,7 @@ __device__ __forceinline__ void eval_lj_a(const island_desc &di, const tile &tl,
     const uint32_t g = threadIdx.x / L, r = threadIdx.x % L;
     if (g >= lj_group_count(atoms)) return;
     const uint32_t gg = g < tl.count ? g : tl.count - 1u;
-    const double f = lj_energy<L, APL>(lds + gg * 3u * atoms, atoms, static_cast<int>(r));
+    const double f = lj_energy<L, APL>(lds + gg * lj_stride(atoms), atoms, static_cast<int>(r));
     if (g < tl.count) remember(di, c, tl.first + gg, f, r, L);
 }
 
@@ -491,7 +491,7 @@ pgh_status pgh_archi_create(pgh_archi **out, const pgh_island_spec *islands, uin
             di.lanes = static_cast<uint32_t>(lj_lanes(p.dim));
             per = lj_group_count(p.dim);
             if (per == 0u) return fail(PGH_UNSUPPORTED, "lennard_jones cluster too large for one workgroup");
-            a->lds_bytes = std::max<size_t>(a->lds_bytes, static_cast<size_t>(per) * 3u * p.dim * sizeof(double));
+            a->lds_bytes = std::max<size_t>(a->lds_bytes, static_cast<size_t>(per) * lj_stride(p.dim) * sizeof(double));
         } else if (p.kind != PGH_ROSENBROCK) {
             di.lanes = static_cast<uint32_t>(sep_lanes(p.nx));
             per = 256u / di.lanes;

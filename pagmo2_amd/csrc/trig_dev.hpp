@@ -1,6 +1,7 @@
 // trig_dev.hpp -- cos and sin of moderate arguments for the separable kernels.
 //
-// Written from the mathematics: y = k*pi + r with k = rint(y / pi) and
+// cos_2pi_small (the rastrigin/ackley streaming kernels) is described where it
+// is defined.  The general routines are written from the mathematics: y = k*pi + r with k = rint(y / pi) and
 // |r| <= pi/2 (two-constant reduction evaluated with fused multiply-adds, after
 // Cody & Waite), cos(y) = (-1)^k cos(r), sin(y) = (-1)^k sin(r), and the
 // Taylor series of cos(r) through r^22 and of sin(r) through r^23 in Horner
@@ -11,6 +12,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <cstdint>
 
 namespace pgh
 {
@@ -82,6 +85,42 @@ __host__ __device__ __forceinline__ double sin_pi_reduced(double y)
     double k;
     const double r = reduce_pi(y, k);
     return parity_sign(k, sin_reduced(r));
+}
+
+// S * cos(2 pi x) (S = 1 or 10) for moderate x (callers keep |x| < 2^10; the
+// identity holds for |x| < 2^50).  2x = k + u with k = rint(2x) and |u| <= 1/2,
+// u = 2x - k exact in binary64, so cos(2 pi x) = (-1)^k cos(pi u): no
+// multiplication by pi before the reduction, hence no reduction error.  k is
+// formed by adding 1.5 * 2^52 (rounding 2x to an integer, ties to even, in the
+// low mantissa bits), whose lowest bit is then the parity of k.  cos(pi u) is
+// the degree-8 polynomial in u^2 of smallest maximum error on [0, 1/4]
+// (gen_cospi_coeffs.py: 3.9e-18 before rounding, about 3e-16 in binary64).
+// Compared with the reference's cos(omega * x) the result differs by the
+// rounding of omega * x there (at most |2 pi x| * 1.5e-16).
+constexpr double kCosPi[9] = {0x1.0000000000000p+0,  -0x1.3bd3cc9be45dbp+2, 0x1.03c1f081b5993p+2,
+                              -0x1.55d3c7e3bfc44p+0, 0x1.e1f506813c9d9p-3,  -0x1.a6d1efc968c10p-6,
+                              0x1.f9d25488fb381p-10, -0x1.b69582d77d94dp-14, 0x1.16797132c8c4dp-18};
+constexpr double kCosPi10[9] = {0x1.4000000000000p+3,  -0x1.8ac8bfc2dd752p+5, 0x1.44b26ca222ff8p+5,
+                                -0x1.ab48b9dcafb55p+3, 0x1.2d392410c5e28p+1,  -0x1.084335dde178ap-2,
+                                0x1.3c2374d59d030p-6,  -0x1.121d71c6ae7d0p-10, 0x1.5c17cd7f7af60p-15};
+
+template <bool TIMES10>
+__device__ __forceinline__ double cos_2pi_small(double x)
+{
+    constexpr double kShift = 0x1.8p52;
+    const double *c = TIMES10 ? kCosPi10 : kCosPi;
+    const double t = fma(x, 2.0, kShift);
+    const double k = t - kShift;
+    const double u = fma(x, 2.0, -k);
+    const double z = u * u;
+    double p = c[8];
+#pragma unroll
+    for (int i = 7; i >= 0; --i) p = fma(p, z, c[i]);
+    // negate p when k is odd: adding bit 0 of t at bit 31 of p's high word
+    // flips its sign bit (one shift-and-add)
+    const uint32_t t_lo = static_cast<uint32_t>(__double2loint(t));
+    const uint32_t p_hi = static_cast<uint32_t>(__double2hiint(p)) + (t_lo << 31);
+    return __hiloint2double(static_cast<int>(p_hi), __double2loint(p));
 }
 
 // Any argument: the reduction above when it is accurate, the math library's

@@ -71,16 +71,22 @@ def test_separable_rel_1e12(kind, dim, n):
 
 @pytest.mark.parametrize("kind", [P.RASTRIGIN, P.ACKLEY])
 def test_separable_out_of_range_arguments(kind):
-    """Arguments beyond the fast cosine's range (|2 pi x| >= 2^20), inf and NaN
-    are legal inputs (bounds are never checked, problem.cpp:948-949); those
-    rows are recomputed with the math library's cosine."""
-    x = inputs(kind, 300, 12, 9)
+    """Arguments beyond the streaming kernel's cosine range (|x| >= 2^10), inf
+    and NaN are legal inputs (bounds are never checked, problem.cpp:948-949);
+    those rows are recomputed with the reference's expression and an accurate
+    cosine."""
+    x = inputs(kind, 300, 14, 9)
     x[1, 17] = 3.0e5
     x[2, 0] = -7.5e9
     x[3, 299] = 1.0e300
     x[4, 5] = np.inf
     x[5, 6] = np.nan
-    x[6, 7] = 166886.0  # just below 2^20 / (2 pi): stays on the fast path
+    x[6, 7] = 166886.0  # just below 2^20 / (2 pi): the accurate cosine's own fast reduction
+    x[7, 8] = 1024.5    # just past the streaming range
+    x[8, :] = 1023.75   # inside it, but the lanes' sums of squares exceed 2^20
+    x[9, 9] = -np.inf
+    x[10, :] = 400.3    # streaming range, identical elements: their errors add up
+    x[11, :] = -400.3
     g = gpu_bfe(kind, 300, x)
     r = O.bfe(kind, 300, x, threads=4)
     np.testing.assert_allclose(g, r, rtol=1e-12, atol=0, equal_nan=True)
