@@ -16,9 +16,25 @@ namespace
 constexpr int kBlock = 256;
 
 // ---- separable problems: an L-lane group per individual -------------------
+// For rastrigin and ackley an individual with an element outside the streaming
+// cosine's range comes back marked; one lane then evaluates it again with the
+// reference's expression and an accurate cosine (rare).  That path stays out
+// of line and the kernel is held to 64 VGPRs (8 waves per SIMD), so that its
+// register needs do not lower the streaming loop's occupancy; it may spill.
+template <int KIND, bool ALIGNED>
+__device__ __noinline__ double separable_cold(const double *__restrict__ x, uint32_t nx)
+{
+    return separable_eval_exact<KIND, 1, ALIGNED>(x, nx, 0);
+}
+
+constexpr int sep_min_waves(int kind)
+{
+    return (kind == RASTRIGIN || kind == ACKLEY) ? 8 : 1;
+}
+
 template <int KIND, int L, bool ALIGNED>
-__global__ __launch_bounds__(kBlock) void k_separable(const double *__restrict__ dvs, uint64_t n, uint32_t nx,
-                                                      double *__restrict__ fvs)
+__global__ __launch_bounds__(kBlock, sep_min_waves(KIND)) void k_separable(const double *__restrict__ dvs, uint64_t n,
+                                                                           uint32_t nx, double *__restrict__ fvs)
 {
     constexpr uint32_t G = kBlock / L;
     const uint64_t ind = static_cast<uint64_t>(blockIdx.x) * G + threadIdx.x / L;
@@ -26,21 +42,11 @@ __global__ __launch_bounds__(kBlock) void k_separable(const double *__restrict__
     // a trailing partial group re-reads the last row so that every lane of the
     // group takes part in the shuffles; its result is dropped
     const uint64_t row = ind < n ? ind : n - 1u;
-    const double f = separable_eval<KIND, L, ALIGNED>(dvs + row * nx, nx, r);
+    double f = separable_eval<KIND, L, ALIGNED>(dvs + row * nx, nx, r);
+    if constexpr (KIND == RASTRIGIN || KIND == ACKLEY) {
+        if (r == 0 && is_recompute_mark(f)) f = separable_cold<KIND, ALIGNED>(dvs + row * nx, nx);
+    }
     if (ind < n && r == 0) fvs[ind] = f;
-}
-
-// Recomputes, with the math library's cosine and the reference's sequential
-// order, the individuals that k_separable marked (an argument outside the
-// fast reduction's range).  One lane per individual: the scan of the marks is
-// a single coalesced pass.
-template <int KIND, bool ALIGNED>
-__global__ __launch_bounds__(kBlock) void k_separable_fix(const double *__restrict__ dvs, uint64_t n, uint32_t nx,
-                                                          double *__restrict__ fvs)
-{
-    const uint64_t ind = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    if (ind >= n || !is_recompute_mark(fvs[ind])) return;
-    fvs[ind] = separable_eval_exact<KIND, 1, ALIGNED>(dvs + ind * nx, nx, 0);
 }
 
 // ---- rosenbrock: one lane per individual ------------------------------------
@@ -126,14 +132,10 @@ pgh_status launch_sep_l(const double *dvs, uint64_t n, uint32_t nx, double *fvs,
     const uint64_t blocks = (n + G - 1u) / G;
     const bool aligned = (nx % 2u == 0u) && (reinterpret_cast<uintptr_t>(dvs) % 16u == 0u);
     const dim3 grid(static_cast<uint32_t>(blocks));
-    const dim3 fix_grid(static_cast<uint32_t>((n + kBlock - 1u) / kBlock));
-    constexpr bool fixable = (KIND == RASTRIGIN || KIND == ACKLEY);
     if (aligned) {
         k_separable<KIND, L, true><<<grid, dim3(kBlock), 0, s>>>(dvs, n, nx, fvs);
-        if (fixable) k_separable_fix<KIND, true><<<fix_grid, dim3(kBlock), 0, s>>>(dvs, n, nx, fvs);
     } else {
         k_separable<KIND, L, false><<<grid, dim3(kBlock), 0, s>>>(dvs, n, nx, fvs);
-        if (fixable) k_separable_fix<KIND, false><<<fix_grid, dim3(kBlock), 0, s>>>(dvs, n, nx, fvs);
     }
     PGH_HIP(hipGetLastError());
     return PGH_OK;
