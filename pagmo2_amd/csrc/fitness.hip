@@ -80,12 +80,12 @@ __global__ __launch_bounds__(kBlock) void k_lj(const double *__restrict__ dvs, u
     const uint32_t nx = 3u * atoms - 6u;
     const uint64_t ind0 = static_cast<uint64_t>(blockIdx.x) * G;
     const uint32_t count = static_cast<uint32_t>(min(static_cast<uint64_t>(G), n - ind0));
-    lj_stage(dvs + ind0 * nx, count, atoms, lds);
+    lj_stage(dvs + ind0 * nx, count, atoms, false, lds);
     __syncthreads();
     const uint32_t g = threadIdx.x / L;
     const int r = static_cast<int>(threadIdx.x % L);
     const uint32_t gg = g < count ? g : count - 1u;
-    const double e = lj_energy<L, APL>(lds + gg * lj_stride(atoms), atoms, r);
+    const double e = lj_energy<L, APL>(lds + gg * lj_stride(atoms, false), atoms, r);
     if (g < count && r == 0) fvs[ind0 + g] = e;
 }
 
@@ -97,12 +97,12 @@ __global__ __launch_bounds__(kBlock) void k_lj_grad(const double *__restrict__ d
     const uint32_t nx = 3u * atoms - 6u;
     const uint64_t ind0 = static_cast<uint64_t>(blockIdx.x) * G;
     const uint32_t count = static_cast<uint32_t>(min(static_cast<uint64_t>(G), n - ind0));
-    lj_stage(dvs + ind0 * nx, count, atoms, lds);
+    lj_stage(dvs + ind0 * nx, count, atoms, true, lds);
     __syncthreads();
     const uint32_t g = threadIdx.x / L;
     const int r = static_cast<int>(threadIdx.x % L);
     if (g >= count) return; // whole groups retire together: no shuffle partner is lost
-    const double e = lj_energy_grad<L, APL>(lds + g * lj_stride(atoms), atoms, r, grad + (ind0 + g) * nx);
+    const double e = lj_energy_grad<L, APL>(lds + g * lj_stride(atoms, true), atoms, r, grad + (ind0 + g) * nx);
     if (fvs && r == 0) fvs[ind0 + g] = e;
 }
 
@@ -144,9 +144,11 @@ pgh_status launch_sep_l(const double *dvs, uint64_t n, uint32_t nx, double *fvs,
 template <int KIND>
 pgh_status launch_sep(const double *dvs, uint64_t n, uint32_t nx, double *fvs, hipStream_t s)
 {
-    switch (sep_lanes(nx)) {
+    switch (sep_lanes(KIND, nx)) {
         case 64:
             return launch_sep_l<KIND, 64>(dvs, n, nx, fvs, s);
+        case 32:
+            return launch_sep_l<KIND, 32>(dvs, n, nx, fvs, s);
         case 16:
             return launch_sep_l<KIND, 16>(dvs, n, nx, fvs, s);
         default:
@@ -157,9 +159,10 @@ pgh_status launch_sep(const double *dvs, uint64_t n, uint32_t nx, double *fvs, h
 template <int L, uint32_t APL>
 pgh_status launch_lj_la(const double *dvs, uint64_t n, uint32_t atoms, double *fvs, double *grad, hipStream_t s)
 {
-    const uint32_t G = lj_group_count(atoms);
+    const bool with_grad = grad != nullptr;
+    const uint32_t G = lj_group_count(atoms, with_grad);
     const uint64_t blocks = (n + G - 1u) / G;
-    const size_t lds = static_cast<size_t>(G) * lj_stride(atoms) * sizeof(double);
+    const size_t lds = static_cast<size_t>(G) * lj_stride(atoms, with_grad) * sizeof(double);
     if (grad) {
         k_lj_grad<L, APL><<<dim3(static_cast<uint32_t>(blocks)), dim3(G * L), lds, s>>>(dvs, n, atoms, G, grad,
                                                                                         fvs);
@@ -189,7 +192,7 @@ pgh_status launch_lj_l(const double *dvs, uint64_t n, uint32_t atoms, double *fv
 
 pgh_status launch_lj(const double *dvs, uint64_t n, uint32_t atoms, double *fvs, double *grad, hipStream_t s)
 {
-    if (lj_group_count(atoms) == 0u) {
+    if (lj_group_count(atoms, grad != nullptr) == 0u) {
         return fail(PGH_UNSUPPORTED, "lennard_jones with " + std::to_string(atoms)
                                          + " atoms does not fit the LDS of one workgroup");
     }

@@ -73,11 +73,13 @@ __host__ __device__ __forceinline__ void bounds_of(int kind, uint32_t j, double 
 // Lanes per individual
 // ---------------------------------------------------------------------------
 
-// Separable problems: an L-lane group per individual.  Wide rows use a whole
-// wavefront so that a row streams as 1 KiB coalesced loads.
-__host__ __device__ constexpr int sep_lanes(uint32_t nx)
+// Separable problems: an L-lane group per individual.  Wide rows use 32 or 64
+// lanes so that a row streams as 512 B / 1 KiB coalesced loads; ackley takes
+// 32 (two individuals per wavefront), which halves the share of its
+// per-individual epilogue (two group sums, two exp, a sqrt).
+__host__ __device__ constexpr int sep_lanes(int kind, uint32_t nx)
 {
-    return nx >= 256u ? 64 : (nx >= 32u ? 16 : 4);
+    return nx >= 256u ? (kind == ACKLEY ? 32 : 64) : (nx >= 32u ? 16 : 4);
 }
 
 // Lennard-Jones: at least ~8 pair evaluations per lane, at most 16 lanes.
@@ -88,19 +90,26 @@ __host__ __device__ constexpr int lj_lanes(uint32_t atoms)
     return want >= 16u ? 16 : (want >= 8u ? 8 : (want >= 4u ? 4 : (want >= 2u ? 2 : 1)));
 }
 
-// Doubles of LDS per staged Lennard-Jones individual: the atoms' (x, y, z)
-// twice over (see lj_stage).
-__host__ __device__ constexpr uint32_t lj_stride(uint32_t atoms)
+// Lennard-Jones individuals are staged in LDS with the atoms' (x, y, z) and a
+// second copy of the first lj_copy_atoms atoms after them (see lj_stage): the
+// energy visits partners a + k with k <= A/2, the gradient k <= A - 1.
+__host__ __device__ constexpr uint32_t lj_copy_atoms(uint32_t atoms, bool grad)
 {
-    return 6u * atoms;
+    return grad ? atoms - 1u : atoms / 2u;
+}
+
+// Doubles of LDS per staged individual.
+__host__ __device__ constexpr uint32_t lj_stride(uint32_t atoms, bool grad)
+{
+    return 3u * (atoms + lj_copy_atoms(atoms, grad));
 }
 
 // Individuals per workgroup for Lennard-Jones: a workgroup uses at most 64 KiB
 // of LDS.
-__host__ __device__ constexpr uint32_t lj_group_count(uint32_t atoms)
+__host__ __device__ constexpr uint32_t lj_group_count(uint32_t atoms, bool grad)
 {
     const uint32_t L = static_cast<uint32_t>(lj_lanes(atoms));
-    const uint32_t bytes = 8u * lj_stride(atoms);
+    const uint32_t bytes = 8u * lj_stride(atoms, grad);
     uint32_t g = 256u / L;
     while (g > 1u && g * bytes > 65536u) g >>= 1;
     return (g * bytes > 65536u) ? 0u : g;
@@ -368,15 +377,16 @@ __device__ __forceinline__ void lj_slot(uint32_t e, uint32_t &atom, uint32_t &co
 }
 
 // Stages `count` individuals (rows of nx = 3*atoms-6) into LDS, lj_stride
-// doubles each: atom a's (x, y, z) at [3a, 3a+3) and again at [3(a+A),
-// 3(a+A)+3), so that a partner a + k (k < A) is read without a wrap and at a
-// constant offset from atom a.  Called by all threads of the workgroup; a
-// __syncthreads() must follow.
-__device__ __forceinline__ void lj_stage(const double *__restrict__ rows, uint32_t count, uint32_t atoms,
+// doubles each: atom a's (x, y, z) at [3a, 3a+3), and for a < lj_copy_atoms
+// again at [3(a+A), 3(a+A)+3), so that a partner a + k is read without a wrap
+// and at a constant offset from atom a.  Called by all threads of the
+// workgroup; a __syncthreads() must follow.
+__device__ __forceinline__ void lj_stage(const double *__restrict__ rows, uint32_t count, uint32_t atoms, bool grad,
                                          double *__restrict__ lds)
 {
     const uint32_t nx = 3u * atoms - 6u;
-    const uint32_t stride = lj_stride(atoms);
+    const uint32_t stride = lj_stride(atoms, grad);
+    const uint32_t copies = lj_copy_atoms(atoms, grad);
     const uint32_t copy = 3u * atoms;
     for (uint32_t t = threadIdx.x; t < count * 6u; t += blockDim.x) {
         // the six fixed coordinates: atom 0 (x,y,z), atom 1 (x,y), atom 2 (x)
@@ -385,7 +395,7 @@ __device__ __forceinline__ void lj_stage(const double *__restrict__ rows, uint32
         const uint32_t coord = f < 3u ? f : (f < 5u ? f - 3u : 0u);
         double *dst = lds + g * stride + 3u * atom + coord;
         dst[0] = 0.0;
-        dst[copy] = 0.0;
+        if (atom < copies) dst[copy] = 0.0;
     }
     for (uint32_t t = threadIdx.x; t < count * nx; t += blockDim.x) {
         const uint32_t g = t / nx, e = t - g * nx;
@@ -394,7 +404,7 @@ __device__ __forceinline__ void lj_stage(const double *__restrict__ rows, uint32
         const double v = rows[static_cast<uint64_t>(g) * nx + e];
         double *dst = lds + g * stride + 3u * atom + coord;
         dst[0] = v;
-        dst[copy] = v;
+        if (atom < copies) dst[copy] = v;
     }
 }
 
@@ -594,22 +604,27 @@ struct lj_slots {
     }
 };
 
-// One visit: the pair (owned slot, partner at pb).  r2 feeds the zero check.
+// One visit: the pair (owned slot, partner at pb).  inv = 1 / r^2 and
+// s = inv^3 = r^-6.  For r^2 outside [2^-1000, 2^1000] these are not the
+// reference's values, but then the energy term (or, for r^2 = 0 or inf, every
+// result) is not finite unless the true term underflows to 0 as well, and the
+// callers' finiteness test sends such an individual to the careful path.
 struct lj_pair {
-    double dx, dy, dz, r2, s;
+    double dx, dy, dz, inv, s;
     __device__ __forceinline__ lj_pair(double x, double y, double z, const double *__restrict__ pb)
     {
         dx = x - pb[0];
         dy = y - pb[1];
         dz = z - pb[2];
-        r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-        s = lj_recip_fast(r2 * r2 * r2);
+        const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        inv = lj_recip_fast(r2);
+        s = (inv * inv) * inv;
     }
 };
 
 template <int L, uint32_t NS>
 __device__ __forceinline__ void lj_energy_slots(const double *__restrict__ P, uint32_t atoms, uint32_t a0, uint32_t r,
-                                                double &acc, double &rmin)
+                                                double &acc)
 {
     const lj_cycle cyc(atoms);
     lj_slots<L, NS> sl;
@@ -620,7 +635,6 @@ __device__ __forceinline__ void lj_energy_slots(const double *__restrict__ P, ui
         for (uint32_t q = 0; q < NS; ++q) {
             const lj_pair v(sl.x[q], sl.y[q], sl.z[q], sl.p[q] + 3u * k);
             acc = fma(sl.w[q], fma(v.s, v.s, -v.s), acc);
-            rmin = fmin(rmin, v.r2);
         }
     }
     if (full < cyc.half) {
@@ -630,46 +644,35 @@ __device__ __forceinline__ void lj_energy_slots(const double *__restrict__ P, ui
             const lj_pair v(sl.x[q], sl.y[q], sl.z[q], sl.p[q] + 3u * cyc.half);
             const double w = sl.a[q] < cyc.half ? sl.w[q] : 0.0;
             acc = fma(w, fma(v.s, v.s, -v.s), acc);
-            rmin = fmin(rmin, v.r2);
         }
     }
-}
-
-template <int L>
-__device__ __forceinline__ double group_min(double v)
-{
-#pragma unroll
-    for (int o = L / 2; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, L));
-    return v;
 }
 
 // Energy of the individual staged at P by the L lanes of a group (lane r):
 // sum of 4 (s^2 - s), s = r^-6, over the cyclic schedule, then the group's
-// butterfly sum.  The streaming loop has no masks and no range checks; a
-// zero or non-finite distance, or a non-finite sum, sends the individual to
-// lj_energy_careful, which handles the reference's special cases.
+// butterfly sum.  The streaming loop has no masks and no range checks: a
+// zero, tiny or NaN distance makes the sum NaN or infinite (see lj_pair), and
+// such an individual goes to lj_energy_careful, which handles the reference's
+// special cases.
 template <int L, uint32_t APL>
 __device__ __forceinline__ double lj_energy(const double *__restrict__ P, uint32_t atoms, int r)
 {
-    double acc = 0., rmin = __builtin_inf();
+    double acc = 0.;
     if constexpr (APL > 0) {
-        lj_energy_slots<L, APL>(P, atoms, static_cast<uint32_t>(r), static_cast<uint32_t>(r), acc, rmin);
+        lj_energy_slots<L, APL>(P, atoms, static_cast<uint32_t>(r), static_cast<uint32_t>(r), acc);
     } else {
         for (uint32_t a0 = static_cast<uint32_t>(r); a0 < atoms; a0 += 4u * L) {
-            lj_energy_slots<L, 4>(P, atoms, a0, static_cast<uint32_t>(r), acc, rmin);
+            lj_energy_slots<L, 4>(P, atoms, a0, static_cast<uint32_t>(r), acc);
         }
     }
     acc = group_sum<L>(acc);
-    rmin = group_min<L>(rmin);
-    // every r2 > 2^-333 keeps r2^3 normal; r2 < 2^333 keeps it finite
-    const bool fast_ok = rmin > 0x1p-333 && acc - acc == 0.0;
-    if (!fast_ok) return lj_energy_careful<L>(P, atoms, r);
+    if (!(acc - acc == 0.0)) return lj_energy_careful<L>(P, atoms, r);
     return 4 * acc;
 }
 
 template <int L, uint32_t NS>
 __device__ __forceinline__ void lj_grad_slots(const double *__restrict__ P, uint32_t atoms, uint32_t a0, uint32_t r,
-                                              double &acc, double &rmin, double &fsum, double *__restrict__ g)
+                                              double &acc, double &fsum, double *__restrict__ g)
 {
     lj_slots<L, NS> sl;
     sl.load(P, a0, r, atoms);
@@ -680,14 +683,12 @@ __device__ __forceinline__ void lj_grad_slots(const double *__restrict__ P, uint
 #pragma unroll
         for (uint32_t q = 0; q < NS; ++q) {
             const lj_pair v(sl.x[q], sl.y[q], sl.z[q], sl.p[q] + 3u * k);
-            const double inv_r2 = (v.s * v.r2) * v.r2;
-            const double coef = (fma(-48., v.s, 24.) * v.s) * inv_r2;
+            const double coef = (fma(-48., v.s, 24.) * v.s) * v.inv;
             fx[q] = fma(coef, v.dx, fx[q]);
             fy[q] = fma(coef, v.dy, fy[q]);
             fz[q] = fma(coef, v.dz, fz[q]);
-            // every pair is visited from both of its atoms: half weight each
+            // every pair is visited from both of its atoms: the sum is halved
             acc = fma(sl.w[q], fma(v.s, v.s, -v.s), acc);
-            rmin = fmin(rmin, v.r2);
         }
     }
 #pragma unroll
@@ -702,24 +703,23 @@ __device__ __forceinline__ void lj_grad_slots(const double *__restrict__ P, uint
 // dE/dp_a = sum_b (p_a - p_b) (24 s - 48 s^2) / r^2 (s = r^-6; 1/r^2 = s r^4)
 // and the energy terms of all its visits (each pair twice, so the sum is
 // halved).  The gradient row (nx doubles) is written to g; returns the
-// energy.  A zero or non-finite distance, or a non-finite result, sends the
-// individual to lj_energy_grad_careful.
+// energy.  A non-finite energy or force (a zero, tiny or NaN distance, see
+// lj_pair) sends the individual to lj_energy_grad_careful.
 template <int L, uint32_t APL>
 __device__ __forceinline__ double lj_energy_grad(const double *__restrict__ P, uint32_t atoms, int r,
                                                  double *__restrict__ g)
 {
-    double acc = 0., rmin = __builtin_inf(), fsum = 0.;
+    double acc = 0., fsum = 0.;
     if constexpr (APL > 0) {
-        lj_grad_slots<L, APL>(P, atoms, static_cast<uint32_t>(r), static_cast<uint32_t>(r), acc, rmin, fsum, g);
+        lj_grad_slots<L, APL>(P, atoms, static_cast<uint32_t>(r), static_cast<uint32_t>(r), acc, fsum, g);
     } else {
         for (uint32_t a0 = static_cast<uint32_t>(r); a0 < atoms; a0 += 4u * L) {
-            lj_grad_slots<L, 4>(P, atoms, a0, static_cast<uint32_t>(r), acc, rmin, fsum, g);
+            lj_grad_slots<L, 4>(P, atoms, a0, static_cast<uint32_t>(r), acc, fsum, g);
         }
     }
     acc = group_sum<L>(acc);
-    rmin = group_min<L>(rmin);
     fsum = group_sum<L>(fsum);
-    const bool fast_ok = rmin > 0x1p-333 && acc - acc == 0.0 && fsum - fsum == 0.0;
+    const bool fast_ok = acc - acc == 0.0 && fsum - fsum == 0.0;
     if (!fast_ok) return lj_energy_grad_careful<L>(P, atoms, r, g);
     return 2 * acc;
 }

@@ -210,9 +210,16 @@ __device__ __forceinline__ void eval_separable(const island_desc &di, const tile
     const double *x = c.X + di.xoff + static_cast<uint64_t>(p) * di.nx;
     double f = (di.nx % 2u == 0u) ? separable_eval<KIND, L, true>(x, di.nx, static_cast<int>(r))
                                   : separable_eval<KIND, L, false>(x, di.nx, static_cast<int>(r));
-    if (is_recompute_mark(f)) { // group-uniform; an argument beyond the fast cosine's range
-        f = (di.nx % 2u == 0u) ? separable_eval_exact<KIND, L, true>(x, di.nx, static_cast<int>(r))
-                               : separable_eval_exact<KIND, L, false>(x, di.nx, static_cast<int>(r));
+    if (is_recompute_mark(f)) {
+        // group-uniform: an element beyond the streaming cosine's range.  One
+        // lane recomputes the individual as the batch evaluator does (the
+        // reference's sequential order) and the group takes its value.
+        double e = 0.;
+        if (r == 0u) {
+            e = (di.nx % 2u == 0u) ? separable_eval_exact<KIND, 1, true>(x, di.nx, 0)
+                                   : separable_eval_exact<KIND, 1, false>(x, di.nx, 0);
+        }
+        f = __shfl(e, 0, L);
     }
     if (g < tl.count) remember(di, c, p, f, r, L);
 }
@@ -223,6 +230,9 @@ __device__ __forceinline__ void eval_sep_dispatch(const island_desc &di, const t
     switch (di.lanes) {
         case 64:
             eval_separable<KIND, 64>(di, tl, c);
+            return;
+        case 32:
+            eval_separable<KIND, 32>(di, tl, c);
             return;
         case 16:
             eval_separable<KIND, 16>(di, tl, c);
@@ -236,12 +246,12 @@ template <int L, uint32_t APL>
 __device__ __forceinline__ void eval_lj_a(const island_desc &di, const tile &tl, const eval_ctx &c, double *lds)
 {
     const uint32_t atoms = di.dim;
-    lj_stage(c.X + di.xoff + static_cast<uint64_t>(tl.first) * di.nx, tl.count, atoms, lds);
+    lj_stage(c.X + di.xoff + static_cast<uint64_t>(tl.first) * di.nx, tl.count, atoms, false, lds);
     __syncthreads();
     const uint32_t g = threadIdx.x / L, r = threadIdx.x % L;
-    if (g >= lj_group_count(atoms)) return;
+    if (g >= lj_group_count(atoms, false)) return;
     const uint32_t gg = g < tl.count ? g : tl.count - 1u;
-    const double f = lj_energy<L, APL>(lds + gg * lj_stride(atoms), atoms, static_cast<int>(r));
+    const double f = lj_energy<L, APL>(lds + gg * lj_stride(atoms, false), atoms, static_cast<int>(r));
     if (g < tl.count) remember(di, c, tl.first + gg, f, r, L);
 }
 
@@ -489,11 +499,12 @@ pgh_status pgh_archi_create(pgh_archi **out, const pgh_island_spec *islands, uin
         uint32_t per = 256u;
         if (p.kind == PGH_LENNARD_JONES) {
             di.lanes = static_cast<uint32_t>(lj_lanes(p.dim));
-            per = lj_group_count(p.dim);
+            per = lj_group_count(p.dim, false);
             if (per == 0u) return fail(PGH_UNSUPPORTED, "lennard_jones cluster too large for one workgroup");
-            a->lds_bytes = std::max<size_t>(a->lds_bytes, static_cast<size_t>(per) * lj_stride(p.dim) * sizeof(double));
+            a->lds_bytes =
+                std::max<size_t>(a->lds_bytes, static_cast<size_t>(per) * lj_stride(p.dim, false) * sizeof(double));
         } else if (p.kind != PGH_ROSENBROCK) {
-            di.lanes = static_cast<uint32_t>(sep_lanes(p.nx));
+            di.lanes = static_cast<uint32_t>(sep_lanes(p.kind, p.nx));
             per = 256u / di.lanes;
         } else {
             di.lanes = 1u;

@@ -201,6 +201,31 @@ def test_lennard_jones_gradient(atoms, n):
         assert abs(f[i] - rf[i]) <= 1e-12 * lj_abs_scale(x[i])
 
 
+def test_lennard_jones_gradient_special_distances():
+    """Rows that leave the streaming path: coincident atoms (energy +inf, the
+    forces between them NaN, like the oracle's 1/r^2), and an atom so far away
+    that r^6 overflows (its terms and forces vanish)."""
+    atoms, n = 6, 4
+    x = inputs(P.LENNARD_JONES, atoms, n, 5, lo=-3.0, hi=3.0)
+    x[0, 3:6] = 0.0           # atom 3 on atom 0
+    x[1, 6:9] = x[1, 9:12]    # atoms 4 and 5 coincide
+    x[2, 9:12] = 1.0e120      # atom 5 far away: r^2 ~ 1e240, r^6 overflows
+    g = torch.empty((n, x.shape[1]), dtype=torch.float64, device=DEV)
+    f = torch.empty((n, 1), dtype=torch.float64, device=DEV)
+    P.batch_gradient(P.LENNARD_JONES, atoms, dev(x), g, f)
+    torch.cuda.synchronize()
+    g, f = g.cpu().numpy(), f.cpu().numpy()[:, 0]
+    ref = O.gradient(P.LENNARD_JONES, atoms, x)
+    rf = O.bfe(P.LENNARD_JONES, atoms, x)[:, 0]
+    assert f[0] == math.inf and f[1] == math.inf
+    for i in range(n):
+        assert np.array_equal(np.isnan(g[i]), np.isnan(ref[i])), i
+        ok = ~np.isnan(ref[i])
+        assert np.linalg.norm(g[i][ok] - ref[i][ok]) <= 1e-12 * max(np.linalg.norm(ref[i][ok]), 1e-300), i
+    for i in (2, 3):
+        assert abs(f[i] - rf[i]) <= 1e-12 * lj_abs_scale(x[i]), (i, f[i], rf[i])
+
+
 def test_lennard_jones_gradient_finite_differences():
     # central differences, h = 1e-6, and estimate_gradient_h
     # (gradients_and_hessians.hpp:199-244) at dx = 1e-4; <= 1e-7 relative
