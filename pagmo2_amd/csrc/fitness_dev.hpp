@@ -231,17 +231,31 @@ __device__ __forceinline__ double separable_eval_impl(const double *__restrict__
     const uint32_t npairs = (nx + 1u) >> 1;
     uint32_t c = static_cast<uint32_t>(r);
     if constexpr (ALIGNED) {
-        // the next pair's load is issued before the current pair is used
-        if (c < npairs) {
-            double2 v = *reinterpret_cast<const double2 *>(x + 2u * c);
-            for (; c + L < npairs; c += L) {
-                const double2 w = *reinterpret_cast<const double2 *>(x + 2u * (c + L));
-                acc.add(v.x, 2u * c);
-                acc.add(v.y, 2u * c + 1u);
+        // `full` rounds in which every lane of the group has a pair (a trip
+        // count uniform over the wavefront, so the loop is scalar-controlled
+        // and the loads take a scalar offset), then at most one more pair per
+        // lane.  The next pair's load is issued before the current pair is
+        // used.  Pairs are added in increasing c either way.
+        const double2 *row = reinterpret_cast<const double2 *>(x) + c;
+        const uint32_t full = npairs / L;
+        if (full > 0u) {
+            double2 v = row[0];
+            for (uint32_t j = 1u; j < full; ++j) {
+                const double2 w = row[j * L];
+                // keep the load at the top of the round: the scheduler would
+                // otherwise sink it below the use of v to reuse v's registers
+                __builtin_amdgcn_sched_barrier(0);
+                acc.add(v.x, 2u * (c + (j - 1u) * L));
+                acc.add(v.y, 2u * (c + (j - 1u) * L) + 1u);
                 v = w;
             }
-            acc.add(v.x, 2u * c);
-            acc.add(v.y, 2u * c + 1u);
+            acc.add(v.x, 2u * (c + (full - 1u) * L));
+            acc.add(v.y, 2u * (c + (full - 1u) * L) + 1u);
+        }
+        if (c + full * L < npairs) {
+            const double2 v = row[full * L];
+            acc.add(v.x, 2u * (c + full * L));
+            acc.add(v.y, 2u * (c + full * L) + 1u);
         }
     } else {
         for (; c < npairs; c += L) separable_pair<KIND, L, ALIGNED, FAST>(acc, x, nx, c);
