@@ -270,11 +270,16 @@ __device__ __forceinline__ double separable_eval_impl(const double *__restrict__
         // rastrigin.cpp:69
         return group_sum<L>(acc.a) + 10. * n;
     } else if constexpr (KIND == ACKLEY) {
-        // ackley.cpp:72-73
+        // ackley.cpp:72-73, on the group's first lane only (the exp / sqrt
+        // sequence then runs with one active lane) and broadcast
         const double s1 = group_sum<L>(acc.a);
         const double s2 = group_sum<L>(acc.b);
-        const double nepero = exp(1.0);
-        return -20 * exp(-0.2 * sqrt(1.0 / n * s1)) - exp(1.0 / n * s2) + 20 + nepero;
+        double f = 0.;
+        if (r == 0) {
+            const double nepero = exp(1.0);
+            f = -20 * exp(-0.2 * sqrt(1.0 / n * s1)) - exp(1.0 / n * s2) + 20 + nepero;
+        }
+        return L > 1 ? __shfl(f, 0, L) : f;
     } else if constexpr (KIND == SCHWEFEL) {
         // schwefel.cpp:66
         return 418.9828872724338 * n - group_sum<L>(acc.a);
@@ -618,8 +623,10 @@ struct lj_slots {
     }
 };
 
-// One visit: the pair (owned slot, partner at pb).  inv = 1 / r^2 and
-// s = inv^3 = r^-6.  For r^2 outside [2^-1000, 2^1000] these are not the
+// One visit: the pair (owned slot, partner at pb).  inv = 1 / r^2 from
+// v_rcp_f64 (24 bits) and one Newton step (measured: 48.7 bits, a relative
+// error of 2.2e-15), and s = inv^3 = r^-6; an energy term is then within about
+// 1e-14 of its own size.  For r^2 outside [2^-1000, 2^1000] these are not the
 // reference's values, but then the energy term (or, for r^2 = 0 or inf, every
 // result) is not finite unless the true term underflows to 0 as well, and the
 // callers' finiteness test sends such an individual to the careful path.
@@ -631,7 +638,8 @@ struct lj_pair {
         dy = y - pb[1];
         dz = z - pb[2];
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-        inv = lj_recip_fast(r2);
+        const double r = __builtin_amdgcn_rcp(r2);
+        inv = fma(r, fma(-r2, r, 1.0), r);
         s = (inv * inv) * inv;
     }
 };

@@ -347,7 +347,7 @@ __global__ __launch_bounds__(kPeelBlock) void k_peel(const uint64_t *__restrict_
 // the last front is known.  lvl[2L] = claim[L], lvl[2L+1] = done[L].
 constexpr uint32_t kGridBlock = 1024;
 constexpr uint32_t kGridMaxN = 32768;
-constexpr uint32_t kChunk = 2048;         // items per claim
+constexpr uint32_t kChunk = 1024;         // items per claim: one per thread
 constexpr uint32_t kSpinLimit = 1u << 26; // ~2 s of s_sleep(1) polls
 
 __device__ __forceinline__ uint64_t ld_relaxed64(const uint64_t *p)
@@ -585,9 +585,10 @@ pgh_status launch_grid_peel(pgh_fnds_ws *ws, hipStream_t s, uint32_t n32, uint32
     uint32_t P = 1;
     while (P < n32) P <<= 1;
     const size_t lds = static_cast<size_t>(std::max(P, kGridBlock)) * 4u;
-    // one workgroup per 512 elements, at least 8 and at most one per CU: the
-    // items of a level rarely fill more
-    const uint32_t blocks = std::min(cus, std::max(8u, n32 / 512u));
+    // one workgroup per 256 elements, at least 8 and at most one per CU (a
+    // sweep of 64..1024 elements per workgroup and 512..8192 items per claim
+    // on the 8192 x 2 and 16384 x 3 sorts put the optimum here)
+    const uint32_t blocks = std::min(cus, std::max(8u, n32 / 256u));
     auto *cnt = static_cast<uint32_t *>(ws->cnt32.ptr);
     auto *key = static_cast<uint32_t *>(ws->key32.ptr);
     auto *comp = static_cast<uint32_t *>(ws->comp.ptr);
