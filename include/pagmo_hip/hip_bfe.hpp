@@ -21,8 +21,9 @@
 //
 // Header-only; link with -lpagmo_hip.  Besides the UDBFE the header offers the
 // batched gradient (problem::gradient, src/problem.cpp:456-467, for a whole
-// population) and fast_non_dominated_sorting
-// (src/utils/multi_objective.cpp:200-257) on the device.
+// population), fast_non_dominated_sorting, crowding_distance,
+// sort_population_mo and select_best_N_mo (src/utils/multi_objective.cpp:
+// 200-465) on the device.
 #pragma once
 
 #include <cstdint>
@@ -170,13 +171,14 @@ inline pagmo::vector_double batch_gradient(const pagmo::problem &p, const pagmo:
     return g;
 }
 
-// fast_non_dominated_sorting on the device, same return value and exceptions.
-inline pagmo::fnds_return_type fast_non_dominated_sorting(const std::vector<pagmo::vector_double> &points)
+namespace detail
 {
-    const auto n = points.size();
-    const auto m = n ? points[0].size() : 0u;
+// Row-major copy of equally sized objective vectors; a size mismatch throws
+// pareto_dominance's message (multi_objective.cpp:99-104).
+inline std::vector<double> flatten_objectives(const std::vector<pagmo::vector_double> &points, std::size_t m)
+{
     std::vector<double> flat;
-    flat.reserve(n * m);
+    flat.reserve(points.size() * m);
     for (const auto &v : points) {
         if (v.size() != m) {
             pagmo_throw(std::invalid_argument, "Different number of objectives found in input fitnesses: "
@@ -185,6 +187,16 @@ inline pagmo::fnds_return_type fast_non_dominated_sorting(const std::vector<pagm
         }
         flat.insert(flat.end(), v.begin(), v.end());
     }
+    return flat;
+}
+} // namespace detail
+
+// fast_non_dominated_sorting on the device, same return value and exceptions.
+inline pagmo::fnds_return_type fast_non_dominated_sorting(const std::vector<pagmo::vector_double> &points)
+{
+    const auto n = points.size();
+    const auto m = n ? points[0].size() : 0u;
+    const auto flat = detail::flatten_objectives(points, m);
     pgh_fnds_ws *ws = nullptr;
     detail::check(pgh_fnds_create(&ws));
     struct guard {
@@ -211,6 +223,60 @@ inline pagmo::fnds_return_type fast_non_dominated_sorting(const std::vector<pagm
     return std::make_tuple(std::move(fronts), std::move(dom_list),
                            std::vector<pagmo::pop_size_t>(dom_count.begin(), dom_count.end()),
                            std::vector<pagmo::pop_size_t>(rank.begin(), rank.end()));
+}
+
+
+// crowding_distance (src/utils/multi_objective.cpp:280-318) on the device, same
+// checks and messages.  Equal objective values keep their order from the
+// previous objective's sort (std::sort leaves it open in the reference).
+inline pagmo::vector_double crowding_distance(const std::vector<pagmo::vector_double> &non_dom_front)
+{
+    const auto N = non_dom_front.size();
+    if (N < 2u) {
+        pagmo_throw(std::invalid_argument,
+                    "A non dominated front must contain at least two points: " + std::to_string(N) + " detected.");
+    }
+    const auto M = non_dom_front[0].size();
+    if (M < 2u) {
+        pagmo_throw(std::invalid_argument, "Points in the non dominated front must contain at least two objectives: "
+                                               + std::to_string(M) + " detected.");
+    }
+    for (const auto &v : non_dom_front) {
+        if (v.size() != M) {
+            pagmo_throw(std::invalid_argument, "A non dominated front must contain points of uniform dimensionality. "
+                                               "Some different sizes were instead detected.");
+        }
+    }
+    const auto flat = detail::flatten_objectives(non_dom_front, M);
+    pagmo::vector_double out(N);
+    detail::check(pgh_crowding_distance_host(flat.data(), N, M, out.data()));
+    return out;
+}
+
+// sort_population_mo (multi_objective.cpp:425-465) on the device; equal
+// (rank, crowding) pairs in index order.
+inline std::vector<pagmo::pop_size_t> sort_population_mo(const std::vector<pagmo::vector_double> &input_f)
+{
+    const auto n = input_f.size();
+    const auto m = n ? input_f[0].size() : 0u;
+    const auto flat = detail::flatten_objectives(input_f, m);
+    std::vector<uint64_t> out(n);
+    detail::check(pgh_sort_population_mo_host(flat.empty() ? nullptr : flat.data(), n, m, out.data()));
+    return std::vector<pagmo::pop_size_t>(out.begin(), out.end());
+}
+
+// select_best_N_mo (multi_objective.cpp:344-396) on the device.
+inline std::vector<pagmo::pop_size_t> select_best_N_mo(const std::vector<pagmo::vector_double> &input_f,
+                                                       pagmo::pop_size_t N)
+{
+    const auto n = input_f.size();
+    if (N == 0u || n == 0u) return {};
+    const auto m = input_f[0].size();
+    const auto flat = detail::flatten_objectives(input_f, m);
+    std::vector<uint64_t> out(n);
+    uint64_t len = 0;
+    detail::check(pgh_select_best_N_mo_host(flat.empty() ? nullptr : flat.data(), n, m, N, out.data(), &len));
+    return std::vector<pagmo::pop_size_t>(out.begin(), out.begin() + static_cast<std::ptrdiff_t>(len));
 }
 
 } // namespace pagmo_hip

@@ -128,6 +128,34 @@ pgh_status pgh_fnds_sizes(pgh_fnds_ws *ws, uint64_t *n_fronts, uint64_t *n_dom_i
 pgh_status pgh_fnds_copy(pgh_fnds_ws *ws, uint64_t *dom_count, uint64_t *rank, uint64_t *front_offsets,
                          uint64_t *front_items, uint64_t *dom_offsets, uint64_t *dom_items, int to_host);
 
+/* ---- crowding distance and the multi-objective orderings built on fnds
+ *      (src/utils/multi_objective.cpp:280-465).  The reference sorts with
+ *      std::sort, which leaves the order of equal keys open; here equal keys
+ *      keep their input order (std::stable_sort over the reference's own index
+ *      vectors): equal values of objective i within a front in their order
+ *      after objective i-1 (the first: position in the front), equal
+ *      (rank, crowding) pairs by index, equal crowding values of the partially
+ *      selected front by position.  With distinct keys the results are the
+ *      reference's bit for bit.  These calls use the workspace's fnds
+ *      buffers (the fnds results are replaced) and synchronise the stream. */
+/* crowding_distance (multi_objective.cpp:280-318) of the n points (row-major
+ * n x m) of one non-dominated front; out: n doubles.  INVALID_ARGUMENT with the
+ * reference's messages when n < 2 or m < 2. */
+pgh_status pgh_crowding_distance(pgh_fnds_ws *ws, const double *points, uint64_t n, uint64_t m, double *out,
+                                 void *stream);
+pgh_status pgh_crowding_distance_host(const double *points, uint64_t n, uint64_t m, double *out);
+/* sort_population_mo (multi_objective.cpp:425-465): out receives the n indices
+ * ordered by (non-domination rank, crowding distance descending). */
+pgh_status pgh_sort_population_mo(pgh_fnds_ws *ws, const double *objs, uint64_t n, uint64_t m, uint64_t *out,
+                                  void *stream);
+pgh_status pgh_sort_population_mo_host(const double *objs, uint64_t n, uint64_t m, uint64_t *out);
+/* select_best_N_mo (multi_objective.cpp:344-396): out (room for min(N, n)
+ * indices) receives the selection, *out_len its length. */
+pgh_status pgh_select_best_N_mo(pgh_fnds_ws *ws, const double *objs, uint64_t n, uint64_t m, uint64_t N,
+                                uint64_t *out, uint64_t *out_len, void *stream);
+pgh_status pgh_select_best_N_mo_host(const double *objs, uint64_t n, uint64_t m, uint64_t N, uint64_t *out,
+                                     uint64_t *out_len);
+
 /* ---- seeded inputs: std::mt19937 (include/pagmo/rng.hpp:43) and
  *      batch_random_decision_vector (include/pagmo/utils/generic.hpp:326-388)
  *      generated on the device, bit-identical to libstdc++. ------------- */

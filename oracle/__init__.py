@@ -64,6 +64,10 @@ def _declare(L):
         "orc_pso_gen_evolve": [cint, u64, u64, _dp, _dp, u32, dbl, dbl, dbl, dbl, u32, u32, u32, cint, u32,
                                cint, u32, u32, C.c_void_p, C.c_void_p],
         "orc_pso_gen_timed": [cint, u64, u64, _dp, _dp, u32, u32, u32, u32, C.POINTER(dbl)],
+        "orc_crowding_distance": [_dp, u64, u64, cint, _dp],
+        "orc_select_best_N_mo": [_dp, u64, u64, u64, cint, _u64p, C.POINTER(u64)],
+        "orc_sort_population_mo": [_dp, u64, u64, cint, _u64p],
+        "orc_sort_population_mo_timed": [_dp, u64, u64, u32, C.POINTER(dbl)],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
@@ -199,6 +203,46 @@ def fnds(objs: np.ndarray):
         return r
     finally:
         L.orc_fnds_free(h)
+
+
+def _pts(objs):
+    objs = np.ascontiguousarray(objs, np.float64)
+    if objs.ndim != 2:
+        objs = objs.reshape(len(objs), -1)
+    return objs, objs.shape[0], objs.shape[1], (objs if objs.size else np.zeros(1))
+
+
+def crowding_distance(objs, stable=False) -> np.ndarray:
+    """crowding_distance (multi_objective.cpp:280-318); stable=True breaks ties
+    by position (std::stable_sort) instead of std::sort."""
+    objs, n, m, buf = _pts(objs)
+    out = np.empty(max(n, 1))
+    _chk(lib().orc_crowding_distance(buf, n, m, int(stable), out))
+    return out[:n]
+
+
+def select_best_N_mo(objs, N, stable=False) -> np.ndarray:
+    """select_best_N_mo (multi_objective.cpp:344-396)."""
+    objs, n, m, buf = _pts(objs)
+    out = np.empty(max(n, 1), np.uint64)
+    ln = u64()
+    _chk(lib().orc_select_best_N_mo(buf, n, m, N, int(stable), out, C.byref(ln)))
+    return out[: ln.value]
+
+
+def sort_population_mo(objs, stable=False) -> np.ndarray:
+    """sort_population_mo (multi_objective.cpp:425-465)."""
+    objs, n, m, buf = _pts(objs)
+    out = np.empty(max(n, 1), np.uint64)
+    _chk(lib().orc_sort_population_mo(buf, n, m, int(stable), out))
+    return out[:n]
+
+
+def sort_population_mo_timed(objs, reps: int) -> float:
+    objs, n, m, buf = _pts(objs)
+    sec = dbl()
+    _chk(lib().orc_sort_population_mo_timed(buf, n, m, reps, C.byref(sec)))
+    return sec.value
 
 
 def fnds_timed(objs: np.ndarray, reps: int) -> float:

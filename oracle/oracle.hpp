@@ -39,6 +39,7 @@
 #include <thread>
 #include <tuple>
 #include <utility>
+#include <numeric>
 #include <vector>
 
 namespace oracle
@@ -609,6 +610,123 @@ inline fnds_result fast_non_dominated_sorting(const double *points, std::size_t 
         }
     }
     return r;
+}
+
+// The reference sorts with std::sort, whose order among equal keys the
+// standard leaves open.  STABLE = false restates the reference (std::sort,
+// libstdc++'s introsort, so the same order as the reference for a given input
+// on this standard library); STABLE = true breaks ties by input position
+// (std::stable_sort), the rule the device implements.
+template <typename It, typename Cmp>
+inline void mo_sort(It first, It last, Cmp cmp, bool stable)
+{
+    if (stable) {
+        std::stable_sort(first, last, cmp);
+    } else {
+        std::sort(first, last, cmp);
+    }
+}
+
+// src/utils/multi_objective.cpp:280-318 over the N points (m objectives each)
+// of a front given row-major.
+inline vector_double crowding_distance(const double *pts, std::size_t N, std::size_t M, bool stable = false)
+{
+    if (N < 2u) {
+        throw std::invalid_argument("A non dominated front must contain at least two points: " + std::to_string(N)
+                                    + " detected.");
+    }
+    if (M < 2u) {
+        throw std::invalid_argument("Points in the non dominated front must contain at least two objectives: "
+                                    + std::to_string(M) + " detected.");
+    }
+    std::vector<pop_size_t> indexes(N);
+    std::iota(indexes.begin(), indexes.end(), pop_size_t(0u));
+    vector_double retval(N, 0.);
+    for (std::size_t i = 0u; i < M; ++i) {
+        mo_sort(indexes.begin(), indexes.end(),
+                [i, pts, M](pop_size_t a, pop_size_t b) { return less_than_f(pts[a * M + i], pts[b * M + i]); },
+                stable);
+        retval[indexes[0]] = std::numeric_limits<double>::infinity();
+        retval[indexes[N - 1u]] = std::numeric_limits<double>::infinity();
+        const double df = pts[indexes[N - 1u] * M + i] - pts[indexes[0] * M + i];
+        for (std::size_t j = 1u; j < N - 1u; ++j) {
+            retval[indexes[j]] += (pts[indexes[j + 1u] * M + i] - pts[indexes[j - 1u] * M + i]) / df;
+        }
+    }
+    return retval;
+}
+
+inline vector_double front_points(const double *pts, std::size_t M, const std::vector<pop_size_t> &front)
+{
+    vector_double out(front.size() * M);
+    for (std::size_t k = 0; k < front.size(); ++k) {
+        for (std::size_t i = 0; i < M; ++i) out[k * M + i] = pts[front[k] * M + i];
+    }
+    return out;
+}
+
+// src/utils/multi_objective.cpp:344-396.
+inline std::vector<pop_size_t> select_best_N_mo(const double *pts, std::size_t n, std::size_t M, std::size_t N,
+                                                bool stable = false)
+{
+    if (N == 0u || n == 0u) return {};
+    if (n == 1u) return {0u};
+    if (N >= n) {
+        std::vector<pop_size_t> r(n);
+        std::iota(r.begin(), r.end(), pop_size_t(0u));
+        return r;
+    }
+    std::vector<pop_size_t> retval;
+    std::size_t front_id = 0u;
+    const auto t = fast_non_dominated_sorting(pts, n, M);
+    for (const auto &front : t.fronts) {
+        if (retval.size() + front.size() <= N) {
+            for (auto i : front) retval.push_back(i);
+            if (retval.size() == N) return retval;
+            ++front_id;
+        } else {
+            break;
+        }
+    }
+    const auto &front = t.fronts[front_id];
+    const auto fp = front_points(pts, M, front);
+    const vector_double cds = crowding_distance(fp.data(), front.size(), M, stable);
+    std::vector<pop_size_t> idxs(front.size());
+    std::iota(idxs.begin(), idxs.end(), pop_size_t(0u));
+    mo_sort(idxs.begin(), idxs.end(), [&cds](pop_size_t a, pop_size_t b) { return greater_than_f(cds[a], cds[b]); },
+            stable);
+    const auto remaining = N - retval.size();
+    for (std::size_t i = 0u; i < remaining; ++i) retval.push_back(front[idxs[i]]);
+    return retval;
+}
+
+// src/utils/multi_objective.cpp:425-465.
+inline std::vector<pop_size_t> sort_population_mo(const double *pts, std::size_t n, std::size_t M, bool stable = false)
+{
+    if (n < 2u) {
+        if (n == 0u) return {};
+        return {0u};
+    }
+    std::vector<pop_size_t> retval(n);
+    std::iota(retval.begin(), retval.end(), pop_size_t(0u));
+    const auto t = fast_non_dominated_sorting(pts, n, M);
+    vector_double crowding(n);
+    for (const auto &front : t.fronts) {
+        if (front.size() == 1u) {
+            crowding[front[0]] = 0.;
+        } else {
+            const auto fp = front_points(pts, M, front);
+            const vector_double tmp = crowding_distance(fp.data(), front.size(), M, stable);
+            for (std::size_t i = 0u; i < front.size(); ++i) crowding[front[i]] = tmp[i];
+        }
+    }
+    mo_sort(retval.begin(), retval.end(),
+            [&t, &crowding](pop_size_t a, pop_size_t b) {
+                if (t.rank[a] == t.rank[b]) return greater_than_f(crowding[a], crowding[b]);
+                return t.rank[a] < t.rank[b];
+            },
+            stable);
+    return retval;
 }
 
 // ---------------------------------------------------------------------------

@@ -235,6 +235,50 @@ void orc_fnds_free(void *h)
     delete static_cast<oracle::fnds_result *>(h);
 }
 
+// crowding_distance / select_best_N_mo / sort_population_mo; stable != 0 breaks
+// ties by input position (the device's rule), else std::sort as the reference.
+int orc_crowding_distance(const double *pts, uint64_t n, uint64_t m, int stable, double *out)
+{
+    return guarded([&] {
+        const auto r = oracle::crowding_distance(pts, n, m, stable != 0);
+        std::copy(r.begin(), r.end(), out);
+    });
+}
+
+int orc_select_best_N_mo(const double *pts, uint64_t n, uint64_t m, uint64_t N, int stable, uint64_t *out,
+                         uint64_t *out_len)
+{
+    return guarded([&] {
+        const auto r = oracle::select_best_N_mo(pts, n, m, N, stable != 0);
+        std::copy(r.begin(), r.end(), out);
+        *out_len = r.size();
+    });
+}
+
+int orc_sort_population_mo(const double *pts, uint64_t n, uint64_t m, int stable, uint64_t *out)
+{
+    return guarded([&] {
+        const auto r = oracle::sort_population_mo(pts, n, m, stable != 0);
+        std::copy(r.begin(), r.end(), out);
+    });
+}
+
+// Timed sort_population_mo (single-threaded, as in the reference).
+int orc_sort_population_mo_timed(const double *pts, uint64_t n, uint64_t m, uint32_t reps, double *sec)
+{
+    return guarded([&] {
+        std::vector<double> t;
+        for (uint32_t k = 0; k < reps; ++k) {
+            const auto t0 = std::chrono::steady_clock::now();
+            auto r = oracle::sort_population_mo(pts, n, m, false);
+            const auto t1 = std::chrono::steady_clock::now();
+            t.push_back(std::chrono::duration<double>(t1 - t0).count());
+        }
+        std::sort(t.begin(), t.end());
+        *sec = t[t.size() / 2];
+    });
+}
+
 // Timed fnds (single-threaded, as in the reference): median seconds per call.
 int orc_fnds_timed(const double *objs, uint64_t n, uint64_t m, uint32_t reps, double *sec)
 {

@@ -25,6 +25,8 @@ EXPORTS = [
     "pgh_bfe_eval", "pgh_bfe_eval_host", "pgh_batch_gradient", "pgh_batch_gradient_host",
     "pgh_fnds_create", "pgh_fnds_destroy", "pgh_fnds_run", "pgh_fnds_run_host", "pgh_fnds_sizes",
     "pgh_fnds_copy",
+    "pgh_crowding_distance", "pgh_crowding_distance_host", "pgh_sort_population_mo", "pgh_sort_population_mo_host",
+    "pgh_select_best_N_mo", "pgh_select_best_N_mo_host",
     "pgh_mt19937_generate", "pgh_batch_random_dv",
     "pgh_archi_create", "pgh_archi_destroy", "pgh_archi_evolve", "pgh_archi_get", "pgh_archi_set_trace",
     "pgh_archi_get_trace", "pgh_pso_gen_evolve_host",
@@ -90,6 +92,12 @@ def _declare(L):
         "pgh_fnds_run_host": [vp, vp, u64, u64],
         "pgh_fnds_sizes": [vp, C.POINTER(u64), C.POINTER(u64)],
         "pgh_fnds_copy": [vp, vp, vp, vp, vp, vp, vp, ci],
+        "pgh_crowding_distance": [vp, vp, u64, u64, vp, vp],
+        "pgh_crowding_distance_host": [vp, u64, u64, vp],
+        "pgh_sort_population_mo": [vp, vp, u64, u64, vp, vp],
+        "pgh_sort_population_mo_host": [vp, u64, u64, vp],
+        "pgh_select_best_N_mo": [vp, vp, u64, u64, u64, vp, C.POINTER(u64), vp],
+        "pgh_select_best_N_mo_host": [vp, u64, u64, u64, vp, C.POINTER(u64)],
         "pgh_mt19937_generate": [u32, u64, u64, vp, vp],
         "pgh_batch_random_dv": [udp_p, u64, u32, vp, vp],
         "pgh_archi_create": [C.POINTER(vp), C.POINTER(pgh_island_spec), u64, C.POINTER(vp), C.POINTER(vp)],
@@ -218,6 +226,24 @@ class Fnds:
         return r
 
 
+    # crowding distance and the orderings built on fnds (these replace the
+    # workspace's fnds results and synchronise)
+    def crowding_distance(self, points, out, stream=None):
+        n, m = points.shape
+        check(lib().pgh_crowding_distance(self.h, points.data_ptr(), n, m, out.data_ptr(), _stream(stream)))
+
+    def sort_population_mo(self, objs, out, stream=None):
+        n, m = objs.shape
+        check(lib().pgh_sort_population_mo(self.h, objs.data_ptr(), n, m, out.data_ptr(), _stream(stream)))
+
+    def select_best_N_mo(self, objs, N, out, stream=None):
+        n, m = objs.shape
+        ln = C.c_uint64()
+        check(lib().pgh_select_best_N_mo(self.h, objs.data_ptr(), n, m, N, out.data_ptr(), C.byref(ln),
+                                         _stream(stream)))
+        return ln.value
+
+
 # ---- host entry points (numpy) ------------------------------------------------
 
 def bfe_eval_host(kind, dim, dvs: np.ndarray, centers=None, nobj=1) -> np.ndarray:
@@ -300,3 +326,32 @@ class Archipelago:
         T = np.empty((self.gens[k], pop))
         check(lib().pgh_archi_get_trace(self.h, k, T.ctypes.data))
         return T
+
+
+def _mat(objs):
+    objs = np.ascontiguousarray(objs, np.float64)
+    if objs.ndim != 2:
+        objs = objs.reshape(len(objs), -1)
+    return objs, objs.shape[0], objs.shape[1]
+
+
+def crowding_distance_host(points) -> np.ndarray:
+    pts, n, m = _mat(points)
+    out = np.empty(max(n, 1))
+    check(lib().pgh_crowding_distance_host(_hptr(pts) if pts.size else None, n, m, _hptr(out)))
+    return out[:n]
+
+
+def sort_population_mo_host(objs) -> np.ndarray:
+    pts, n, m = _mat(objs)
+    out = np.empty(max(n, 1), np.uint64)
+    check(lib().pgh_sort_population_mo_host(_hptr(pts) if pts.size else None, n, m, _hptr(out)))
+    return out[:n]
+
+
+def select_best_N_mo_host(objs, N) -> np.ndarray:
+    pts, n, m = _mat(objs)
+    out = np.empty(max(n, 1), np.uint64)
+    ln = C.c_uint64()
+    check(lib().pgh_select_best_N_mo_host(_hptr(pts) if pts.size else None, n, m, N, _hptr(out), C.byref(ln)))
+    return out[: ln.value]

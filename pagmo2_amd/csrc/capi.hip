@@ -175,11 +175,6 @@ pgh_status stage_centers(prob &p, const pgh_udp *udp, dev_buf &buf, hipStream_t 
     return PGH_OK;
 }
 
-#define PGH_TRY(expr)                                                                                                \
-    do {                                                                                                             \
-        const pgh_status pgh_st_ = (expr);                                                                           \
-        if (pgh_st_ != PGH_OK) return pgh_st_;                                                                       \
-    } while (0)
 
 } // namespace
 } // namespace pgh

@@ -31,17 +31,7 @@
 
 #include "pgh_internal.hpp"
 
-struct pgh_fnds_ws {
-    hipStream_t stream = nullptr;
-    hipStream_t own = nullptr; // created by pgh_fnds_run_host
-    pgh::dev_buf staged;
-    uint64_t n = 0, m = 0, words = 0;
-    pgh::dev_buf keys, D, cnt32, ndom32, key32, dom_count, rank, front_items, front_offsets, dom_offsets, dom_items,
-        scalars;
-    pgh::dev_buf comp, ctl, lvl;
-    uint64_t h_sizes[3] = {0, 0, 0}; // n_fronts, n_dom_items, peel barrier timeout
-    bool sized = false;
-};
+#include "fnds_ws.hpp"
 
 namespace pgh
 {
@@ -51,13 +41,6 @@ namespace
 constexpr uint32_t kBlock = 256;
 constexpr uint32_t kPeelBlock = 1024;
 
-__device__ __forceinline__ int64_t order_key(double x)
-{
-    if (x != x) return INT64_MAX;
-    if (x == 0.0) return 0; // -0 == +0
-    const int64_t b = __double_as_longlong(x);
-    return b >= 0 ? b : (b ^ INT64_MAX);
-}
 
 __global__ __launch_bounds__(kBlock) void k_keys(const double *__restrict__ objs, uint64_t count,
                                                  int64_t *__restrict__ keys)
@@ -560,11 +543,6 @@ uint32_t grid_for(uint64_t count)
     return static_cast<uint32_t>((count + kBlock - 1u) / kBlock);
 }
 
-#define PGH_TRY(expr)                                                                                                \
-    do {                                                                                                             \
-        const pgh_status pgh_st_ = (expr);                                                                           \
-        if (pgh_st_ != PGH_OK) return pgh_st_;                                                                       \
-    } while (0)
 
 // Launches k_front0 and the grid-wide peel.
 pgh_status launch_grid_peel(pgh_fnds_ws *ws, hipStream_t s, uint32_t n32, uint32_t w32)

@@ -22,6 +22,13 @@ pgh_status hip_fail(hipError_t e, const char *what);
         if (pgh_e_ != hipSuccess) return ::pgh::hip_fail(pgh_e_, #call);                                             \
     } while (0)
 
+// Propagates a non-OK status.
+#define PGH_TRY(expr)                                                                                                \
+    do {                                                                                                             \
+        const pgh_status pgh_st_ = (expr);                                                                           \
+        if (pgh_st_ != PGH_OK) return pgh_st_;                                                                       \
+    } while (0)
+
 // ---- problem description after validation ---------------------------------
 struct prob {
     int kind = 0;

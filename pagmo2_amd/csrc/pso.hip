@@ -324,11 +324,6 @@ __global__ __launch_bounds__(kBlock) void k_eval(const island_desc *__restrict__
     }
 }
 
-#define PGH_TRY(expr)                                                                                                \
-    do {                                                                                                             \
-        const pgh_status pgh_st_ = (expr);                                                                           \
-        if (pgh_st_ != PGH_OK) return pgh_st_;                                                                       \
-    } while (0)
 
 uint64_t align_up(uint64_t v, uint64_t a)
 {

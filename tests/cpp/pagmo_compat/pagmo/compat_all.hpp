@@ -407,6 +407,30 @@ inline fnds_return_type fast_non_dominated_sorting(const std::vector<vector_doub
     return std::make_tuple(r.fronts, r.dom_list, r.dom_count, r.rank);
 }
 
+inline vector_double crowding_distance(const std::vector<vector_double> &front)
+{
+    const auto n = front.size(), m = n ? front[0].size() : 0u;
+    std::vector<double> flat;
+    for (const auto &v : front) flat.insert(flat.end(), v.begin(), v.end());
+    return oracle::crowding_distance(flat.data(), n, m);
+}
+
+inline std::vector<pop_size_t> sort_population_mo(const std::vector<vector_double> &f)
+{
+    const auto n = f.size(), m = n ? f[0].size() : 0u;
+    std::vector<double> flat;
+    for (const auto &v : f) flat.insert(flat.end(), v.begin(), v.end());
+    return oracle::sort_population_mo(flat.data(), n, m);
+}
+
+inline std::vector<pop_size_t> select_best_N_mo(const std::vector<vector_double> &f, pop_size_t N)
+{
+    const auto n = f.size(), m = n ? f[0].size() : 0u;
+    std::vector<double> flat;
+    for (const auto &v : f) flat.insert(flat.end(), v.begin(), v.end());
+    return oracle::select_best_N_mo(flat.data(), n, m, N);
+}
+
 // ---- population and pso_gen ------------------------------------------------------
 inline oracle::udp to_oracle(const problem &p)
 {

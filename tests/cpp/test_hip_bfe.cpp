@@ -9,9 +9,11 @@
 //
 // Built against pagmo's headers when they exist, and otherwise against the
 // stand-in in tests/cpp/pagmo_compat.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <random>
 #include <string>
 #include <type_traits>
@@ -226,6 +228,41 @@ static void gpu_tests()
         const auto m = throws<std::invalid_argument>(
             [&] { pagmo_hip::fast_non_dominated_sorting(std::vector<vector_double>{{1., 2.}}); });
         CHECK(m == "At least two points are needed for fast_non_dominated_sorting: 1 detected.");
+    }
+
+    // crowding_distance / sort_population_mo / select_best_N_mo: the reference's
+    // known answers (tests/multi_objective.cpp:147-262) and agreement with the
+    // CPU restatement on a larger tie-free set
+    {
+        const double inf = std::numeric_limits<double>::infinity();
+        CHECK((pagmo_hip::crowding_distance({{0, 0}, {-1, 1}, {2, -2}}) == vector_double{2, inf, inf}));
+        CHECK((pagmo_hip::crowding_distance({{0, 0}, {1, -1}, {2, -2}, {4, -4}}) == vector_double{inf, 1., 1.5, inf}));
+        CHECK((pagmo_hip::crowding_distance({{0, 0}, {0, 0}}) == vector_double{inf, inf}));
+        CHECK(throws<std::invalid_argument>([] { pagmo_hip::crowding_distance({{1, 2}}); })
+              == "A non dominated front must contain at least two points: 1 detected.");
+        CHECK(throws<std::invalid_argument>([] { pagmo_hip::crowding_distance({{1}, {2}}); })
+              == "Points in the non dominated front must contain at least two objectives: 1 detected.");
+        CHECK(throws<std::invalid_argument>([] { pagmo_hip::crowding_distance({{2, 3}, {3, 4}, {2, 4, 5}}); })
+              == "A non dominated front must contain points of uniform dimensionality. Some different sizes were "
+                 "instead detected.");
+        const std::vector<vector_double> ex = {{0, 7}, {1, 5}, {2, 3}, {4, 2}, {7, 1}, {10, 0},
+                                               {2, 6}, {4, 4}, {10, 2}, {6, 6}, {9, 5}};
+        CHECK((pagmo_hip::sort_population_mo(ex) == std::vector<pop_size_t>{0, 5, 4, 3, 1, 2, 6, 8, 7, 9, 10}));
+        CHECK((pagmo_hip::select_best_N_mo({{1, 2}, {2, 4}, {-3, 2}, {-3, -3}}, 4u) == std::vector<pop_size_t>{0, 1, 2, 3}));
+        CHECK(pagmo_hip::select_best_N_mo({{1, 2}, {3, 4}}, 0u).empty());
+        std::mt19937 e(11);
+        std::uniform_real_distribution<double> u(0., 1.);
+        std::vector<vector_double> pts(600, vector_double(3));
+        for (auto &v : pts)
+            for (auto &x : v) x = u(e);
+        CHECK(pagmo_hip::crowding_distance(std::vector<vector_double>(pts.begin(), pts.begin() + 40))
+              == crowding_distance(std::vector<vector_double>(pts.begin(), pts.begin() + 40)));
+        // equal (rank, crowding) pairs (the fronts' infinite ends) are ordered by
+        // index on the device: the restatement's std::stable_sort form
+        std::vector<double> flat;
+        for (const auto &v : pts) flat.insert(flat.end(), v.begin(), v.end());
+        CHECK(pagmo_hip::sort_population_mo(pts) == oracle::sort_population_mo(flat.data(), 600u, 3u, true));
+        CHECK(pagmo_hip::select_best_N_mo(pts, 250u) == oracle::select_best_N_mo(flat.data(), 600u, 3u, 250u, true));
     }
 }
 

@@ -17,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def test_exports_match_header():
     with open(os.path.join(ROOT, "include", "pagmo_hip", "pgh.h")) as f:
         hdr = f.read()
-    declared = set(re.findall(r"\b(pgh_[a-z0-9_]+)\s*\(", hdr))
+    declared = set(re.findall(r"\b(pgh_[A-Za-z0-9_]+)\s*\(", hdr))
     assert declared == set(_capi.EXPORTS)
     assert P.missing_exports() == []
 
@@ -96,3 +96,19 @@ def test_no_device_reported():
     out = np.zeros(2)
     rc = P.lib().pgh_bfe_eval_host(C.byref(u), dvs.ctypes.data, 6, out.ctypes.data, 2)
     assert rc == P.NO_DEVICE
+
+
+def test_crowding_and_selection_host_checks():
+    # validations and corner cases answered before any device work
+    # (multi_objective.cpp:283-298, 346-360, 427-434)
+    with pytest.raises(P.PghError, match="A non dominated front must contain at least two points: 1 detected."):
+        P.crowding_distance_host(np.zeros((1, 2)))
+    with pytest.raises(P.PghError, match="Points in the non dominated front must contain at least two objectives: "
+                                         "1 detected."):
+        P.crowding_distance_host(np.zeros((3, 1)))
+    assert list(P.select_best_N_mo_host(np.zeros((0, 2)), 10)) == []
+    assert list(P.select_best_N_mo_host(np.zeros((5, 2)), 0)) == []
+    assert list(P.select_best_N_mo_host(np.zeros((1, 2)), 10)) == [0]
+    assert list(P.select_best_N_mo_host(np.random.default_rng(0).random((4, 2)), 4)) == [0, 1, 2, 3]
+    assert list(P.sort_population_mo_host(np.zeros((0, 2)))) == []
+    assert list(P.sort_population_mo_host(np.zeros((1, 4)))) == [0]

@@ -352,3 +352,89 @@ def test_pso_gen_single_generation_independent():
     Xp, Fp = _pso_one_gen_python(X, F[:, 0], 77, lb, ub, lambda r: O.fitness_one(O.ROSENBROCK, 5, r)[0])
     assert np.array_equal(Xo, Xp)
     assert np.array_equal(Fo, Fp)
+
+
+# ---- crowding distance and multi-objective sorting (multi_objective.cpp:280-465)
+
+def _f(v):
+    return float("inf") if v == "inf" else float(v)
+
+
+def _mat(points):
+    return np.array(points, np.float64).reshape(len(points), -1 if points else 0)
+
+
+@pytest.mark.parametrize("stable", [False, True])
+def test_crowding_distance_kats(kats, stable):
+    for k in kats["crowding_distance"]:
+        r = O.crowding_distance(_mat(k["points"]), stable=stable)
+        assert list(r) == [_f(v) for v in k["result"]], k["src"]
+    for k in kats["crowding_distance_throws"]:
+        with pytest.raises(O.OracleError, match=k["match"]):
+            O.crowding_distance(_mat(k["points"]), stable=stable)
+
+
+@pytest.mark.parametrize("stable", [False, True])
+def test_sort_population_mo_kats(kats, stable):
+    for k in kats["sort_population_mo"]:
+        assert list(O.sort_population_mo(_mat(k["points"]), stable=stable)) == k["result"], k["src"]
+
+
+@pytest.mark.parametrize("stable", [False, True])
+def test_select_best_N_mo_kats(kats, stable):
+    for k in kats["select_best_N_mo"]:
+        assert list(O.select_best_N_mo(_mat(k["points"]), k["N"], stable=stable)) == k["result"], k["src"]
+    for k in kats["select_best_N_mo_permutation"]:
+        pts = _mat(k["points"])
+        full = list(O.sort_population_mo(pts, stable=stable))
+        for N in range(1, len(pts) + 3):
+            sel = list(O.select_best_N_mo(pts, N, stable=stable))
+            assert sorted(sel) == sorted(full[: len(sel)]), (k["src"], N)
+
+
+def _crowding_brute(pts):
+    """Crowding distance from its definition, with the neighbours in each
+    objective found by value (distinct values only)."""
+    n, m = pts.shape
+    out = np.zeros(n)
+    for i in range(m):
+        order = np.argsort(pts[:, i], kind="stable")
+        df = pts[order[-1], i] - pts[order[0], i]
+        for j in range(n):
+            pos = int(np.where(order == j)[0][0])
+            if pos in (0, n - 1):
+                out[j] = np.inf
+            elif np.isfinite(out[j]):
+                out[j] += (pts[order[pos + 1], i] - pts[order[pos - 1], i]) / df
+    return out
+
+
+@pytest.mark.parametrize("n,m", [(2, 2), (5, 3), (40, 2), (120, 4)])
+def test_crowding_distance_brute_force(n, m):
+    # distinct values: the sort order is unique and the definition applies
+    pts = np.random.default_rng(n * m).permutation(n * m * 3)[: n * m].reshape(n, m) / 7.0
+    np.testing.assert_array_equal(O.crowding_distance(pts), _crowding_brute(pts))
+    np.testing.assert_array_equal(O.crowding_distance(pts, stable=True), _crowding_brute(pts))
+
+
+def test_sort_population_mo_properties():
+    # ordered by (rank asc, crowding desc); the stable variant breaks the
+    # remaining ties by index; both are permutations of the population
+    rng = np.random.default_rng(4)
+    pts = rng.integers(0, 8, size=(300, 2)).astype(np.float64)
+    r = O.fnds(pts)
+    rank = r["rank"].astype(np.int64)
+    crowd = np.zeros(len(pts))
+    fo, fi = r["front_offsets"], r["front_items"]
+    for f in range(len(fo) - 1):
+        front = fi[fo[f]:fo[f + 1]].astype(np.int64)
+        if len(front) > 1:
+            crowd[front] = O.crowding_distance(pts[front], stable=True)
+    for stable in (False, True):
+        order = O.sort_population_mo(pts, stable=stable).astype(np.int64)
+        assert sorted(order) == list(range(len(pts)))
+        key = [(rank[i], -crowd[i]) for i in order]
+        if stable:
+            assert key == sorted(key)
+            key_i = [(rank[i], -crowd[i], i) for i in order]
+            assert key_i == sorted(key_i)
