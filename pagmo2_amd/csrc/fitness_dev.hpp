@@ -238,19 +238,34 @@ __device__ __forceinline__ double separable_eval_impl(const double *__restrict__
         // used.  Pairs are added in increasing c either way.
         const double2 *row = reinterpret_cast<const double2 *>(x) + c;
         const uint32_t full = npairs / L;
+        const auto add_round = [&](const double2 &v, uint32_t j) {
+            acc.add(v.x, 2u * (c + j * L));
+            acc.add(v.y, 2u * (c + j * L) + 1u);
+        };
         if (full > 0u) {
-            double2 v = row[0];
-            for (uint32_t j = 1u; j < full; ++j) {
-                const double2 w = row[j * L];
-                // keep the load at the top of the round: the scheduler would
-                // otherwise sink it below the use of v to reuse v's registers
+            // two rounds per iteration with the loads alternating between a
+            // and b, so that no register copy moves a loaded pair: b's load
+            // is in flight while a is used, then a's next one while b is.
+            // The barriers keep each load above the use that follows it (the
+            // scheduler would otherwise sink it to reuse registers).
+            double2 a = row[0];
+            uint32_t j = 1u;
+            for (; j + 1u < full; j += 2u) {
+                const double2 b = row[j * L];
                 __builtin_amdgcn_sched_barrier(0);
-                acc.add(v.x, 2u * (c + (j - 1u) * L));
-                acc.add(v.y, 2u * (c + (j - 1u) * L) + 1u);
-                v = w;
+                add_round(a, j - 1u);
+                a = row[(j + 1u) * L];
+                __builtin_amdgcn_sched_barrier(0);
+                add_round(b, j);
             }
-            acc.add(v.x, 2u * (c + (full - 1u) * L));
-            acc.add(v.y, 2u * (c + (full - 1u) * L) + 1u);
+            if (j < full) {
+                const double2 b = row[j * L];
+                __builtin_amdgcn_sched_barrier(0);
+                add_round(a, j - 1u);
+                add_round(b, j);
+            } else {
+                add_round(a, j - 1u);
+            }
         }
         if (c + full * L < npairs) {
             const double2 v = row[full * L];
