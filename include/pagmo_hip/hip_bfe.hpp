@@ -23,7 +23,8 @@
 // batched gradient (problem::gradient, src/problem.cpp:456-467, for a whole
 // population), fast_non_dominated_sorting, crowding_distance,
 // sort_population_mo and select_best_N_mo (src/utils/multi_objective.cpp:
-// 200-465) on the device.
+// 200-465) on the device, and local_descent, a memetic local step on the
+// batched gradient for a population between two evolve() calls.
 #pragma once
 
 #include <cstdint>
@@ -35,6 +36,7 @@
 #include <pagmo/batch_evaluators/member_bfe.hpp>
 #include <pagmo/batch_evaluators/thread_bfe.hpp>
 #include <pagmo/exceptions.hpp>
+#include <pagmo/population.hpp>
 #include <pagmo/problem.hpp>
 #include <pagmo/problems/ackley.hpp>
 #include <pagmo/problems/griewank.hpp>
@@ -169,6 +171,41 @@ inline pagmo::vector_double batch_gradient(const pagmo::problem &p, const pagmo:
     pagmo::vector_double g(dvs.size());
     detail::check(pgh_batch_gradient_host(&u, dvs.data(), dvs.size(), g.data(), g.size(), nullptr));
     return g;
+}
+
+// Memetic local step: `iters` rounds of the projected gradient descent of
+// pgh_local_descent (pagmo_hip/pgh.h) on every individual of `pop`, inside the
+// problem's bounds, starting from the step length `step0`.  Individuals are
+// written back with population::set_xf (src/population.cpp:490-519), which
+// keeps the champion current; an individual's fitness never increases.  No
+// algorithm's selection logic is involved: a caller interleaves it with
+// evolve(), e.g.
+//     for (...) { pop = algo.evolve(pop); pagmo_hip::local_descent(pop, 5u, 0.1); }
+// The problem's fevals grow by (iters + 1) * pop.size(), the fitness
+// evaluations made on the device.
+inline void local_descent(pagmo::population &pop, unsigned iters, double step0)
+{
+    const pagmo::problem &p = pop.get_problem();
+    pgh_udp u;
+    if (!describe(p, u)) {
+        pagmo_throw(std::invalid_argument, "local_descent: no device gradient for the problem '" + p.get_name() + "'");
+    }
+    const auto n = pop.size();
+    if (n == 0u) return;
+    const auto nx = p.get_nx();
+    std::vector<double> x;
+    x.reserve(n * nx);
+    for (const auto &xi : pop.get_x()) x.insert(x.end(), xi.begin(), xi.end());
+    std::vector<double> f(n);
+    const auto bounds = p.get_bounds();
+    detail::check(pgh_local_descent_host(&u, x.data(), x.size(), f.data(), f.size(), bounds.first.data(),
+                                         bounds.second.data(), iters, step0));
+    for (decltype(pop.size()) i = 0; i < n; ++i) {
+        pop.set_xf(i, pagmo::vector_double(x.begin() + static_cast<std::ptrdiff_t>(i * nx),
+                                           x.begin() + static_cast<std::ptrdiff_t>((i + 1u) * nx)),
+                   pagmo::vector_double{f[i]});
+    }
+    p.increment_fevals(static_cast<unsigned long long>(iters + 1u) * n);
 }
 
 namespace detail

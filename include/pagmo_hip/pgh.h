@@ -105,6 +105,32 @@ pgh_status pgh_batch_gradient(const pgh_udp *udp, const double *dvs, uint64_t n_
 pgh_status pgh_batch_gradient_host(const pgh_udp *udp, const double *dvs, uint64_t dvs_len, double *grad,
                                    uint64_t grad_len, double *fvs);
 
+/* ---- memetic local step on the batched gradient.  No counterpart in the
+ *      reference: it feeds the gradient of pgh_batch_gradient into a local
+ *      refinement that a caller applies to a population between two calls of
+ *      an algorithm's evolve() (pagmo's population::set_xf,
+ *      src/population.cpp:490-519), so no algorithm's selection logic changes.
+ *      For each individual, `iters` rounds of
+ *          m = max_j |g_j|,  s = alpha / m,  y_j = x_j - s * g_j,
+ *          y_j = lb_j if y_j < lb_j,  y_j = ub_j if y_j > ub_j,
+ *          accept y (x = y, f = f(y), alpha = 2 alpha) if f(y) < f(x),
+ *          otherwise alpha = alpha / 2,
+ *      starting from alpha = step0.  A gradient with a non-finite element or
+ *      m == 0 gives y = x, which is rejected.  Problems with a device
+ *      gradient only (rosenbrock, rastrigin, lennard_jones; nf = 1).
+ *      dvs (n_dvs x nx) is updated in place; fvs (n_dvs) receives f(x) at the
+ *      end.  lb and ub (nx each) are both NULL (no clamping) or both given.
+ *      work holds at least pgh_local_descent_work_len(n_dvs, nx) doubles of
+ *      device memory; the device form allocates nothing. */
+uint64_t pgh_local_descent_work_len(uint64_t n_dvs, uint64_t nx);
+pgh_status pgh_local_descent(const pgh_udp *udp, double *dvs, uint64_t n_dvs, uint64_t nx, double *fvs,
+                             const double *lb, const double *ub, uint32_t iters, double step0, double *work,
+                             uint64_t work_len, void *stream);
+/* Host-buffer form: dvs (dvs_len = n_dvs * nx) in/out, fvs (n_dvs) out;
+ * lb/ub host arrays of nx (or both NULL).  Synchronous. */
+pgh_status pgh_local_descent_host(const pgh_udp *udp, double *dvs, uint64_t dvs_len, double *fvs, uint64_t fvs_len,
+                                  const double *lb, const double *ub, uint32_t iters, double step0);
+
 /* ---- fast_non_dominated_sorting (src/utils/multi_objective.cpp:200-257,
  *      return type include/pagmo/utils/multi_objective.hpp:68-72) over a
  *      device-resident n x m objective matrix.  Outputs are CSR:

@@ -54,6 +54,7 @@ def _declare(L):
         "orc_bfe": [cint, u64, C.c_void_p, u64, _dp, u64, u64, _dp, u32],
         "orc_bfe_timed": [cint, u64, C.c_void_p, u64, _dp, u64, u64, _dp, u32, u32, C.POINTER(dbl)],
         "orc_gradient": [cint, u64, _dp, u64, u64, _dp],
+        "orc_local_descent": [cint, u64, _dp, u64, _dp, C.c_void_p, C.c_void_p, u32, dbl],
         "orc_estimate_gradient": [cint, u64, C.c_void_p, u64, _dp, u64, dbl, cint, _dp],
         "orc_estimate_gradient_kat": [_dp, dbl, cint, _dp],
         "orc_pareto_dominance": [_dp, _dp, u64, C.POINTER(cint)],
@@ -153,6 +154,18 @@ def gradient(kind: int, dim: int, dvs: np.ndarray) -> np.ndarray:
     out = np.empty((n, nx))
     _chk(lib().orc_gradient(kind, dim, dvs, n, nx, out))
     return out
+
+
+def local_descent(kind: int, dim: int, dvs: np.ndarray, iters: int, step0: float, lb=None, ub=None):
+    """The memetic local step (no reference counterpart) on a copy of dvs
+    (n x nx); returns (x, f)."""
+    x = np.array(dvs, dtype=np.float64, order="C", copy=True)
+    n = x.shape[0]
+    f = np.empty(max(n, 1))
+    lbv = None if lb is None else np.ascontiguousarray(lb, np.float64)
+    ubv = None if ub is None else np.ascontiguousarray(ub, np.float64)
+    _chk(lib().orc_local_descent(kind, dim, x, n, f, _cptr(lbv), _cptr(ubv), iters, step0))
+    return x, f[:n]
 
 
 def estimate_gradient(kind: int, dim: int, x, dx: float, high: bool, centers=None, nobj: int = 1):

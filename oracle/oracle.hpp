@@ -23,6 +23,11 @@
 //     libstdc++'s generate_canonical source (bits/random.tcc).
 //   * LJ gradient (no reference implementation): central finite differences
 //     and estimate_gradient_h.
+//   * crowding_distance / select_best_N_mo / sort_population_mo:
+//     tests/multi_objective.cpp:147-262 plus brute-force crowding on ties.
+//   * local_descent (the memetic step, no reference implementation):
+//     hand-derived traces (rastrigin from 0.1, rosenbrock from the origin)
+//     and invariants (f never increases, f is the fitness of x, bounds hold).
 //   * pso_gen trajectories: PARITY UNPINNED by the reference (its tests only
 //     check self-consistency, tests/pso_gen.cpp:154-240,302-336); pinned here by
 //     the same self-consistency checks plus hand-checked single steps.
@@ -432,6 +437,59 @@ inline vector_double bfe_call(const udp &p, const vector_double &dvs, unsigned n
     auto retval = thread_bfe(p, dvs, n_threads);
     bfe_check_output_fvs(p, dvs, retval);
     return retval;
+}
+
+// ---------------------------------------------------------------------------
+// Memetic local step on the analytic gradient.  The reference has no such
+// routine; this states the rule of pgh_local_descent (pagmo_hip/pgh.h) one
+// individual at a time: per round, m = max_j |g_j|, s = alpha / m,
+// y_j = x_j - s * g_j clamped to [lb_j, ub_j] (when bounds are given), and y
+// replaces x when f(y) < f(x) (alpha doubled, kept finite), otherwise alpha is
+// halved.  A non-finite gradient element or m == 0 leaves y = x.
+// ---------------------------------------------------------------------------
+inline void local_descent(const udp &p, double *X, std::size_t n, double *F, const double *lb, const double *ub,
+                          unsigned iters, double step0)
+{
+    const std::size_t nx = p.get_nx();
+    for (std::size_t i = 0; i < n; ++i) {
+        vector_double x(X + i * nx, X + (i + 1u) * nx);
+        double f = p.fitness(x)[0];
+        double alpha = step0;
+        for (unsigned it = 0; it < iters; ++it) {
+            const auto g = p.gradient(x);
+            double m = 0.;
+            bool bad = false;
+            for (std::size_t j = 0; j < nx; ++j) {
+                const double a = std::abs(g[j]);
+                if (!(a <= std::numeric_limits<double>::max())) bad = true;
+                if (a > m) m = a;
+            }
+            const bool move = !bad && m > 0.;
+            const double s = move ? alpha / m : 0.;
+            vector_double y(x);
+            if (move) {
+                for (std::size_t j = 0; j < nx; ++j) {
+                    double v = x[j] - s * g[j];
+                    if (lb) {
+                        if (v < lb[j]) v = lb[j];
+                        if (v > ub[j]) v = ub[j];
+                    }
+                    y[j] = v;
+                }
+            }
+            const double fy = p.fitness(y)[0];
+            if (fy < f) {
+                x = y;
+                f = fy;
+                const double a2 = 2. * alpha;
+                if (a2 <= std::numeric_limits<double>::max()) alpha = a2;
+            } else {
+                alpha = 0.5 * alpha;
+            }
+        }
+        std::copy(x.begin(), x.end(), X + i * nx);
+        F[i] = f;
+    }
 }
 
 // ---------------------------------------------------------------------------

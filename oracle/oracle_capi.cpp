@@ -148,6 +148,18 @@ int orc_gradient(int kind, uint64_t dim, const double *dvs, uint64_t n_dvs, uint
     });
 }
 
+// Memetic local step on n_dvs x nx decision vectors, in place; F gets f(x).
+// lb and ub are both NULL (no clamping) or nx values each.
+int orc_local_descent(int kind, uint64_t dim, double *dvs, uint64_t n_dvs, double *fvs, const double *lb,
+                      const double *ub, uint32_t iters, double step0)
+{
+    return guarded([&] {
+        auto p = make_udp(kind, dim, nullptr, 1);
+        if (!p.has_gradient()) throw std::invalid_argument("the problem has no gradient");
+        oracle::local_descent(p, dvs, n_dvs, fvs, lb, ub, iters, step0);
+    });
+}
+
 // estimate_gradient (high = 0) or estimate_gradient_h (high = 1) of the
 // problem's fitness at x; out has nf * nx entries (layout grad[j + i*nx]).
 int orc_estimate_gradient(int kind, uint64_t dim, const double *centers, uint64_t nobj, const double *x, uint64_t nx,

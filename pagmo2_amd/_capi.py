@@ -27,6 +27,7 @@ EXPORTS = [
     "pgh_fnds_copy",
     "pgh_crowding_distance", "pgh_crowding_distance_host", "pgh_sort_population_mo", "pgh_sort_population_mo_host",
     "pgh_select_best_N_mo", "pgh_select_best_N_mo_host",
+    "pgh_local_descent_work_len", "pgh_local_descent", "pgh_local_descent_host",
     "pgh_mt19937_generate", "pgh_batch_random_dv",
     "pgh_archi_create", "pgh_archi_destroy", "pgh_archi_evolve", "pgh_archi_get", "pgh_archi_set_trace",
     "pgh_archi_get_trace", "pgh_pso_gen_evolve_host",
@@ -86,6 +87,8 @@ def _declare(L):
         "pgh_bfe_eval_host": [udp_p, vp, u64, vp, u64],
         "pgh_batch_gradient": [udp_p, vp, u64, u64, vp, vp, vp],
         "pgh_batch_gradient_host": [udp_p, vp, u64, vp, u64, vp],
+        "pgh_local_descent": [udp_p, vp, u64, u64, vp, vp, vp, u32, C.c_double, vp, u64, vp],
+        "pgh_local_descent_host": [udp_p, vp, u64, vp, u64, vp, vp, u32, C.c_double],
         "pgh_fnds_create": [C.POINTER(vp)],
         "pgh_fnds_destroy": [vp],
         "pgh_fnds_run": [vp, vp, u64, u64, vp],
@@ -116,6 +119,8 @@ def _declare(L):
         fn.restype = ci
     L.pgh_last_error.argtypes = []
     L.pgh_last_error.restype = C.c_char_p
+    L.pgh_local_descent_work_len.argtypes = [u64, u64]
+    L.pgh_local_descent_work_len.restype = u64
 
 
 def missing_exports():
@@ -172,6 +177,23 @@ def batch_gradient(kind, dim, dvs, grad, fvs=None, stream=None):
     u = make_udp(kind, dim)
     n, nx = dvs.shape
     check(lib().pgh_batch_gradient(C.byref(u), dvs.data_ptr(), n, nx, grad.data_ptr(), _dptr(fvs), _stream(stream)))
+
+
+def local_descent_work_len(n, nx):
+    return lib().pgh_local_descent_work_len(n, nx)
+
+
+def local_descent(kind, dim, dvs, fvs, iters, step0, lb=None, ub=None, work=None, stream=None):
+    """Memetic local step on device tensors: dvs (n x nx) in place, fvs (n)
+    out; lb/ub device tensors of nx or None; work is allocated when None."""
+    u = make_udp(kind, dim)
+    n, nx = dvs.shape
+    need = local_descent_work_len(n, nx)
+    if work is None:
+        import torch
+        work = torch.empty(max(need, 1), dtype=torch.float64, device=dvs.device)
+    check(lib().pgh_local_descent(C.byref(u), dvs.data_ptr(), n, nx, fvs.data_ptr(), _dptr(lb), _dptr(ub), iters,
+                                  step0, work.data_ptr(), work.numel(), _stream(stream)))
 
 
 def mt19937_generate(seed, n, out, skip=0, stream=None):
@@ -265,6 +287,19 @@ def batch_gradient_host(kind, dim, dvs: np.ndarray, with_fitness=False):
     f = np.empty(dvs.size // nx) if with_fitness else None
     check(lib().pgh_batch_gradient_host(C.byref(u), dvs.ctypes.data, dvs.size, g.ctypes.data, g.size, _hptr(f)))
     return (g, f) if with_fitness else g
+
+
+def local_descent_host(kind, dim, dvs: np.ndarray, iters, step0, lb=None, ub=None):
+    """Memetic local step on host arrays; returns (x, f) (copies)."""
+    x = np.array(dvs, np.float64, order="C", copy=True)
+    u = make_udp(kind, dim)
+    n = x.shape[0] if x.ndim == 2 else 0
+    f = np.empty(max(n, 1))
+    lbv = None if lb is None else np.ascontiguousarray(lb, np.float64)
+    ubv = None if ub is None else np.ascontiguousarray(ub, np.float64)
+    check(lib().pgh_local_descent_host(C.byref(u), x.ctypes.data, x.size, f.ctypes.data, n, _hptr(lbv), _hptr(ubv),
+                                       iters, step0))
+    return x, f[:n]
 
 
 def pso_gen_evolve_host(kind, dim, X, F, prm: pgh_pso_params):

@@ -1,5 +1,6 @@
 // capi.hip -- the extern "C" entry points of libpagmo_hip.so: validation,
 // error reporting, host staging and dispatch to the kernel launchers.
+#include <cfloat>
 #include <cstring>
 #include <limits>
 #include <string>
@@ -163,6 +164,20 @@ pgh_status check_input_len(const prob &p, uint64_t len, uint64_t &n_dvs)
     return PGH_OK;
 }
 
+// pgh_local_descent's arguments beyond the problem and the buffers
+pgh_status check_local_descent(const prob &p, const double *lb, const double *ub, double step0)
+{
+    PGH_TRY(check_has_gradient(p));
+    if ((lb == nullptr) != (ub == nullptr)) {
+        return fail(PGH_INVALID_ARGUMENT, "the bounds of a local descent are both given or both absent");
+    }
+    if (!(step0 > 0.) || !(step0 <= DBL_MAX)) {
+        return fail(PGH_INVALID_ARGUMENT, "the initial step of a local descent must be positive and finite, got "
+                                              + std::to_string(step0));
+    }
+    return PGH_OK;
+}
+
 // Copies host MO_SPHERE centres to the device buffer `buf` and repoints p.
 pgh_status stage_centers(prob &p, const pgh_udp *udp, dev_buf &buf, hipStream_t s)
 {
@@ -174,7 +189,6 @@ pgh_status stage_centers(prob &p, const pgh_udp *udp, dev_buf &buf, hipStream_t 
     p.centers = static_cast<const double *>(buf.ptr);
     return PGH_OK;
 }
-
 
 } // namespace
 } // namespace pgh
@@ -294,6 +308,68 @@ pgh_status pgh_batch_gradient_host(const pgh_udp *udp, const double *dvs, uint64
     if (fvs) {
         PGH_HIP(hipMemcpyAsync(fvs, ctx->d.ptr, n_dvs * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     }
+    PGH_HIP(hipStreamSynchronize(ctx->stream));
+    return PGH_OK;
+}
+
+uint64_t pgh_local_descent_work_len(uint64_t n_dvs, uint64_t nx)
+{
+    return local_descent_work_len(n_dvs, nx);
+}
+
+pgh_status pgh_local_descent(const pgh_udp *udp, double *dvs, uint64_t n_dvs, uint64_t nx, double *fvs,
+                             const double *lb, const double *ub, uint32_t iters, double step0, double *work,
+                             uint64_t work_len, void *stream)
+{
+    prob p;
+    PGH_TRY(resolve(udp, p));
+    PGH_TRY(check_shape(p, nx, p.nf));
+    PGH_TRY(check_local_descent(p, lb, ub, step0));
+    if (n_dvs == 0u) return PGH_OK;
+    if (!dvs || !fvs || !work) return fail(PGH_INVALID_ARGUMENT, "null buffer");
+    const uint64_t need = local_descent_work_len(n_dvs, nx);
+    if (work_len < need) {
+        return fail(PGH_INVALID_ARGUMENT, "the work buffer holds " + std::to_string(work_len) + " doubles, while "
+                                              + std::to_string(need) + " are needed");
+    }
+    return launch_local_descent(p, dvs, n_dvs, fvs, lb, ub, iters, step0, work, static_cast<hipStream_t>(stream));
+}
+
+pgh_status pgh_local_descent_host(const pgh_udp *udp, double *dvs, uint64_t dvs_len, double *fvs, uint64_t fvs_len,
+                                  const double *lb, const double *ub, uint32_t iters, double step0)
+{
+    prob p;
+    PGH_TRY(resolve(udp, p));
+    PGH_TRY(check_local_descent(p, lb, ub, step0));
+    uint64_t n_dvs = 0;
+    PGH_TRY(check_input_len(p, dvs_len, n_dvs));
+    if (fvs_len != n_dvs * p.nf) {
+        return fail(PGH_INVALID_ARGUMENT, "the fitness buffer holds " + std::to_string(fvs_len)
+                                              + " values, while " + std::to_string(n_dvs * p.nf) + " are produced");
+    }
+    if (n_dvs == 0u) return PGH_OK;
+    thread_ctx *ctx = nullptr;
+    PGH_TRY(get_thread_ctx(&ctx));
+    const size_t x_bytes = dvs_len * sizeof(double);
+    PGH_TRY(ctx->a.reserve(x_bytes));
+    PGH_TRY(ctx->b.reserve(local_descent_work_len(n_dvs, p.nx) * sizeof(double)));
+    PGH_TRY(ctx->d.reserve(n_dvs * sizeof(double)));
+    const double *dlb = nullptr, *dub = nullptr;
+    if (lb) {
+        PGH_TRY(ctx->c.reserve(2u * p.nx * sizeof(double)));
+        double *c = static_cast<double *>(ctx->c.ptr);
+        PGH_HIP(hipMemcpyAsync(c, lb, p.nx * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        PGH_HIP(hipMemcpyAsync(c + p.nx, ub, p.nx * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        dlb = c;
+        dub = c + p.nx;
+    }
+    double *x = static_cast<double *>(ctx->a.ptr);
+    double *f = static_cast<double *>(ctx->d.ptr);
+    PGH_HIP(hipMemcpyAsync(x, dvs, x_bytes, hipMemcpyHostToDevice, ctx->stream));
+    PGH_TRY(launch_local_descent(p, x, n_dvs, f, dlb, dub, iters, step0, static_cast<double *>(ctx->b.ptr),
+                                 ctx->stream));
+    PGH_HIP(hipMemcpyAsync(dvs, x, x_bytes, hipMemcpyDeviceToHost, ctx->stream));
+    PGH_HIP(hipMemcpyAsync(fvs, f, n_dvs * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     PGH_HIP(hipStreamSynchronize(ctx->stream));
     return PGH_OK;
 }

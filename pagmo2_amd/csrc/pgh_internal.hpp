@@ -46,6 +46,14 @@ pgh_status launch_fitness(const prob &p, const double *dvs, uint64_t n_dvs, doub
 pgh_status launch_gradient(const prob &p, const double *dvs, uint64_t n_dvs, double *grad, double *fvs,
                            hipStream_t s);
 
+// Refuses grids beyond what one launch addresses (fitness.hip).
+pgh_status check_launch_size(uint64_t n, uint32_t nx);
+
+// ---- memetic local step (local.hip) -------------------------------------------
+uint64_t local_descent_work_len(uint64_t n, uint64_t nx);
+pgh_status launch_local_descent(const prob &p, double *x, uint64_t n, double *f, const double *lb, const double *ub,
+                                uint32_t iters, double step0, double *work, hipStream_t s);
+
 // ---- rng (pso.hip) ------------------------------------------------------------
 pgh_status launch_mt19937(uint32_t seed, uint64_t skip, uint64_t n, uint32_t *out, hipStream_t s);
 pgh_status launch_batch_random_dv(const prob &p, uint64_t n, uint32_t seed, double *out, hipStream_t s);

@@ -264,12 +264,87 @@ static void gpu_tests()
         CHECK(pagmo_hip::sort_population_mo(pts) == oracle::sort_population_mo(flat.data(), 600u, 3u, true));
         CHECK(pagmo_hip::select_best_N_mo(pts, 250u) == oracle::select_best_N_mo(flat.data(), 600u, 3u, 250u, true));
     }
+
+    // memetic local step: a hand-derived trace (rosenbrock at the origin:
+    // g = (-2, 0); alpha 0.5 gives (0.5, 0), f = 6.5, rejected; alpha 0.25
+    // gives (0.25, 0), f = 0.953125, accepted), then bit-exact parity with the
+    // oracle's statement of the rule on a seeded population
+    {
+        population pop{problem{rosenbrock{2u}}, 1u, 1u};
+        pop.set_xf(0u, {0., 0.}, {1.});
+        const auto fe = pop.get_problem().get_fevals();
+        pagmo_hip::local_descent(pop, 2u, 0.5);
+        CHECK((pop.get_x()[0] == vector_double{0.25, 0.}));
+        CHECK((pop.get_f()[0] == vector_double{0.953125}));
+        CHECK(pop.get_problem().get_fevals() == fe + 3u);
+
+        population p10{problem{rosenbrock{10u}}, 64u, 42u};
+        std::vector<double> X;
+        for (const auto &x : p10.get_x()) X.insert(X.end(), x.begin(), x.end());
+        std::vector<double> F(64);
+        oracle::udp ou;
+        ou.kind = oracle::ROSENBROCK;
+        ou.dim = 10u;
+        const auto b = ou.get_bounds();
+        oracle::local_descent(ou, X.data(), 64u, F.data(), b.first.data(), b.second.data(), 30u, 0.5);
+        pagmo_hip::local_descent(p10, 30u, 0.5);
+        bool same = true;
+        for (std::size_t i = 0; i < 64u; ++i) {
+            same = same && p10.get_f()[i][0] == F[i];
+            for (std::size_t j = 0; j < 10u; ++j) same = same && p10.get_x()[i][j] == X[10u * i + j];
+        }
+        CHECK(same);
+    }
+
+    // the memetic loop: pso_gen generations (hip_bfe) interleaved with local
+    // steps on lennard_jones; the local step never raises an individual's
+    // fitness and leaves x inside the bounds, and pso_gen runs unchanged
+    {
+        problem prob{lennard_jones{8u}};
+        population pop{prob, 32u, 7u};
+        pso_gen algo{3u, 0.7298, 2.05, 2.05, 0.5, 5u, 2u, 4u, false, 7u};
+        algo.set_bfe(bfe{hip_bfe{}});
+        const auto b = prob.get_bounds();
+        bool monotone = true, inside = true, consistent = true;
+        for (int round = 0; round < 4; ++round) {
+            pop = algo.evolve(pop);
+            const auto before = pop.get_f();
+            pagmo_hip::local_descent(pop, 10u, 0.05);
+            const auto fx = hip_bfe{}(pop.get_problem(), [&] {
+                vector_double all;
+                for (const auto &x : pop.get_x()) all.insert(all.end(), x.begin(), x.end());
+                return all;
+            }());
+            for (std::size_t i = 0; i < pop.size(); ++i) {
+                monotone = monotone && pop.get_f()[i][0] <= before[i][0];
+                consistent = consistent && pop.get_f()[i][0] == fx[i];
+                for (std::size_t j = 0; j < b.first.size(); ++j) {
+                    inside = inside && pop.get_x()[i][j] >= b.first[j] && pop.get_x()[i][j] <= b.second[j];
+                }
+            }
+        }
+        CHECK(monotone);
+        CHECK(inside);
+        CHECK(consistent);
+    }
+}
+
+// local_descent's argument checks happen before any device work
+static void local_descent_cpu_tests()
+{
+    population pa{problem{ackley{3u}}, 2u, 1u};
+    CHECK(throws<std::runtime_error>([&] { pagmo_hip::local_descent(pa, 1u, 0.1); }).find("gradient has not been")
+          != std::string::npos);
+    population pr{problem{rosenbrock{3u}}, 2u, 1u};
+    CHECK(throws<std::invalid_argument>([&] { pagmo_hip::local_descent(pr, 1u, -1.); }).find("initial step")
+          != std::string::npos);
 }
 
 int main(int argc, char **argv)
 {
     const bool gpu = argc > 1 && std::strcmp(argv[1], "gpu") == 0;
     cpu_tests();
+    local_descent_cpu_tests();
     if (gpu) gpu_tests();
     std::printf("%s: %d failure(s)\n", gpu ? "gpu" : "cpu", g_fail);
     return g_fail ? 1 : 0;

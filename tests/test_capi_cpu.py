@@ -112,3 +112,23 @@ def test_crowding_and_selection_host_checks():
     assert list(P.select_best_N_mo_host(np.random.default_rng(0).random((4, 2)), 4)) == [0, 1, 2, 3]
     assert list(P.sort_population_mo_host(np.zeros((0, 2)))) == []
     assert list(P.sort_population_mo_host(np.zeros((1, 4)))) == [0]
+
+
+def test_local_descent_host_checks():
+    # argument checks answered before any device work
+    x = np.zeros((2, 3))
+    with pytest.raises(P.PghError, match="The gradient has not been implemented"):
+        P.local_descent_host(P.ACKLEY, 3, x, 1, 0.1)
+    for bad in (0.0, -1.0, float("inf"), float("nan")):
+        with pytest.raises(P.PghError, match="initial step of a local descent"):
+            P.local_descent_host(P.ROSENBROCK, 3, x, 1, bad)
+    u = P.make_udp(P.ROSENBROCK, 3)
+    lb = np.zeros(3)
+    f = np.zeros(2)
+    rc = P.lib().pgh_local_descent_host(C.byref(u), x.ctypes.data, 6, f.ctypes.data, 2, lb.ctypes.data, None, 1, 0.1)
+    assert rc == P.INVALID_ARGUMENT and "both given or both absent" in P.lib().pgh_last_error().decode()
+    with pytest.raises(P.PghError, match="not an exact multiple"):
+        P.local_descent_host(P.ROSENBROCK, 3, np.zeros((1, 4)).reshape(2, 2), 1, 0.1)
+    assert P.lib().pgh_local_descent_work_len(1000, 30) == 2 * 1000 * 30 + 2 * 1000
+    x0, f0 = P.local_descent_host(P.ROSENBROCK, 3, np.zeros((0, 3)), 5, 0.1)
+    assert x0.shape == (0, 3) and f0.shape == (0,)

@@ -438,3 +438,46 @@ def test_sort_population_mo_properties():
             assert key == sorted(key)
             key_i = [(rank[i], -crowd[i], i) for i in order]
             assert key_i == sorted(key_i)
+
+
+# ---- the memetic local step (no reference counterpart) ----------------------
+
+def test_local_descent_hand_traces():
+    # rosenbrock at the origin: g = (-2, 0), m = 2.  alpha 0.5: y = (0.5, 0),
+    # f = 100 * 0.25^2 + 0.25 = 6.5 > 1, rejected; alpha 0.25: y = (0.25, 0),
+    # f = 100 * 0.0625^2 + 0.75^2 = 0.953125 < 1, accepted.  All exact.
+    x, f = O.local_descent(O.ROSENBROCK, 2, np.zeros((1, 2)), 2, 0.5)
+    assert x.tolist() == [[0.25, 0.0]] and f.tolist() == [0.953125]
+    # one more round from (0.25, 0) with alpha 0.5: g = (4.75, -12.5), m = 12.5,
+    # y = (0.06, 0.5), f(y) = 25.52... > 0.953125: rejected, x stays
+    x3, f3 = O.local_descent(O.ROSENBROCK, 2, np.zeros((1, 2)), 3, 0.5)
+    s = 0.5 / 12.5
+    y = np.array([0.25 - s * 4.75, 0.0 - s * -12.5])
+    fy = 100 * (y[0] ** 2 - y[1]) ** 2 + (y[0] - 1) ** 2
+    assert fy > 0.953125 and x3.tolist() == [[0.25, 0.0]] and f3.tolist() == [0.953125]
+    # a bound clamps the trial: ub_0 = 0.125 -> (0.125, 0), f = 0.7900390625
+    x, f = O.local_descent(O.ROSENBROCK, 2, np.zeros((1, 2)), 1, 0.5, [-5.0, -5.0], [0.125, 10.0])
+    assert x.tolist() == [[0.125, 0.0]] and f.tolist() == [0.7900390625]
+    # zero rounds: the fitness only; the minimum does not move (g = 0)
+    x, f = O.local_descent(O.ROSENBROCK, 4, np.ones((1, 4)), 10, 0.5)
+    assert x.tolist() == [[1.0] * 4] and f.tolist() == [0.0]
+
+
+def test_local_descent_rastrigin_reaches_the_local_minimum():
+    # from 0.1 the steps of a one-dimensional rastrigin run into its minimum 0
+    x, f = O.local_descent(O.RASTRIGIN, 1, np.array([[0.1]]), 40, 0.05)
+    assert abs(x[0, 0]) < 1e-6 and f[0] < 1e-10
+
+
+@pytest.mark.parametrize("kind,dim", [(O.ROSENBROCK, 7), (O.RASTRIGIN, 9), (O.LENNARD_JONES, 6)])
+def test_local_descent_invariants(kind, dim):
+    nx, _, lb, ub = O.problem_info(kind, dim)
+    lo, hi = (lb, ub) if kind != O.LENNARD_JONES else (np.full(nx, 0.3), np.full(nx, 2.5))
+    x0 = O.batch_uniform(lo, hi, 40, 2)
+    f0 = O.bfe(kind, dim, x0)[:, 0]
+    x, f = O.local_descent(kind, dim, x0, 15, 0.05, lb, ub)
+    assert np.all(f <= f0) and np.mean(f < f0) > 0.5
+    assert np.array_equal(f, O.bfe(kind, dim, x)[:, 0])
+    assert np.all((x >= lb) & (x <= ub))
+    assert np.array_equal(np.any(x != x0, axis=1), f < f0)
+
