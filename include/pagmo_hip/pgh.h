@@ -154,6 +154,34 @@ pgh_status pgh_fnds_sizes(pgh_fnds_ws *ws, uint64_t *n_fronts, uint64_t *n_dom_i
 pgh_status pgh_fnds_copy(pgh_fnds_ws *ws, uint64_t *dom_count, uint64_t *rank, uint64_t *front_offsets,
                          uint64_t *front_items, uint64_t *dom_offsets, uint64_t *dom_items, int to_host);
 
+/* ---- autotuning of launch mappings.  No counterpart in the reference (its
+ *      evaluators have no launch shape).  Only parameters that cannot change
+ *      any result are tuned: the workgroup count of the front peel (for
+ *      n <= 32768) and the number of Lennard-Jones individuals per workgroup;
+ *      lanes per individual, reduction trees and operation order never depend
+ *      on them.  A tuner times each candidate `reps` times with HIP events on
+ *      `stream` over the caller's own input, keeps the fastest, and finishes
+ *      with one more run so that the outputs hold that run's results.
+ *      Synchronises `stream`.
+ *
+ *      Peel workgroups of a workspace: 0 = the default rule (one per 256
+ *      points, 8 to the CU count); at most 4096.  pgh_fnds_tune tries one per
+ *      64, 128, 256, 512 and 1024 points, within 8 and the CU count, and sets
+ *      the fastest on the workspace (*chosen = 0 when n > 32768: nothing to
+ *      tune). */
+pgh_status pgh_fnds_set_peel_blocks(pgh_fnds_ws *ws, uint32_t blocks);
+pgh_status pgh_fnds_tune(pgh_fnds_ws *ws, const double *objs, uint64_t n, uint64_t m, uint32_t reps, void *stream,
+                         uint32_t *chosen);
+/* Lennard-Jones individuals per workgroup for the energy (grad = 0) or the
+ * energy-and-gradient kernel (grad != 0), process-wide, also used by
+ * archipelagos created afterwards: 0 = the default (the most whose LDS fits
+ * 64 KiB); at most that default.  pgh_lj_tune tries the default and its
+ * halvings (up to four values) on dvs; grad == NULL tunes the energy kernel
+ * (fvs required), otherwise the gradient kernel (fvs may be NULL). */
+pgh_status pgh_lj_set_groups(uint64_t atoms, int grad, uint32_t groups);
+pgh_status pgh_lj_tune(const pgh_udp *udp, const double *dvs, uint64_t n_dvs, double *fvs, double *grad,
+                       uint32_t reps, void *stream, uint32_t *chosen);
+
 /* ---- crowding distance and the multi-objective orderings built on fnds
  *      (src/utils/multi_objective.cpp:280-465).  The reference sorts with
  *      std::sort, which leaves the order of equal keys open; here equal keys

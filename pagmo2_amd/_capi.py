@@ -24,7 +24,7 @@ EXPORTS = [
     "pgh_last_error", "pgh_abi_version", "pgh_device_available", "pgh_problem_dims", "pgh_supported",
     "pgh_bfe_eval", "pgh_bfe_eval_host", "pgh_batch_gradient", "pgh_batch_gradient_host",
     "pgh_fnds_create", "pgh_fnds_destroy", "pgh_fnds_run", "pgh_fnds_run_host", "pgh_fnds_sizes",
-    "pgh_fnds_copy",
+    "pgh_fnds_copy", "pgh_fnds_set_peel_blocks", "pgh_fnds_tune", "pgh_lj_set_groups", "pgh_lj_tune",
     "pgh_crowding_distance", "pgh_crowding_distance_host", "pgh_sort_population_mo", "pgh_sort_population_mo_host",
     "pgh_select_best_N_mo", "pgh_select_best_N_mo_host",
     "pgh_local_descent_work_len", "pgh_local_descent", "pgh_local_descent_host",
@@ -95,6 +95,10 @@ def _declare(L):
         "pgh_fnds_run_host": [vp, vp, u64, u64],
         "pgh_fnds_sizes": [vp, C.POINTER(u64), C.POINTER(u64)],
         "pgh_fnds_copy": [vp, vp, vp, vp, vp, vp, vp, ci],
+        "pgh_fnds_set_peel_blocks": [vp, u32],
+        "pgh_fnds_tune": [vp, vp, u64, u64, u32, vp, C.POINTER(u32)],
+        "pgh_lj_set_groups": [u64, ci, u32],
+        "pgh_lj_tune": [udp_p, vp, u64, vp, vp, u32, vp, C.POINTER(u32)],
         "pgh_crowding_distance": [vp, vp, u64, u64, vp, vp],
         "pgh_crowding_distance_host": [vp, u64, u64, vp],
         "pgh_sort_population_mo": [vp, vp, u64, u64, vp, vp],
@@ -179,6 +183,20 @@ def batch_gradient(kind, dim, dvs, grad, fvs=None, stream=None):
     check(lib().pgh_batch_gradient(C.byref(u), dvs.data_ptr(), n, nx, grad.data_ptr(), _dptr(fvs), _stream(stream)))
 
 
+def lj_set_groups(atoms, grad, groups):
+    check(lib().pgh_lj_set_groups(atoms, int(bool(grad)), groups))
+
+
+def lj_tune(atoms, dvs, fvs=None, grad=None, reps=10, stream=None):
+    """Times Lennard-Jones individuals-per-workgroup values on dvs; keeps and
+    returns the fastest (energy kernel when grad is None)."""
+    u = make_udp(LENNARD_JONES, atoms)
+    out = C.c_uint32()
+    check(lib().pgh_lj_tune(C.byref(u), dvs.data_ptr(), dvs.shape[0], _dptr(fvs), _dptr(grad), reps, _stream(stream),
+                            C.byref(out)))
+    return out.value
+
+
 def local_descent_work_len(n, nx):
     return lib().pgh_local_descent_work_len(n, nx)
 
@@ -226,6 +244,17 @@ class Fnds:
     def run(self, objs, stream=None):
         n, m = objs.shape
         check(lib().pgh_fnds_run(self.h, objs.data_ptr(), n, m, _stream(stream)))
+
+    def set_peel_blocks(self, blocks):
+        check(lib().pgh_fnds_set_peel_blocks(self.h, blocks))
+
+    def tune(self, objs, reps=5, stream=None):
+        """Times the peel grids on objs, keeps the fastest; returns it (0 when
+        there is nothing to tune).  The workspace then holds objs' result."""
+        n, m = objs.shape
+        out = C.c_uint32()
+        check(lib().pgh_fnds_tune(self.h, objs.data_ptr(), n, m, reps, _stream(stream), C.byref(out)))
+        return out.value
 
     def run_host(self, objs: np.ndarray):
         objs = np.ascontiguousarray(objs, np.float64)

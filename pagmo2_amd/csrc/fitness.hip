@@ -3,12 +3,38 @@
 // Data layout: dvs is the row-major n_dvs x nx matrix that pagmo packs before
 // every bfe call (src/algorithms/pso_gen.cpp:420-425, src/problem.cpp:383-392);
 // fvs is n_dvs x nf, grad is n_dvs x nx (dense, nf == 1).
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
 #include "fitness_dev.hpp"
 #include "pgh_internal.hpp"
 
 namespace pgh
 {
 using namespace dev;
+
+// Individuals per Lennard-Jones workgroup set by pgh_lj_set_groups or found by
+// pgh_lj_tune, keyed by 2 * atoms + (gradient kernel); lj_group_count
+// otherwise.  Any count from 1 to lj_group_count gives the same results: each
+// individual is evaluated by its own L-lane group in the same order.
+namespace
+{
+std::mutex g_lj_mu;
+std::map<uint32_t, uint32_t> g_lj_groups;
+} // namespace
+
+uint32_t lj_groups_for(uint32_t atoms, bool grad)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_lj_mu);
+        const auto it = g_lj_groups.find(2u * atoms + (grad ? 1u : 0u));
+        if (it != g_lj_groups.end()) return it->second;
+    }
+    return lj_group_count(atoms, grad);
+}
 
 namespace
 {
@@ -160,7 +186,7 @@ template <int L, uint32_t APL>
 pgh_status launch_lj_la(const double *dvs, uint64_t n, uint32_t atoms, double *fvs, double *grad, hipStream_t s)
 {
     const bool with_grad = grad != nullptr;
-    const uint32_t G = lj_group_count(atoms, with_grad);
+    const uint32_t G = lj_groups_for(atoms, with_grad);
     const uint64_t blocks = (n + G - 1u) / G;
     const size_t lds = static_cast<size_t>(G) * lj_stride(atoms, with_grad) * sizeof(double);
     if (grad) {
@@ -275,3 +301,76 @@ pgh_status launch_gradient(const prob &p, const double *dvs, uint64_t n, double 
 }
 
 } // namespace pgh
+
+using namespace pgh;
+
+extern "C" {
+
+pgh_status pgh_lj_set_groups(uint64_t atoms, int grad, uint32_t groups)
+{
+    if (atoms < 3u || atoms > 4096u) return fail(PGH_INVALID_ARGUMENT, "atoms out of range: " + std::to_string(atoms));
+    const uint32_t a = static_cast<uint32_t>(atoms);
+    const uint32_t most = lj_group_count(a, grad != 0);
+    if (groups > most) {
+        return fail(PGH_INVALID_ARGUMENT, "at most " + std::to_string(most) + " individuals per workgroup for "
+                                              + std::to_string(atoms) + " atoms, got " + std::to_string(groups));
+    }
+    std::lock_guard<std::mutex> lk(g_lj_mu);
+    if (groups == 0u) g_lj_groups.erase(2u * a + (grad ? 1u : 0u));
+    else g_lj_groups[2u * a + (grad ? 1u : 0u)] = groups;
+    return PGH_OK;
+}
+
+pgh_status pgh_lj_tune(const pgh_udp *udp, const double *dvs, uint64_t n_dvs, double *fvs, double *grad,
+                       uint32_t reps, void *stream, uint32_t *chosen)
+{
+    prob p;
+    PGH_TRY(resolve(udp, p));
+    if (p.kind != PGH_LENNARD_JONES) return fail(PGH_INVALID_ARGUMENT, "pgh_lj_tune needs a lennard_jones problem");
+    if (reps == 0u) return fail(PGH_INVALID_ARGUMENT, "the tuner needs at least one timed repetition");
+    if (n_dvs == 0u || !dvs || (!fvs && !grad)) return fail(PGH_INVALID_ARGUMENT, "the tuner needs a batch to time");
+    if (chosen) *chosen = 0u;
+    PGH_TRY(check_launch_size(n_dvs, p.nx));
+    const bool with_grad = grad != nullptr;
+    const uint32_t most = lj_group_count(p.dim, with_grad);
+    if (most == 0u) return fail(PGH_UNSUPPORTED, "lennard_jones cluster too large for one workgroup");
+    std::vector<uint32_t> cands;
+    for (uint32_t g = most; g >= 1u && cands.size() < 4u; g >>= 1) cands.push_back(g);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    PGH_HIP(hipEventCreate(&e0));
+    const hipError_t ce = hipEventCreate(&e1);
+    if (ce != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return hip_fail(ce, "hipEventCreate");
+    }
+    float best = 0.f;
+    uint32_t best_g = 0u;
+    pgh_status st = PGH_OK;
+    for (uint32_t g : cands) {
+        if ((st = pgh_lj_set_groups(p.dim, with_grad, g)) != PGH_OK) break;
+        if ((st = launch_lj(dvs, n_dvs, p.dim, fvs, grad, s)) != PGH_OK) break; // warm-up
+        if ((st = hip_status(hipEventRecord(e0, s), "hipEventRecord")) != PGH_OK) break;
+        for (uint32_t r = 0; r < reps && st == PGH_OK; ++r) st = launch_lj(dvs, n_dvs, p.dim, fvs, grad, s);
+        if (st != PGH_OK) break;
+        if ((st = hip_status(hipEventRecord(e1, s), "hipEventRecord")) != PGH_OK) break;
+        if ((st = hip_status(hipEventSynchronize(e1), "hipEventSynchronize")) != PGH_OK) break;
+        float ms = 0.f;
+        if ((st = hip_status(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime")) != PGH_OK) break;
+        if (best_g == 0u || ms < best) {
+            best = ms;
+            best_g = g;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != PGH_OK) {
+        (void)pgh_lj_set_groups(p.dim, with_grad, 0u);
+        return st;
+    }
+    PGH_TRY(pgh_lj_set_groups(p.dim, with_grad, best_g));
+    if (chosen) *chosen = best_g;
+    return launch_lj(dvs, n_dvs, p.dim, fvs, grad, s);
+}
+
+} // extern "C"

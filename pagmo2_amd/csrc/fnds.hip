@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "pgh_internal.hpp"
 
@@ -330,6 +331,10 @@ __global__ __launch_bounds__(kPeelBlock) void k_peel(const uint64_t *__restrict_
 // the last front is known.  lvl[2L] = claim[L], lvl[2L+1] = done[L].
 constexpr uint32_t kGridBlock = 1024;
 constexpr uint32_t kGridMaxN = 32768;
+// Upper limit of a set peel grid.  The peel is correct for any count:
+// workgroups that are not resident start after the resident ones finish and
+// find the work done.
+constexpr uint32_t kMaxPeelBlocks = 4096;
 constexpr uint32_t kChunk = 1024;         // items per claim: one per thread
 constexpr uint32_t kSpinLimit = 1u << 26; // ~2 s of s_sleep(1) polls
 
@@ -545,6 +550,22 @@ uint32_t grid_for(uint64_t count)
 
 
 // Launches k_front0 and the grid-wide peel.
+// One workgroup per 256 elements, at least 8 and at most one per CU (a sweep
+// of 64..1024 elements per workgroup and 512..8192 items per claim on the
+// 8192 x 2 and 16384 x 3 sorts put the optimum here).
+uint32_t default_peel_blocks(uint32_t n32, uint32_t cus)
+{
+    return std::min(cus, std::max(8u, n32 / 256u));
+}
+
+uint32_t device_cus()
+{
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256u;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 256u;
+    return static_cast<uint32_t>(std::max(v, 1));
+}
+
 pgh_status launch_grid_peel(pgh_fnds_ws *ws, hipStream_t s, uint32_t n32, uint32_t w32)
 {
     static thread_local int dev_cached = -1;
@@ -563,10 +584,7 @@ pgh_status launch_grid_peel(pgh_fnds_ws *ws, hipStream_t s, uint32_t n32, uint32
     uint32_t P = 1;
     while (P < n32) P <<= 1;
     const size_t lds = static_cast<size_t>(std::max(P, kGridBlock)) * 4u;
-    // one workgroup per 256 elements, at least 8 and at most one per CU (a
-    // sweep of 64..1024 elements per workgroup and 512..8192 items per claim
-    // on the 8192 x 2 and 16384 x 3 sorts put the optimum here)
-    const uint32_t blocks = std::min(cus, std::max(8u, n32 / 256u));
+    const uint32_t blocks = ws->peel_blocks ? ws->peel_blocks : default_peel_blocks(n32, cus);
     auto *cnt = static_cast<uint32_t *>(ws->cnt32.ptr);
     auto *key = static_cast<uint32_t *>(ws->key32.ptr);
     auto *comp = static_cast<uint32_t *>(ws->comp.ptr);
@@ -707,6 +725,82 @@ pgh_status pgh_fnds_run(pgh_fnds_ws *ws, const double *objs, uint64_t n, uint64_
     }
     PGH_HIP(hipGetLastError());
     return PGH_OK;
+}
+
+pgh_status pgh_fnds_set_peel_blocks(pgh_fnds_ws *ws, uint32_t blocks)
+{
+    if (!ws) return fail(PGH_INVALID_ARGUMENT, "null workspace");
+    if (blocks > kMaxPeelBlocks) {
+        return fail(PGH_INVALID_ARGUMENT, "at most " + std::to_string(kMaxPeelBlocks) + " peel workgroups, got "
+                                              + std::to_string(blocks));
+    }
+    ws->peel_blocks = blocks;
+    return PGH_OK;
+}
+
+pgh_status pgh_fnds_tune(pgh_fnds_ws *ws, const double *objs, uint64_t n, uint64_t m, uint32_t reps, void *stream,
+                         uint32_t *chosen)
+{
+    if (!ws) return fail(PGH_INVALID_ARGUMENT, "null workspace");
+    if (reps == 0u) return fail(PGH_INVALID_ARGUMENT, "the tuner needs at least one timed repetition");
+    if (chosen) *chosen = 0u;
+    if (n > kGridMaxN) { // the single-workgroup peel: nothing to tune
+        ws->peel_blocks = 0u;
+        return pgh_fnds_run(ws, objs, n, m, stream);
+    }
+    const uint32_t n32 = static_cast<uint32_t>(n);
+    const uint32_t cus = device_cus();
+    std::vector<uint32_t> cands;
+    for (uint32_t per : {64u, 128u, 256u, 512u, 1024u}) {
+        const uint32_t b = std::min(cus, std::max(8u, n32 / per));
+        if (std::find(cands.begin(), cands.end(), b) == cands.end()) cands.push_back(b);
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    PGH_HIP(hipEventCreate(&e0));
+    const hipError_t ce = hipEventCreate(&e1);
+    if (ce != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return hip_fail(ce, "hipEventCreate");
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    float best = 0.f;
+    uint32_t best_b = 0u;
+    pgh_status st = PGH_OK;
+    for (uint32_t b : cands) {
+        ws->peel_blocks = b;
+        st = pgh_fnds_run(ws, objs, n, m, stream); // warm-up (and argument checks)
+        if (st != PGH_OK) break;
+        if ((st = hip_status(hipEventRecord(e0, s), "hipEventRecord")) != PGH_OK) break;
+        for (uint32_t r = 0; r < reps && st == PGH_OK; ++r) st = pgh_fnds_run(ws, objs, n, m, stream);
+        if (st != PGH_OK) break;
+        if ((st = hip_status(hipEventRecord(e1, s), "hipEventRecord")) != PGH_OK) break;
+        if ((st = hip_status(hipEventSynchronize(e1), "hipEventSynchronize")) != PGH_OK) break;
+        // a peel that ran out of its spin limit is an error, not a timing
+        uint64_t flag[3] = {0, 0, 0};
+        if ((st = hip_status(hipMemcpy(flag, ws->scalars.ptr, 24u, hipMemcpyDeviceToHost), "hipMemcpy")) != PGH_OK) {
+            break;
+        }
+        if (flag[2]) {
+            st = fail(PGH_DEVICE_ERROR, "fast_non_dominated_sorting: a grid barrier timed out");
+            break;
+        }
+        float ms = 0.f;
+        if ((st = hip_status(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime")) != PGH_OK) break;
+        if (best_b == 0u || ms < best) {
+            best = ms;
+            best_b = b;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != PGH_OK) {
+        ws->peel_blocks = 0u;
+        return st;
+    }
+    ws->peel_blocks = best_b;
+    if (chosen) *chosen = best_b;
+    // leave the workspace holding a result computed with the chosen grid
+    return pgh_fnds_run(ws, objs, n, m, stream);
 }
 
 pgh_status pgh_fnds_sizes(pgh_fnds_ws *ws, uint64_t *n_fronts, uint64_t *n_dom_items)

@@ -32,6 +32,9 @@ struct pgh_fnds_ws {
     pgh::dev_buf comp, ctl, lvl;
     uint64_t h_sizes[3] = {0, 0, 0}; // n_fronts, n_dom_items, peel barrier timeout
     bool sized = false;
+    // workgroups of the grid-wide front peel: 0 = the default rule, else the
+    // value set or found by pgh_fnds_tune (any count gives the same result)
+    uint32_t peel_blocks = 0;
     // mo_sort.hip: sort keys and values (two of each), crowding, radix-sort
     // scratch, and a staged output
     pgh::dev_buf sk0, sk1, sv0, sv1, crowd, sort_tmp, out_staged;

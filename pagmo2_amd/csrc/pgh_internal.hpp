@@ -16,6 +16,12 @@ void set_error(const std::string &msg);
 pgh_status fail(pgh_status st, const std::string &msg);
 pgh_status hip_fail(hipError_t e, const char *what);
 
+// PGH_OK, or the status and message of a failed HIP call.
+inline pgh_status hip_status(hipError_t e, const char *what)
+{
+    return e == hipSuccess ? PGH_OK : hip_fail(e, what);
+}
+
 #define PGH_HIP(call)                                                                                                \
     do {                                                                                                             \
         hipError_t pgh_e_ = (call);                                                                                  \
@@ -45,6 +51,10 @@ pgh_status resolve(const pgh_udp *udp, prob &out);
 pgh_status launch_fitness(const prob &p, const double *dvs, uint64_t n_dvs, double *fvs, hipStream_t s);
 pgh_status launch_gradient(const prob &p, const double *dvs, uint64_t n_dvs, double *grad, double *fvs,
                            hipStream_t s);
+
+// Individuals per Lennard-Jones workgroup in use for (atoms, gradient kernel):
+// a value set or tuned through the C ABI, else lj_group_count (fitness.hip).
+uint32_t lj_groups_for(uint32_t atoms, bool grad);
 
 // Refuses grids beyond what one launch addresses (fitness.hip).
 pgh_status check_launch_size(uint64_t n, uint32_t nx);

@@ -494,7 +494,7 @@ pgh_status pgh_archi_create(pgh_archi **out, const pgh_island_spec *islands, uin
         uint32_t per = 256u;
         if (p.kind == PGH_LENNARD_JONES) {
             di.lanes = static_cast<uint32_t>(lj_lanes(p.dim));
-            per = lj_group_count(p.dim, false);
+            per = lj_groups_for(p.dim, false);
             if (per == 0u) return fail(PGH_UNSUPPORTED, "lennard_jones cluster too large for one workgroup");
             a->lds_bytes =
                 std::max<size_t>(a->lds_bytes, static_cast<size_t>(per) * lj_stride(p.dim, false) * sizeof(double));
