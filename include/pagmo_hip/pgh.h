@@ -252,6 +252,13 @@ enum pgh_archi_mode {
 
 pgh_status pgh_archi_create(pgh_archi **a, const pgh_island_spec *islands, uint64_t n_islands,
                             const double *const *X0, const double *const *F0);
+/* The same with every island's initial population made on the device, as
+ * population(problem, bfe, pop, pop_seeds[k]) does (population.cpp:82-103):
+ * batch_random_decision_vector from std::mt19937(pop_seeds[k])
+ * (generic.hpp:326-388), bit for bit, then one batch evaluation by the device
+ * kernels.  No host population arrays are involved.  Synchronous. */
+pgh_status pgh_archi_create_seeded(pgh_archi **a, const pgh_island_spec *islands, uint64_t n_islands,
+                                   const uint32_t *pop_seeds);
 pgh_status pgh_archi_destroy(pgh_archi *a);
 /* One pso_gen::evolve call of algo.gen generations on every island
  * (velocities re-drawn first unless memory is set, pso_gen.cpp:193-200). */

@@ -29,7 +29,8 @@ EXPORTS = [
     "pgh_select_best_N_mo", "pgh_select_best_N_mo_host",
     "pgh_local_descent_work_len", "pgh_local_descent", "pgh_local_descent_host",
     "pgh_mt19937_generate", "pgh_batch_random_dv",
-    "pgh_archi_create", "pgh_archi_destroy", "pgh_archi_evolve", "pgh_archi_get", "pgh_archi_set_trace",
+    "pgh_archi_create", "pgh_archi_create_seeded", "pgh_archi_destroy", "pgh_archi_evolve", "pgh_archi_get",
+    "pgh_archi_set_trace",
     "pgh_archi_get_trace", "pgh_pso_gen_evolve_host",
 ]
 
@@ -108,6 +109,7 @@ def _declare(L):
         "pgh_mt19937_generate": [u32, u64, u64, vp, vp],
         "pgh_batch_random_dv": [udp_p, u64, u32, vp, vp],
         "pgh_archi_create": [C.POINTER(vp), C.POINTER(pgh_island_spec), u64, C.POINTER(vp), C.POINTER(vp)],
+        "pgh_archi_create_seeded": [C.POINTER(vp), C.POINTER(pgh_island_spec), u64, C.POINTER(u32)],
         "pgh_archi_destroy": [vp],
         "pgh_archi_evolve": [vp, ci, vp],
         "pgh_archi_get": [vp, u64, vp, vp],
@@ -367,6 +369,30 @@ class Archipelago:
         self.gens = [isl["algo"].gen for isl in islands]
         if trace:
             check(lib().pgh_archi_set_trace(self.h, 1))
+
+    @classmethod
+    def seeded(cls, islands, trace=False):
+        """islands: list of dicts {kind, dim, pop, seed, algo}; each initial
+        population is drawn (std::mt19937(seed)) and evaluated on the device."""
+        self = cls.__new__(cls)
+        n = len(islands)
+        specs = (pgh_island_spec * n)()
+        seeds = (C.c_uint32 * n)()
+        self._keep = []
+        self.shapes = []
+        for k, isl in enumerate(islands):
+            specs[k].udp = make_udp(isl["kind"], isl["dim"])
+            specs[k].pop = isl["pop"]
+            specs[k].algo = isl["algo"]
+            seeds[k] = isl["seed"]
+            self.shapes.append((isl["pop"], dims(isl["kind"], isl["dim"])[0]))
+        h = C.c_void_p()
+        check(lib().pgh_archi_create_seeded(C.byref(h), specs, n, seeds))
+        self.h = h
+        self.gens = [isl["algo"].gen for isl in islands]
+        if trace:
+            check(lib().pgh_archi_set_trace(self.h, 1))
+        return self
 
     def close(self):
         if getattr(self, "h", None):

@@ -416,16 +416,15 @@ pgh_status launch_range(pgh_archi *a, uint32_t i0, uint32_t i1, bool draw_v, boo
     return PGH_OK;
 }
 
-} // namespace
-
-extern "C" {
-
-pgh_status pgh_archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_t n_islands,
-                            const double *const *X0, const double *const *F0)
+// Initial populations from host arrays (X0, F0) or, with `seeds`, drawn and
+// evaluated on the device.
+pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_t n_islands,
+                        const double *const *X0, const double *const *F0, const uint32_t *seeds)
 {
-    if (!out || !islands || !X0 || !F0) return fail(PGH_INVALID_ARGUMENT, "null argument");
+    if (!out || !islands || (!seeds && (!X0 || !F0))) return fail(PGH_INVALID_ARGUMENT, "null argument");
     if (n_islands == 0u) return fail(PGH_INVALID_ARGUMENT, "an archipelago needs at least one island");
     auto a = std::make_unique<pgh_archi>();
+    std::vector<prob> probs;
     uint64_t xoff = 0, foff = 0, woff = 0, toff = 0, eoff = 0;
     a->tile_begin.push_back(0);
     for (uint64_t k = 0; k < n_islands; ++k) {
@@ -433,6 +432,7 @@ pgh_status pgh_archi_create(pgh_archi **out, const pgh_island_spec *islands, uin
         prob p;
         PGH_TRY(resolve(&sp.udp, p));
         if (p.kind == PGH_MO_SPHERE) return fail(PGH_UNSUPPORTED, "pso_gen needs a single-objective problem");
+        probs.push_back(p);
         const pgh_pso_params &q = sp.algo;
         // pso_gen.cpp:83-112 parameter checks
         if (q.omega < 0. || q.omega > 1.) {
@@ -530,14 +530,39 @@ pgh_status pgh_archi_create(pgh_archi **out, const pgh_island_spec *islands, uin
     for (uint64_t k = 0; k < n_islands; ++k) {
         mt_seed_host(islands[k].algo.seed, states[k]);
         const island_desc &di = a->h_desc[k];
-        PGH_HIP(hipMemcpy(static_cast<double *>(a->X.ptr) + di.xoff, X0[k],
-                          static_cast<uint64_t>(di.pop) * di.nx * 8u, hipMemcpyHostToDevice));
-        PGH_HIP(hipMemcpy(static_cast<double *>(a->fit.ptr) + di.foff, F0[k], static_cast<uint64_t>(di.pop) * 8u,
-                          hipMemcpyHostToDevice));
+        double *Xk = static_cast<double *>(a->X.ptr) + di.xoff;
+        double *Fk = static_cast<double *>(a->fit.ptr) + di.foff;
+        if (seeds) {
+            // population(problem, bfe, pop, seed), population.cpp:82-103:
+            // batch_random_decision_vector from std::mt19937(seed), then one
+            // batch evaluation, both on the device
+            PGH_TRY(launch_batch_random_dv(probs[k], di.pop, seeds[k], Xk, a->own));
+            PGH_TRY(launch_fitness(probs[k], Xk, di.pop, Fk, a->own));
+        } else {
+            PGH_HIP(hipMemcpy(Xk, X0[k], static_cast<uint64_t>(di.pop) * di.nx * 8u, hipMemcpyHostToDevice));
+            PGH_HIP(hipMemcpy(Fk, F0[k], static_cast<uint64_t>(di.pop) * 8u, hipMemcpyHostToDevice));
+        }
     }
     PGH_HIP(hipMemcpy(a->states.ptr, states.data(), n_islands * sizeof(mt_state), hipMemcpyHostToDevice));
+    if (seeds) PGH_HIP(hipStreamSynchronize(a->own));
     *out = a.release();
     return PGH_OK;
+}
+} // namespace
+
+extern "C" {
+
+pgh_status pgh_archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_t n_islands,
+                            const double *const *X0, const double *const *F0)
+{
+    return archi_create(out, islands, n_islands, X0, F0, nullptr);
+}
+
+pgh_status pgh_archi_create_seeded(pgh_archi **out, const pgh_island_spec *islands, uint64_t n_islands,
+                                   const uint32_t *pop_seeds)
+{
+    if (!pop_seeds) return fail(PGH_INVALID_ARGUMENT, "null argument");
+    return archi_create(out, islands, n_islands, nullptr, nullptr, pop_seeds);
 }
 
 pgh_status pgh_archi_destroy(pgh_archi *a)

@@ -108,3 +108,40 @@ def test_unsupported_variant():
     with pytest.raises(P.PghError) as e:
         P.Archipelago([dict(kind=P.ROSENBROCK, dim=4, X0=X0, F0=F0, algo=P.pso_params(variant=1))])
     assert e.value.status == P.UNSUPPORTED
+
+
+def test_device_seeded_archipelago():
+    # population(problem, bfe, pop, seed) made on the device: X bit-exact with
+    # batch_random_decision_vector, F the device fitness of X; evolving it is
+    # the same as evolving an archipelago created from those host arrays
+    isl = uneven_islands(9, 5, gens=5)
+    specs = [dict(kind=i["kind"], dim=i["dim"], pop=i["X0"].shape[0], seed=1000 + k, algo=i["algo"])
+             for k, i in enumerate(isl)]
+    a = P.Archipelago.seeded(specs)
+    host = []
+    for k, sp in enumerate(specs):
+        X0 = O.batch_random_dv(sp["kind"], sp["dim"], sp["pop"], sp["seed"])
+        f = torch.empty((sp["pop"], 1), dtype=torch.float64, device="cuda:0")
+        P.bfe_eval(sp["kind"], sp["dim"], torch.from_numpy(X0).to("cuda:0"), f)
+        torch.cuda.synchronize()
+        F0 = f.cpu().numpy()[:, 0]
+        X, F = a.get(k)
+        assert np.array_equal(X, X0), k
+        assert np.array_equal(F, F0), k
+        if sp["kind"] == P.ROSENBROCK:
+            assert np.array_equal(F0, O.bfe(sp["kind"], sp["dim"], X0)[:, 0])
+        host.append(dict(kind=sp["kind"], dim=sp["dim"], X0=X0, F0=F0, algo=sp["algo"]))
+    b = P.Archipelago(host)
+    a.evolve()
+    b.evolve(P.ARCHI_PER_ISLAND)
+    for k in range(len(specs)):
+        Xa, Fa = a.get(k)
+        Xb, Fb = b.get(k)
+        assert np.array_equal(Xa, Xb) and np.array_equal(Fa, Fb), k
+    a.close()
+    b.close()
+
+
+def test_device_seeded_archipelago_checks():
+    with pytest.raises(P.PghError, match="at least one island"):
+        P.Archipelago.seeded([])
