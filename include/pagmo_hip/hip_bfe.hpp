@@ -11,7 +11,7 @@
 // (src/bfe.cpp:102-107).
 //
 // Problems with a device kernel (rosenbrock, rastrigin, ackley, schwefel,
-// griewank, lennard_jones, found with problem::extract<T>(),
+// griewank, lennard_jones, zdt, dtlz, found with problem::extract<T>(),
 // include/pagmo/problem.hpp:1104-1147) are evaluated by libpagmo_hip.so.  Any
 // other problem goes through default_bfe's dispatch
 // (src/batch_evaluators/default_bfe.cpp:53-68): member_bfe if it has
@@ -27,6 +27,7 @@
 // batched gradient for a population between two evolve() calls.
 #pragma once
 
+#include <cmath>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -39,11 +40,13 @@
 #include <pagmo/population.hpp>
 #include <pagmo/problem.hpp>
 #include <pagmo/problems/ackley.hpp>
+#include <pagmo/problems/dtlz.hpp>
 #include <pagmo/problems/griewank.hpp>
 #include <pagmo/problems/lennard_jones.hpp>
 #include <pagmo/problems/rastrigin.hpp>
 #include <pagmo/problems/rosenbrock.hpp>
 #include <pagmo/problems/schwefel.hpp>
+#include <pagmo/problems/zdt.hpp>
 #include <pagmo/threading.hpp>
 #include <pagmo/types.hpp>
 #include <pagmo/utils/multi_objective.hpp>
@@ -77,6 +80,41 @@ inline void check(pgh_status st)
 
 } // namespace detail
 
+namespace detail
+{
+// The number at the end of a UDP name "<prefix><id>" (zdt.cpp:164-167,
+// dtlz.cpp:168-171), or 0.
+inline unsigned name_id(const std::string &name, const std::string &prefix)
+{
+    if (name.size() <= prefix.size() || name.compare(0, prefix.size(), prefix) != 0) return 0u;
+    unsigned id = 0u;
+    for (auto c = name.begin() + static_cast<std::ptrdiff_t>(prefix.size()); c != name.end(); ++c) {
+        if (*c < '0' || *c > '9' || id > 1000u) return 0u;
+        id = 10u * id + static_cast<unsigned>(*c - '0');
+    }
+    return id;
+}
+
+// dtlz keeps alpha private (dtlz.hpp:163); it enters DTLZ4 only through
+// x_j^alpha (dtlz.cpp:320-341).  At x = (0.5, ..., 0.5) the distance function
+// is 0 and the last objective is sin(0.5^alpha * pi / 2): alpha is read back
+// from it and confirmed by recomputing that objective (same libm as the UDP).
+// Values it cannot pin (0.5^alpha underflowing, above 1000) report false.
+inline bool dtlz4_alpha(const pagmo::dtlz &q, std::size_t nx, uint64_t &alpha)
+{
+    const double pi_half = 3.1415926535897932384626433832795028841971693993 / 2.;
+    const auto f = q.fitness(pagmo::vector_double(nx, 0.5));
+    const double last = f.back();
+    if (!(last > 0.) || !(last <= 1.)) return false;
+    const double a = std::round(-std::log2(std::asin(last) / pi_half));
+    if (!(a >= 0.) || a > 1000.) return false;
+    const auto cand = static_cast<unsigned>(a);
+    if (std::sin(std::pow(0.5, cand) * pi_half) != last) return false;
+    alpha = cand;
+    return true;
+}
+} // namespace detail
+
 // The device description of p, if a kernel exists for it.
 inline bool describe(const pagmo::problem &p, pgh_udp &u)
 {
@@ -100,6 +138,23 @@ inline bool describe(const pagmo::problem &p, pgh_udp &u)
         // m_atoms is private (lennard_jones.hpp:87); nx = 3*atoms - 6
         u.kind = PGH_LENNARD_JONES;
         u.dim = (p.get_nx() + 6u) / 3u;
+    } else if (const auto *z = p.extract<pagmo::zdt>()) {
+        // m_prob_id and m_param are private (zdt.hpp:205-206): the id is in
+        // the name, the param follows from nx (zdt.cpp:108-132)
+        u.kind = PGH_ZDT;
+        u.prob_id = detail::name_id(z->get_name(), "ZDT");
+        const auto nx = p.get_nx();
+        if (u.prob_id == 5u && (nx < 30u || (nx - 30u) % 5u != 0u)) return false;
+        u.dim = u.prob_id == 5u ? (nx - 30u) / 5u + 1u : nx;
+        u.nobj = 2;
+        return pgh_supported(&u) == 1;
+    } else if (const auto *d = p.extract<pagmo::dtlz>()) {
+        u.kind = PGH_DTLZ;
+        u.prob_id = detail::name_id(d->get_name(), "DTLZ");
+        u.dim = p.get_nx();
+        u.nobj = d->get_nobj();
+        if (u.prob_id == 4u && !detail::dtlz4_alpha(*d, p.get_nx(), u.alpha)) return false;
+        return pgh_supported(&u) == 1;
     } else {
         return false;
     }

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define PGH_ABI_VERSION 1
+#define PGH_ABI_VERSION 2
 
 /* Status codes.  The C++ layer (pagmo_hip/hip_bfe.hpp) maps them onto the
  * exception types of the reference: INVALID_ARGUMENT -> std::invalid_argument
@@ -38,9 +38,10 @@ typedef enum pgh_status {
     PGH_OUT_OF_MEMORY = 6
 } pgh_status;
 
-/* Problems with a device kernel.  The first six are pagmo's UDPs
- * (src/problems/{rosenbrock,rastrigin,ackley,schwefel,griewank,lennard_jones}.cpp);
- * MO_SPHERE is the user-defined problem f_k(x) = sum_i (x_i - c_{k,i})^2. */
+/* Problems with a device kernel.  pagmo's UDPs
+ * src/problems/{rosenbrock,rastrigin,ackley,schwefel,griewank,lennard_jones,
+ * zdt,dtlz}.cpp; MO_SPHERE is the user-defined problem
+ * f_k(x) = sum_i (x_i - c_{k,i})^2. */
 typedef enum pgh_udp_kind {
     PGH_ROSENBROCK = 1,
     PGH_RASTRIGIN = 2,
@@ -48,21 +49,29 @@ typedef enum pgh_udp_kind {
     PGH_SCHWEFEL = 4,
     PGH_GRIEWANK = 5,
     PGH_LENNARD_JONES = 6,
-    PGH_MO_SPHERE = 7
+    PGH_MO_SPHERE = 7,
+    PGH_ZDT = 8,
+    PGH_DTLZ = 9
 } pgh_udp_kind;
 
 /* What hip_bfe extracts from a pagmo::problem (problem.hpp:1104-1147).
  *  dim: rosenbrock m_dim (rosenbrock.hpp:91); rastrigin/ackley/schwefel/
  *       griewank get_nx(); lennard_jones atom count (nx = 3*atoms-6,
- *       lennard_jones.cpp:99-111); MO_SPHERE nx.
- *  nobj, centers: MO_SPHERE only (nobj x dim, row-major; a device pointer for
- *       device entry points, a host pointer for _host ones). */
+ *       lennard_jones.cpp:99-111); MO_SPHERE nx; ZDT the constructor's
+ *       `param` (zdt.hpp:145: nx = param, or 30 + 5 (param - 1) for ZDT5);
+ *       DTLZ the constructor's `dim` (= nx, dtlz.hpp:105).
+ *  prob_id: ZDT 1..6, DTLZ 1..7 (zdt.cpp:55-66, dtlz.cpp:53-74); 0 otherwise.
+ *  nobj: MO_SPHERE objectives; DTLZ `fdim`; ignored otherwise.
+ *  centers: MO_SPHERE only (nobj x dim, row-major; a device pointer for
+ *       device entry points, a host pointer for _host ones).
+ *  alpha: DTLZ's `alpha` (used by DTLZ4); ignored otherwise. */
 typedef struct pgh_udp {
     int32_t kind;
-    uint32_t reserved;
+    uint32_t prob_id;
     uint64_t dim;
     uint64_t nobj;
     const double *centers;
+    uint64_t alpha;
 } pgh_udp;
 
 /* ---- library ------------------------------------------------------------ */

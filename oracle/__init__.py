@@ -18,7 +18,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-ROSENBROCK, RASTRIGIN, ACKLEY, SCHWEFEL, GRIEWANK, LENNARD_JONES, MO_SPHERE = range(1, 8)
+ROSENBROCK, RASTRIGIN, ACKLEY, SCHWEFEL, GRIEWANK, LENNARD_JONES, MO_SPHERE, ZDT, DTLZ = range(1, 10)
 
 _lib = None
 
@@ -54,6 +54,8 @@ def _declare(L):
         "orc_bfe": [cint, u64, C.c_void_p, u64, _dp, u64, u64, _dp, u32],
         "orc_bfe_timed": [cint, u64, C.c_void_p, u64, _dp, u64, u64, _dp, u32, u32, C.POINTER(dbl)],
         "orc_gradient": [cint, u64, _dp, u64, u64, _dp],
+        "orc_mo_info": [cint, u32, u64, u64, u32, C.POINTER(u64), C.POINTER(u64), C.c_void_p, C.c_void_p],
+        "orc_mo_fitness": [cint, u32, u64, u64, u32, _dp, u64, u64, _dp],
         "orc_local_descent": [cint, u64, _dp, u64, _dp, C.c_void_p, C.c_void_p, u32, dbl],
         "orc_estimate_gradient": [cint, u64, C.c_void_p, u64, _dp, u64, dbl, cint, _dp],
         "orc_estimate_gradient_kat": [_dp, dbl, cint, _dp],
@@ -153,6 +155,29 @@ def gradient(kind: int, dim: int, dvs: np.ndarray) -> np.ndarray:
     n, nx = dvs.shape
     out = np.empty((n, nx))
     _chk(lib().orc_gradient(kind, dim, dvs, n, nx, out))
+    return out
+
+
+def mo_info(kind: int, prob_id: int, dim: int, nobj: int = 2, alpha: int = 100):
+    """(nx, nf, lb, ub) of a ZDT (kind ZDT, dim = the constructor's param) or
+    DTLZ (dim, nobj = fdim, alpha) problem."""
+    nx, nf = u64(), u64()
+    _chk(lib().orc_mo_info(kind, prob_id, dim, nobj, alpha, C.byref(nx), C.byref(nf), None, None))
+    lb, ub = np.empty(nx.value), np.empty(nx.value)
+    _chk(lib().orc_mo_info(kind, prob_id, dim, nobj, alpha, C.byref(nx), C.byref(nf), lb.ctypes.data,
+                           ub.ctypes.data))
+    return nx.value, nf.value, lb, ub
+
+
+def mo_fitness(kind: int, prob_id: int, dim: int, dvs, nobj: int = 2, alpha: int = 100) -> np.ndarray:
+    """Fitness of every row of dvs for a ZDT or DTLZ problem (n x nf)."""
+    dvs = np.ascontiguousarray(dvs, np.float64)
+    if dvs.ndim == 1:
+        dvs = dvs[None, :]
+    n, nx = dvs.shape
+    _, nf, _, _ = mo_info(kind, prob_id, dim, nobj, alpha)
+    out = np.empty((n, nf))
+    _chk(lib().orc_mo_fitness(kind, prob_id, dim, nobj, alpha, dvs, n, nx, out))
     return out
 
 

@@ -68,6 +68,8 @@ enum kind_t : int {
     GRIEWANK = 5,
     LENNARD_JONES = 6,
     MO_SPHERE = 7,
+    ZDT = 8,
+    DTLZ = 9,
 };
 
 // ---------------------------------------------------------------------------
@@ -179,6 +181,131 @@ inline vector_double lennard_jones_fitness(unsigned m_atoms, const vector_double
     return f;
 }
 
+// ZDT1-6 (src/problems/zdt.cpp:233-356), restated in the reference's order.
+inline vector_double zdt_fitness(unsigned id, const vector_double &x)
+{
+    const double pi = 3.1415926535897932384626433832795028841971693993;
+    vector_double f(2, 0.);
+    const auto N = x.size();
+    if (id == 5u) {
+        // zdt.cpp:298-339
+        const auto n_vectors = ((N - 30u) / 5u) + 1u;
+        std::vector<std::size_t> u(n_vectors, 0u), v(n_vectors);
+        vector_double r;
+        for (double item : x) r.push_back(std::round(item));
+        u[0] = static_cast<std::size_t>(std::count(r.begin(), r.begin() + 30, 1.));
+        std::size_t k = 30u;
+        for (std::size_t i = 1u; i < n_vectors; ++i) {
+            for (int j = 0; j < 5; ++j) {
+                if (r[k] == 1.) ++u[i];
+                ++k;
+            }
+        }
+        f[0] = 1.0 + static_cast<double>(u[0]);
+        for (std::size_t i = 1u; i < n_vectors; ++i) v[i] = u[i] < 5u ? 2u + u[i] : 1u;
+        double g = 0.;
+        for (std::size_t i = 1u; i < n_vectors; ++i) g += static_cast<double>(v[i]);
+        f[1] = g * (1. / f[0]);
+        return f;
+    }
+    double g = 0.;
+    f[0] = x[0];
+    if (id == 4u) {
+        // zdt.cpp:281-296
+        g = 1 + 10 * static_cast<double>(N - 1u);
+        for (std::size_t i = 1u; i < N; ++i) g += x[i] * x[i] - 10. * std::cos(4. * pi * x[i]);
+        f[1] = g * (1. - std::sqrt(x[0] / g));
+        return f;
+    }
+    for (std::size_t i = 1u; i < N; ++i) g += x[i];
+    if (id == 6u) {
+        // zdt.cpp:341-356
+        f[0] = 1 - std::exp(-4 * x[0]) * std::pow(std::sin(6 * pi * x[0]), 6);
+        g = 1 + 9 * std::pow((g / static_cast<double>(N - 1u)), 0.25);
+        f[1] = g * (1 - (f[0] / g) * (f[0] / g));
+        return f;
+    }
+    g = 1. + (9. * g) / static_cast<double>(N - 1u);
+    if (id == 1u) {
+        f[1] = g * (1. - std::sqrt(x[0] / g));
+    } else if (id == 2u) {
+        f[1] = g * (1. - (x[0] / g) * (x[0] / g));
+    } else {
+        f[1] = g * (1. - std::sqrt(x[0] / g) - x[0] / g * std::sin(10. * pi * x[0]));
+    }
+    return f;
+}
+
+// DTLZ1-7 (src/problems/dtlz.cpp:181-407), restated in the reference's order
+// with its std::pow calls.
+inline double dtlz_g(unsigned id, const vector_double &xm)
+{
+    const double pi = 3.1415926535897932384626433832795028841971693993;
+    double y = 0.;
+    switch (id) {
+        case 1u:
+        case 3u:
+            for (std::size_t i = 0u; i < xm.size(); ++i) {
+                y += std::pow(xm[i] - 0.5, 2) - std::cos(20. * pi * (xm[i] - 0.5));
+            }
+            return 100. * (y + static_cast<double>(xm.size()));
+        case 6u:
+            for (std::size_t i = 0u; i < xm.size(); ++i) y += std::pow(xm[i], 0.1);
+            return y;
+        case 7u:
+            for (std::size_t i = 0u; i < xm.size(); ++i) y += xm[i];
+            return (9. / static_cast<double>(xm.size())) * y;
+        default:
+            for (std::size_t i = 0u; i < xm.size(); ++i) y += std::pow(xm[i] - 0.5, 2);
+            return y;
+    }
+}
+
+inline vector_double dtlz_fitness(unsigned id, std::size_t M, unsigned alpha, const vector_double &x)
+{
+    const double pi = 3.1415926535897932384626433832795028841971693993;
+    const double pi_half = 3.1415926535897932384626433832795028841971693993 / 2.;
+    vector_double f(M);
+    const vector_double xm(x.begin() + static_cast<std::ptrdiff_t>(M - 1u), x.end());
+    if (id == 7u) {
+        const double g = 1. + dtlz_g(id, xm);
+        for (std::size_t i = 0u; i < M - 1u; ++i) f[i] = x[i];
+        double y = 0.;
+        for (std::size_t i = 0u; i < M - 1u; ++i) y += (f[i] / (1.0 + g)) * (1.0 + std::sin(3 * pi * f[i]));
+        f[M - 1u] = (1. + g) * (static_cast<double>(M) - y);
+        return f;
+    }
+    const double g = dtlz_g(id, xm);
+    if (id == 1u) {
+        f[0] = 0.5 * (1. + g);
+        for (std::size_t i = 0u; i < M - 1u; ++i) f[0] *= x[i];
+        for (std::size_t i = 1u; i < M - 1u; ++i) {
+            f[i] = 0.5 * (1.0 + g);
+            for (std::size_t j = 0u; j < M - (i + 1); ++j) f[i] *= x[j];
+            f[i] *= 1. - x[M - (i + 1u)];
+        }
+        f[M - 1u] = 0.5 * (1. - x[0]) * (1. + g);
+        return f;
+    }
+    vector_double v(M);
+    if (id == 5u || id == 6u) {
+        v[0] = x[0];
+        const double t = 1. / (2. * (1. + g));
+        for (std::size_t i = 1u; i < M; ++i) v[i] = t + ((g * x[i]) / (1.0 + g));
+    } else {
+        for (std::size_t i = 0u; i < M; ++i) v[i] = id == 4u ? std::pow(x[i], alpha) : x[i];
+    }
+    f[0] = (1. + g);
+    for (std::size_t i = 0u; i < M - 1u; ++i) f[0] *= std::cos(v[i] * pi_half);
+    for (std::size_t i = 1u; i < M - 1u; ++i) {
+        f[i] = (1. + g);
+        for (std::size_t j = 0u; j < M - (i + 1u); ++j) f[i] *= std::cos(v[j] * pi_half);
+        f[i] *= std::sin(v[M - (i + 1u)] * pi_half);
+    }
+    f[M - 1u] = (1. + g) * std::sin(v[0] * pi_half);
+    return f;
+}
+
 // Synthetic multi-objective problem of the ranking workload:
 // f_k(x) = sum_i (x_i - c_{k,i})^2, accumulated left to right from 0.
 // Not a pagmo UDP (a user-defined problem of the benchmark).
@@ -275,16 +402,20 @@ struct udp {
     int kind = ROSENBROCK;
     unsigned dim = 2;      // rosenbrock m_dim / rastrigin..griewank m_dim / mo_sphere nx
     unsigned atoms = 3;    // lennard_jones m_atoms
-    unsigned nobj = 1;     // mo_sphere number of objectives
+    unsigned nobj = 1;     // mo_sphere / dtlz number of objectives
     vector_double centers; // mo_sphere centres, nobj x dim row-major
+    unsigned prob_id = 0;  // zdt / dtlz problem number
+    unsigned alpha = 100;  // dtlz alpha (dtlz.hpp:105-106)
 
     std::size_t get_nx() const
     {
-        return kind == LENNARD_JONES ? 3u * atoms - 6u : dim;
+        if (kind == LENNARD_JONES) return 3u * atoms - 6u;
+        if (kind == ZDT && prob_id == 5u) return 30u + 5u * (dim - 1u); // zdt.cpp:126-129
+        return dim;
     }
     std::size_t get_nf() const
     {
-        return kind == MO_SPHERE ? nobj : 1u;
+        return kind == MO_SPHERE || kind == DTLZ ? nobj : (kind == ZDT ? 2u : 1u);
     }
     // Box bounds: rosenbrock.cpp:72-75, rastrigin.cpp:80-85, ackley.cpp:85-90,
     // schwefel.cpp:77-82, griewank.cpp:84-89, lennard_jones.cpp:99-111.
@@ -312,6 +443,14 @@ struct udp {
                 }
                 return {lb, ub};
             }
+            case ZDT:
+                if (prob_id == 4u) { // zdt.cpp:118-125
+                    vector_double lb(nx, -5.), ub(nx, 5.);
+                    lb[0] = 0.0;
+                    ub[0] = 1.0;
+                    return {lb, ub};
+                }
+                return {vector_double(nx, 0.), vector_double(nx, 1.)};
             default:
                 return {vector_double(nx, 0.), vector_double(nx, 1.)};
         }
@@ -333,6 +472,10 @@ struct udp {
                 return lennard_jones_fitness(atoms, x);
             case MO_SPHERE:
                 return mo_sphere_fitness(centers, nobj, x);
+            case ZDT:
+                return zdt_fitness(prob_id, x);
+            case DTLZ:
+                return dtlz_fitness(prob_id, nobj, alpha, x);
         }
         throw std::invalid_argument("unknown problem kind");
     }

@@ -44,6 +44,17 @@ int guarded(F &&f)
         return -1;
     }
 }
+oracle::udp make_mo_udp(int kind, uint32_t prob_id, uint64_t dim, uint64_t nobj, uint32_t alpha)
+{
+    oracle::udp p;
+    p.kind = kind;
+    p.prob_id = prob_id;
+    p.dim = static_cast<unsigned>(dim);
+    p.nobj = static_cast<unsigned>(nobj);
+    p.alpha = alpha;
+    return p;
+}
+
 } // namespace
 
 extern "C" {
@@ -88,6 +99,33 @@ int orc_problem_info(int kind, uint64_t dim, uint64_t nobj, uint64_t *nx, uint64
         const auto b = p.get_bounds();
         if (lb) std::memcpy(lb, b.first.data(), b.first.size() * sizeof(double));
         if (ub) std::memcpy(ub, b.second.data(), b.second.size() * sizeof(double));
+    });
+}
+
+// ZDT / DTLZ: problem shape and batch fitness (single-threaded).
+int orc_mo_info(int kind, uint32_t prob_id, uint64_t dim, uint64_t nobj, uint32_t alpha, uint64_t *nx, uint64_t *nf,
+                double *lb, double *ub)
+{
+    return guarded([&] {
+        auto p = make_mo_udp(kind, prob_id, dim, nobj, alpha);
+        *nx = p.get_nx();
+        *nf = p.get_nf();
+        const auto b = p.get_bounds();
+        if (lb) std::memcpy(lb, b.first.data(), b.first.size() * sizeof(double));
+        if (ub) std::memcpy(ub, b.second.data(), b.second.size() * sizeof(double));
+    });
+}
+
+int orc_mo_fitness(int kind, uint32_t prob_id, uint64_t dim, uint64_t nobj, uint32_t alpha, const double *dvs,
+                   uint64_t n_dvs, uint64_t nx, double *fvs)
+{
+    return guarded([&] {
+        auto p = make_mo_udp(kind, prob_id, dim, nobj, alpha);
+        const auto nf = p.get_nf();
+        for (uint64_t i = 0; i < n_dvs; ++i) {
+            const auto f = p.fitness(std::vector<double>(dvs + i * nx, dvs + (i + 1u) * nx));
+            std::memcpy(fvs + i * nf, f.data(), nf * sizeof(double));
+        }
     });
 }
 

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpagmo_hip.so")
 
 OK, INVALID_ARGUMENT, OVERFLOW, UNSUPPORTED, DEVICE_ERROR, NO_DEVICE, OUT_OF_MEMORY = range(7)
-ROSENBROCK, RASTRIGIN, ACKLEY, SCHWEFEL, GRIEWANK, LENNARD_JONES, MO_SPHERE = range(1, 8)
+ROSENBROCK, RASTRIGIN, ACKLEY, SCHWEFEL, GRIEWANK, LENNARD_JONES, MO_SPHERE, ZDT, DTLZ = range(1, 10)
 ARCHI_BATCHED, ARCHI_PER_ISLAND = 0, 1
 
 # every symbol include/pagmo_hip/pgh.h declares
@@ -42,8 +42,8 @@ class PghError(RuntimeError):
 
 
 class pgh_udp(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("reserved", C.c_uint32), ("dim", C.c_uint64), ("nobj", C.c_uint64),
-                ("centers", C.c_void_p)]
+    _fields_ = [("kind", C.c_int32), ("prob_id", C.c_uint32), ("dim", C.c_uint64), ("nobj", C.c_uint64),
+                ("centers", C.c_void_p), ("alpha", C.c_uint64)]
 
 
 class pgh_pso_params(C.Structure):
@@ -139,17 +139,19 @@ def check(status: int) -> None:
         raise PghError(status, lib().pgh_last_error().decode())
 
 
-def make_udp(kind: int, dim: int, centers=None, nobj: int = 1) -> pgh_udp:
-    """centers: a device tensor (device entry points) or a numpy array (host)."""
+def make_udp(kind: int, dim: int, centers=None, nobj: int = 1, prob_id: int = 0, alpha: int = 0) -> pgh_udp:
+    """centers: a device tensor (device entry points) or a numpy array (host).
+    ZDT: dim = the constructor's param, prob_id 1..6; DTLZ: dim, nobj = fdim,
+    prob_id 1..7, alpha."""
     ptr = None
     if centers is not None:
         ptr = centers.data_ptr() if hasattr(centers, "data_ptr") else centers.ctypes.data
-    return pgh_udp(kind, 0, dim, nobj, ptr)
+    return pgh_udp(kind, prob_id, dim, nobj, ptr, alpha)
 
 
-def dims(kind: int, dim: int, nobj: int = 1):
+def dims(kind: int, dim: int, nobj: int = 1, prob_id: int = 0, alpha: int = 0):
     nx, nf = C.c_uint64(), C.c_uint64()
-    u = make_udp(kind, dim, None, nobj)
+    u = make_udp(kind, dim, None, nobj, prob_id, alpha)
     if kind == MO_SPHERE:
         u.centers = 1  # only the shape is resolved here
     check(lib().pgh_problem_dims(C.byref(u), C.byref(nx), C.byref(nf)))
@@ -173,8 +175,8 @@ def _hptr(a):
 
 # ---- device entry points (torch tensors on the GPU) -----------------------
 
-def bfe_eval(kind, dim, dvs, fvs, centers=None, nobj=1, stream=None):
-    u = make_udp(kind, dim, centers, nobj)
+def bfe_eval(kind, dim, dvs, fvs, centers=None, nobj=1, stream=None, prob_id=0, alpha=0):
+    u = make_udp(kind, dim, centers, nobj, prob_id, alpha)
     n, nx = dvs.shape
     check(lib().pgh_bfe_eval(C.byref(u), dvs.data_ptr(), n, nx, fvs.data_ptr(), fvs.shape[1], _stream(stream)))
 
@@ -299,12 +301,11 @@ class Fnds:
 
 # ---- host entry points (numpy) ------------------------------------------------
 
-def bfe_eval_host(kind, dim, dvs: np.ndarray, centers=None, nobj=1) -> np.ndarray:
+def bfe_eval_host(kind, dim, dvs: np.ndarray, centers=None, nobj=1, prob_id=0, alpha=0) -> np.ndarray:
     dvs = np.ascontiguousarray(dvs, np.float64)
     c = None if centers is None else np.ascontiguousarray(centers, np.float64)
-    u = make_udp(kind, dim, c, nobj)
-    nf = nobj if kind == MO_SPHERE else 1
-    nx = dims(kind, dim, nobj)[0]
+    u = make_udp(kind, dim, c, nobj, prob_id, alpha)
+    nx, nf = dims(kind, dim, nobj, prob_id, alpha)
     out = np.empty((dvs.size // nx, nf))
     check(lib().pgh_bfe_eval_host(C.byref(u), dvs.ctypes.data, dvs.size, out.ctypes.data, out.size))
     return out

@@ -111,14 +111,65 @@ pgh_status resolve(const pgh_udp *udp, prob &out)
             if (!udp->centers) return fail(PGH_INVALID_ARGUMENT, "MO_SPHERE needs centers");
             if (udp->nobj > 1024u) return fail(PGH_UNSUPPORTED, "MO_SPHERE supports at most 1024 objectives");
             break;
+        case PGH_ZDT:
+            // zdt.cpp:55-66, in its order
+            if (dim < 2u) {
+                return fail(PGH_INVALID_ARGUMENT, "ZDT test problems must have a minimum value of 2 for the "
+                                                  "constructing parameter (representing the dimension except for "
+                                                  "ZDT5), "
+                                                      + std::to_string(dim) + " requested");
+            }
+            if (udp->prob_id == 0u || udp->prob_id > 6u) {
+                return fail(PGH_INVALID_ARGUMENT, "ZDT test suite contains six (prob_id=[1 ... 6]) problems, prob_id="
+                                                      + std::to_string(udp->prob_id) + " was detected");
+            }
+            if (udp->prob_id == 5u && dim > lim / 5u) {
+                return fail(PGH_OVERFLOW, "problem dimension too large: " + std::to_string(dim));
+            }
+            break;
+        case PGH_DTLZ:
+            // dtlz.cpp:53-74, in its order (the overflow limits are reached
+            // only beyond this library's dimension limit, checked below)
+            if (udp->prob_id == 0u || udp->prob_id > 7u) {
+                return fail(PGH_INVALID_ARGUMENT, "DTLZ test suite contains seven (prob_id = [1 ... 7]) problems, "
+                                                  "prob_id="
+                                                      + std::to_string(udp->prob_id) + " was detected");
+            }
+            if (udp->nobj < 2u) {
+                return fail(PGH_INVALID_ARGUMENT, "DTLZ test problem have a minimum of 2 objectives: fdim="
+                                                      + std::to_string(udp->nobj) + " was detected");
+            }
+            if (dim <= udp->nobj) {
+                return fail(PGH_INVALID_ARGUMENT,
+                            "The problem dimension has to be larger than the number of objectives.");
+            }
+            if (udp->alpha > std::numeric_limits<uint32_t>::max()) {
+                return fail(PGH_INVALID_ARGUMENT, "DTLZ alpha out of range: " + std::to_string(udp->alpha));
+            }
+            if (udp->nobj > 1024u) return fail(PGH_UNSUPPORTED, "DTLZ supports at most 1024 objectives");
+            break;
         default:
             return fail(PGH_UNSUPPORTED, "no kernel for problem kind " + std::to_string(udp->kind));
     }
     if (dim > lim) return fail(PGH_OVERFLOW, "problem dimension too large: " + std::to_string(dim));
     out.dim = static_cast<uint32_t>(dim);
-    out.nx = udp->kind == PGH_LENNARD_JONES ? 3u * out.dim - 6u : out.dim;
-    out.nf = udp->kind == PGH_MO_SPHERE ? static_cast<uint32_t>(udp->nobj) : 1u;
+    switch (udp->kind) {
+        case PGH_LENNARD_JONES:
+            out.nx = 3u * out.dim - 6u;
+            break;
+        case PGH_ZDT:
+            // ZDT5: 30 bits, then 5 per further block (zdt.cpp:126-129)
+            out.nx = udp->prob_id == 5u ? 30u + 5u * (out.dim - 1u) : out.dim;
+            break;
+        default:
+            out.nx = out.dim;
+    }
+    out.nf = udp->kind == PGH_MO_SPHERE || udp->kind == PGH_DTLZ ? static_cast<uint32_t>(udp->nobj)
+             : udp->kind == PGH_ZDT                              ? 2u
+                                                                 : 1u;
     out.centers = udp->centers;
+    out.prob_id = udp->kind == PGH_ZDT || udp->kind == PGH_DTLZ ? udp->prob_id : 0u;
+    out.alpha = udp->kind == PGH_DTLZ ? static_cast<uint32_t>(udp->alpha) : 0u;
     return PGH_OK;
 }
 
@@ -384,6 +435,11 @@ pgh_status pgh_batch_random_dv(const pgh_udp *udp, uint64_t n, uint32_t seed, do
 {
     prob p;
     PGH_TRY(resolve(udp, p));
+    if (p.kind == PGH_ZDT || p.kind == PGH_DTLZ) {
+        // ZDT4's first bound and ZDT5's integer part are not drawn here
+        return fail(PGH_UNSUPPORTED, "batch_random_decision_vector on the device covers the continuous problems "
+                                     "with a kernel except ZDT and DTLZ");
+    }
     if (n && !out) return fail(PGH_INVALID_ARGUMENT, "null buffer");
     return launch_batch_random_dv(p, n, seed, out, static_cast<hipStream_t>(stream));
 }

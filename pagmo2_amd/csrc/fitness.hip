@@ -97,6 +97,140 @@ __global__ __launch_bounds__(kBlock) void k_mo_sphere(const double *__restrict__
     }
 }
 
+// ---- one lane per individual, rows staged in LDS ----------------------------
+// A workgroup copies its `rows` consecutive individuals (a contiguous span of
+// rows * nx doubles) into LDS with coalesced loads, at an odd row stride so
+// that the lanes' reads of one column fall in distinct banks, and each lane
+// then evaluates its own row in the same sequential order as the direct
+// kernels below (which serve rows too long for the tile).  Without the
+// staging every lane walks its own row and a wavefront's loads touch 64
+// cache lines each, which the L1 cannot hold for short rows.
+constexpr uint32_t kRowTileBytes = 32768;
+
+constexpr uint32_t row_stride(uint32_t nx)
+{
+    return nx | 1u;
+}
+
+// Rows per staged workgroup (a multiple of 64, at most 256; 0: not staged,
+// which includes nx = 1, whose reciprocal below does not fit 32 bits).
+constexpr uint32_t staged_rows(uint32_t nx)
+{
+    if (nx < 2u) return 0u;
+    const uint32_t r = kRowTileBytes / (8u * row_stride(nx));
+    return r >= 256u ? 256u : (r / 64u) * 64u;
+}
+
+struct row_task {
+    uint32_t nx, nf, dim, prob_id, alpha;
+    uint32_t rows;  // individuals per workgroup (staged_rows)
+    uint32_t magic; // floor(2^32 / nx) + 1: e / nx == umulhi(e, magic) for e * nx < 2^32
+    const double *centers;
+};
+
+constexpr uint32_t kStageUnroll = 4;
+
+// Element e of the workgroup's span into its LDS slot.
+__device__ __forceinline__ void stage_put(double *__restrict__ tile, uint32_t e, double v, uint32_t nx,
+                                          uint32_t stride, uint32_t magic)
+{
+    const uint32_t r = __umulhi(e, magic);
+    tile[r * stride + (e - r * nx)] = v;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_rows_staged(const double *__restrict__ dvs, uint64_t n, row_task t,
+                                                       double *__restrict__ fvs)
+{
+    extern __shared__ double tile[];
+    const uint32_t nx = t.nx, stride = row_stride(nx);
+    const uint64_t r0 = static_cast<uint64_t>(blockIdx.x) * t.rows;
+    const uint32_t cnt = static_cast<uint32_t>(min(static_cast<uint64_t>(t.rows), n - r0));
+    const uint32_t total = cnt * nx;
+    // the span starts at an even element (rows is a multiple of 64) of a
+    // 16-byte aligned matrix: pairs of elements are read with 16-byte loads,
+    // kStageUnroll of them in flight per lane, then placed in LDS
+    const double2 *src = reinterpret_cast<const double2 *>(dvs + r0 * nx);
+    const uint32_t pairs = total / 2u;
+    for (uint32_t base = threadIdx.x; base < pairs; base += kBlock * kStageUnroll) {
+        double2 v[kStageUnroll];
+#pragma unroll
+        for (uint32_t u = 0; u < kStageUnroll; ++u) {
+            const uint32_t q = base + u * kBlock;
+            if (q < pairs) v[u] = src[q];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kStageUnroll; ++u) {
+            const uint32_t q = base + u * kBlock;
+            if (q < pairs) {
+                stage_put(tile, 2u * q, v[u].x, nx, stride, t.magic);
+                stage_put(tile, 2u * q + 1u, v[u].y, nx, stride, t.magic);
+            }
+        }
+    }
+    if ((total & 1u) && threadIdx.x == 0) {
+        stage_put(tile, total - 1u, dvs[r0 * nx + total - 1u], nx, stride, t.magic);
+    }
+    __syncthreads();
+    const uint32_t i = threadIdx.x;
+    if (i >= cnt) return;
+    const double *x = tile + i * stride;
+    const uint64_t ind = r0 + i;
+    if constexpr (KIND == ROSENBROCK) {
+        fvs[ind] = rosenbrock_eval(x, t.dim);
+    } else if constexpr (KIND == MO_SPHERE) {
+        for (uint32_t k = 0; k < t.nf; ++k) {
+            fvs[ind * t.nf + k] = mo_sphere_eval(x, nx, t.centers + static_cast<uint64_t>(k) * nx);
+        }
+    } else if constexpr (KIND == ZDT) {
+        double f0, f1;
+        zdt_eval(t.prob_id, x, nx, f0, f1);
+        fvs[2u * ind] = f0;
+        fvs[2u * ind + 1u] = f1;
+    } else {
+        dtlz_eval(t.prob_id, x, nx, t.nf, t.alpha, fvs + ind * t.nf);
+    }
+}
+
+// Staged when the row fits the tile and the matrix is 16-byte aligned.
+bool use_staged(const prob &p, const double *dvs)
+{
+    return staged_rows(p.nx) != 0u && reinterpret_cast<uintptr_t>(dvs) % 16u == 0u;
+}
+
+template <int KIND>
+pgh_status launch_rows_staged(const prob &p, const double *dvs, uint64_t n, double *fvs, hipStream_t s)
+{
+    const uint32_t rows = staged_rows(p.nx);
+    const uint32_t magic = static_cast<uint32_t>((uint64_t(1) << 32) / p.nx + 1u);
+    const row_task t{p.nx, p.nf, p.dim, p.prob_id, p.alpha, rows, magic, p.centers};
+    const size_t lds = static_cast<size_t>(rows) * row_stride(p.nx) * sizeof(double);
+    k_rows_staged<KIND><<<dim3(static_cast<uint32_t>((n + rows - 1u) / rows)), dim3(kBlock), lds, s>>>(dvs, n, t,
+                                                                                                        fvs);
+    PGH_HIP(hipGetLastError());
+    return PGH_OK;
+}
+
+// ---- ZDT and DTLZ: one lane per individual ----------------------------------
+__global__ __launch_bounds__(kBlock) void k_zdt(const double *__restrict__ dvs, uint64_t n, uint32_t nx,
+                                                uint32_t id, double *__restrict__ fvs)
+{
+    const uint64_t ind = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (ind >= n) return;
+    double f0, f1;
+    zdt_eval(id, dvs + ind * nx, nx, f0, f1);
+    fvs[2u * ind] = f0;
+    fvs[2u * ind + 1u] = f1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_dtlz(const double *__restrict__ dvs, uint64_t n, uint32_t nx,
+                                                 uint32_t M, uint32_t id, uint32_t alpha, double *__restrict__ fvs)
+{
+    const uint64_t ind = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (ind >= n) return;
+    dtlz_eval(id, dvs + ind * nx, nx, M, alpha, fvs + ind * M);
+}
+
 // ---- lennard_jones: G individuals per workgroup staged in LDS --------------
 template <int L, uint32_t APL>
 __global__ __launch_bounds__(kBlock) void k_lj(const double *__restrict__ dvs, uint64_t n, uint32_t atoms,
@@ -260,6 +394,7 @@ pgh_status launch_fitness(const prob &p, const double *dvs, uint64_t n, double *
     if (check_launch_size(n, p.nx) != PGH_OK) return PGH_OVERFLOW;
     switch (p.kind) {
         case ROSENBROCK:
+            if (use_staged(p, dvs)) return launch_rows_staged<ROSENBROCK>(p, dvs, n, fvs, s);
             k_rosenbrock<<<dim3(blocks_for(n)), dim3(kBlock), 0, s>>>(dvs, n, p.nx, fvs);
             PGH_HIP(hipGetLastError());
             return PGH_OK;
@@ -274,7 +409,18 @@ pgh_status launch_fitness(const prob &p, const double *dvs, uint64_t n, double *
         case LENNARD_JONES:
             return launch_lj(dvs, n, p.dim, fvs, nullptr, s);
         case MO_SPHERE:
+            if (use_staged(p, dvs)) return launch_rows_staged<MO_SPHERE>(p, dvs, n, fvs, s);
             k_mo_sphere<<<dim3(blocks_for(n)), dim3(kBlock), 0, s>>>(dvs, n, p.nx, p.nf, p.centers, fvs);
+            PGH_HIP(hipGetLastError());
+            return PGH_OK;
+        case ZDT:
+            if (use_staged(p, dvs)) return launch_rows_staged<ZDT>(p, dvs, n, fvs, s);
+            k_zdt<<<dim3(blocks_for(n)), dim3(kBlock), 0, s>>>(dvs, n, p.nx, p.prob_id, fvs);
+            PGH_HIP(hipGetLastError());
+            return PGH_OK;
+        case DTLZ:
+            if (use_staged(p, dvs)) return launch_rows_staged<DTLZ>(p, dvs, n, fvs, s);
+            k_dtlz<<<dim3(blocks_for(n)), dim3(kBlock), 0, s>>>(dvs, n, p.nx, p.nf, p.prob_id, p.alpha, fvs);
             PGH_HIP(hipGetLastError());
             return PGH_OK;
     }

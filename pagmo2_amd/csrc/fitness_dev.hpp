@@ -26,7 +26,20 @@ constexpr double kPi = 3.1415926535897932384626433832795028841971693993;
 // rastrigin.cpp:65, ackley.cpp:65: omega = 2. * pi(), an exact doubling.
 constexpr double kOmega = 2. * kPi;
 
-enum : int { ROSENBROCK = 1, RASTRIGIN = 2, ACKLEY = 3, SCHWEFEL = 4, GRIEWANK = 5, LENNARD_JONES = 6, MO_SPHERE = 7 };
+// constants.hpp:41-44
+constexpr double kPiHalf = 3.1415926535897932384626433832795028841971693993 / 2.;
+
+enum : int {
+    ROSENBROCK = 1,
+    RASTRIGIN = 2,
+    ACKLEY = 3,
+    SCHWEFEL = 4,
+    GRIEWANK = 5,
+    LENNARD_JONES = 6,
+    MO_SPHERE = 7,
+    ZDT = 8,
+    DTLZ = 9
+};
 
 // Box bounds of each problem (the get_bounds() of every UDP: rosenbrock.cpp:72-75,
 // rastrigin.cpp:80-85, ackley.cpp:85-90, schwefel.cpp:77-82, griewank.cpp:84-89,
@@ -375,6 +388,150 @@ __device__ __forceinline__ double rosenbrock_grad(const double *__restrict__ x, 
 __device__ __forceinline__ double rastrigin_grad(double xi)
 {
     return 2 * xi + 10.0 * kOmega * sin_any(kOmega * xi);
+}
+
+// ---------------------------------------------------------------------------
+// ZDT (zdt.cpp:233-356): one lane per individual, each expression in the
+// reference's order and association.  Only + - * / sqrt and round for ZDT1,
+// ZDT2 and ZDT5 (bit-exact); ZDT3, ZDT4 and ZDT6 call the device sin, cos,
+// exp and pow.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void zdt_eval(uint32_t id, const double *__restrict__ x, uint32_t N, double &f0,
+                                         double &f1)
+{
+    if (id == 5u) {
+        // zdt.cpp:298-339: 30 bits, then blocks of 5; x rounded half away
+        // from zero and compared with 1
+        const uint32_t nvec = (N - 30u) / 5u + 1u;
+        uint32_t u0 = 0u;
+        for (uint32_t j = 0u; j < 30u; ++j) u0 += round(x[j]) == 1. ? 1u : 0u;
+        double g = 0.;
+        for (uint32_t i = 1u; i < nvec; ++i) {
+            uint32_t ui = 0u;
+            for (uint32_t j = 0u; j < 5u; ++j) ui += round(x[30u + 5u * (i - 1u) + j]) == 1. ? 1u : 0u;
+            const uint32_t vi = ui < 5u ? 2u + ui : 1u;
+            g += static_cast<double>(vi);
+        }
+        f0 = 1.0 + static_cast<double>(u0);
+        f1 = g * (1. / f0);
+        return;
+    }
+    const double x0 = x[0];
+    if (id == 4u) {
+        // zdt.cpp:281-296
+        double g = 1 + 10 * static_cast<double>(N - 1u);
+        for (uint32_t i = 1u; i < N; ++i) g += x[i] * x[i] - 10. * cos(4. * kPi * x[i]);
+        f0 = x0;
+        f1 = g * (1. - sqrt(x0 / g));
+        return;
+    }
+    double g = 0.;
+    for (uint32_t i = 1u; i < N; ++i) g += x[i];
+    if (id == 6u) {
+        // zdt.cpp:341-356
+        f0 = 1 - exp(-4 * x0) * pow(sin(6 * kPi * x0), 6);
+        g = 1 + 9 * pow((g / static_cast<double>(N - 1u)), 0.25);
+        f1 = g * (1 - (f0 / g) * (f0 / g));
+        return;
+    }
+    g = 1. + (9. * g) / static_cast<double>(N - 1u);
+    f0 = x0;
+    if (id == 1u) {
+        f1 = g * (1. - sqrt(x0 / g)); // zdt.cpp:233-247
+    } else if (id == 2u) {
+        f1 = g * (1. - (x0 / g) * (x0 / g)); // zdt.cpp:249-263
+    } else {
+        f1 = g * (1. - sqrt(x0 / g) - x0 / g * sin(10. * kPi * x0)); // zdt.cpp:265-279
+    }
+}
+
+// ---------------------------------------------------------------------------
+// DTLZ (dtlz.cpp:181-407): one lane per individual; M = fdim objectives, the
+// distance variables x_M = x[M-1 .. nx-1].  pow(t, 2) is written t * t, the
+// correctly rounded square (glibc's pow returns it too, to within 0.52 ulp).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double dtlz_g(uint32_t id, const double *__restrict__ xm, uint32_t K)
+{
+    double y = 0.;
+    switch (id) {
+        case 1u:
+        case 3u: // dtlz.cpp:207-214
+            for (uint32_t i = 0u; i < K; ++i) {
+                const double t = xm[i] - 0.5;
+                y += t * t - cos(20. * kPi * (xm[i] - 0.5));
+            }
+            return 100. * (y + static_cast<double>(K));
+        case 6u: // dtlz.cpp:225-232
+            for (uint32_t i = 0u; i < K; ++i) y += pow(xm[i], 0.1);
+            return y;
+        case 7u: // dtlz.cpp:234-244
+            for (uint32_t i = 0u; i < K; ++i) y += xm[i];
+            return (9. / static_cast<double>(K)) * y;
+        default: // 2, 4, 5: dtlz.cpp:216-223
+            for (uint32_t i = 0u; i < K; ++i) {
+                const double t = xm[i] - 0.5;
+                y += t * t;
+            }
+            return y;
+    }
+}
+
+// Position variable j of the shape functions: x_j (DTLZ1-3), x_j^alpha
+// (DTLZ4, dtlz.cpp:320-341), theta_j (DTLZ5-6, dtlz.cpp:343-383).
+__device__ __forceinline__ double dtlz_pos(uint32_t id, const double *__restrict__ x, uint32_t j, double g,
+                                           double alpha)
+{
+    if (id == 4u) return pow(x[j], alpha);
+    if (id == 5u || id == 6u) {
+        if (j == 0u) return x[0];
+        const double t = 1. / (2. * (1. + g));
+        return t + ((g * x[j]) / (1.0 + g));
+    }
+    return x[j];
+}
+
+__device__ __forceinline__ void dtlz_eval(uint32_t id, const double *__restrict__ x, uint32_t nx, uint32_t M,
+                                          uint32_t alpha, double *__restrict__ f)
+{
+    const double gd = dtlz_g(id, x + (M - 1u), nx - M + 1u);
+    if (id == 7u) {
+        // dtlz.cpp:385-407 with h7 (dtlz.cpp:246-256)
+        const double g = 1. + gd;
+        double y = 0.;
+        for (uint32_t i = 0u; i + 1u < M; ++i) {
+            f[i] = x[i];
+            y += (x[i] / (1.0 + g)) * (1.0 + sin(3 * kPi * x[i]));
+        }
+        f[M - 1u] = (1. + g) * (static_cast<double>(M) - y);
+        return;
+    }
+    const double g = gd;
+    if (id == 1u) {
+        // dtlz.cpp:258-283
+        double f0 = 0.5 * (1. + g);
+        for (uint32_t i = 0u; i + 1u < M; ++i) f0 *= x[i];
+        f[0] = f0;
+        for (uint32_t i = 1u; i + 1u < M; ++i) {
+            double fi = 0.5 * (1.0 + g);
+            for (uint32_t j = 0u; j < M - (i + 1u); ++j) fi *= x[j];
+            fi *= 1. - x[M - (i + 1u)];
+            f[i] = fi;
+        }
+        f[M - 1u] = 0.5 * (1. - x[0]) * (1. + g);
+        return;
+    }
+    // DTLZ2-6 (dtlz.cpp:285-383): products of cos(v_j pi/2) and one sin
+    const double a = static_cast<double>(alpha);
+    double f0 = (1. + g);
+    for (uint32_t i = 0u; i + 1u < M; ++i) f0 *= cos(dtlz_pos(id, x, i, g, a) * kPiHalf);
+    f[0] = f0;
+    for (uint32_t i = 1u; i + 1u < M; ++i) {
+        double fi = (1. + g);
+        for (uint32_t j = 0u; j < M - (i + 1u); ++j) fi *= cos(dtlz_pos(id, x, j, g, a) * kPiHalf);
+        fi *= sin(dtlz_pos(id, x, M - (i + 1u), g, a) * kPiHalf);
+        f[i] = fi;
+    }
+    f[M - 1u] = (1. + g) * sin(dtlz_pos(id, x, 0u, g, a) * kPiHalf);
 }
 
 // ---------------------------------------------------------------------------

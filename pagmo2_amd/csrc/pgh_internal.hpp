@@ -42,6 +42,8 @@ struct prob {
     uint32_t nx = 0;
     uint32_t nf = 0;
     const double *centers = nullptr; // MO_SPHERE, device pointer
+    uint32_t prob_id = 0;            // ZDT / DTLZ problem number
+    uint32_t alpha = 0;              // DTLZ alpha
 };
 
 // Validates a pgh_udp; fills `out`.  Does not touch centers' memory.

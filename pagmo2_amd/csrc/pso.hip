@@ -431,7 +431,7 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
         const pgh_island_spec &sp = islands[k];
         prob p;
         PGH_TRY(resolve(&sp.udp, p));
-        if (p.kind == PGH_MO_SPHERE) return fail(PGH_UNSUPPORTED, "pso_gen needs a single-objective problem");
+        if (p.nf != 1u) return fail(PGH_UNSUPPORTED, "pso_gen needs a single-objective problem");
         probs.push_back(p);
         const pgh_pso_params &q = sp.algo;
         // pso_gen.cpp:83-112 parameter checks

@@ -48,6 +48,7 @@ pgh_status launch_mt19937(uint32_t seed, uint64_t skip, uint64_t n, uint32_t *ou
 
 pgh_status launch_batch_random_dv(const prob &p, uint64_t n, uint32_t seed, double *out, hipStream_t s)
 {
+    if (p.kind == ZDT || p.kind == DTLZ) return fail(PGH_UNSUPPORTED, "no device draws for ZDT and DTLZ");
     const uint64_t count = n * p.nx;
     if (count == 0u) return PGH_OK;
     // every bound pair of the supported problems has lb < ub, so each element

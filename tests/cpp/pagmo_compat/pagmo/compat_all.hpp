@@ -145,6 +145,95 @@ private:
     unsigned m_atoms;
 };
 
+// ---- ZDT and DTLZ (zdt.hpp, dtlz.hpp: parameters private, as there) --------
+class zdt
+{
+public:
+    zdt(unsigned prob_id = 1u, unsigned param = 30u) : m_prob_id(prob_id), m_param(param)
+    {
+        if (param < 2u) {
+            throw std::invalid_argument("ZDT test problems must have a minimum value of 2 for the constructing "
+                                        "parameter (representing the dimension except for ZDT5), "
+                                        + std::to_string(param) + " requested");
+        }
+        if (prob_id == 0u || prob_id > 6u) {
+            throw std::invalid_argument("ZDT test suite contains six (prob_id=[1 ... 6]) problems, prob_id="
+                                        + std::to_string(prob_id) + " was detected");
+        }
+    }
+    vector_double fitness(const vector_double &x) const
+    {
+        return oracle::zdt_fitness(m_prob_id, x);
+    }
+    std::pair<vector_double, vector_double> get_bounds() const
+    {
+        return as_udp().get_bounds();
+    }
+    vector_double::size_type get_nobj() const
+    {
+        return 2u;
+    }
+    std::string get_name() const
+    {
+        return "ZDT" + std::to_string(m_prob_id);
+    }
+    oracle::udp as_udp() const
+    {
+        oracle::udp u;
+        u.kind = oracle::ZDT;
+        u.prob_id = m_prob_id;
+        u.dim = m_param;
+        return u;
+    }
+
+private:
+    unsigned m_prob_id;
+    unsigned m_param;
+};
+
+class dtlz
+{
+public:
+    dtlz(unsigned prob_id = 1u, vector_double::size_type dim = 5u, vector_double::size_type fdim = 3u,
+         unsigned alpha = 100u)
+        : m_prob_id(prob_id), m_alpha(alpha), m_dim(dim), m_fdim(fdim)
+    {
+        if (prob_id == 0u || prob_id > 7u) {
+            throw std::invalid_argument("DTLZ test suite contains seven (prob_id = [1 ... 7]) problems, prob_id="
+                                        + std::to_string(prob_id) + " was detected");
+        }
+        if (fdim < 2u) {
+            throw std::invalid_argument("DTLZ test problem have a minimum of 2 objectives: fdim="
+                                        + std::to_string(fdim) + " was detected");
+        }
+        if (dim <= fdim) {
+            throw std::invalid_argument("The problem dimension has to be larger than the number of objectives.");
+        }
+    }
+    vector_double fitness(const vector_double &x) const
+    {
+        return oracle::dtlz_fitness(m_prob_id, m_fdim, m_alpha, x);
+    }
+    std::pair<vector_double, vector_double> get_bounds() const
+    {
+        return {vector_double(m_dim, 0.), vector_double(m_dim, 1.)};
+    }
+    vector_double::size_type get_nobj() const
+    {
+        return m_fdim;
+    }
+    std::string get_name() const
+    {
+        return "DTLZ" + std::to_string(m_prob_id);
+    }
+
+private:
+    unsigned m_prob_id;
+    unsigned m_alpha;
+    vector_double::size_type m_dim;
+    vector_double::size_type m_fdim;
+};
+
 // ---- problem ---------------------------------------------------------------
 namespace detail
 {

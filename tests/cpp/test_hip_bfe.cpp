@@ -265,6 +265,33 @@ static void gpu_tests()
         CHECK(pagmo_hip::select_best_N_mo(pts, 250u) == oracle::select_best_N_mo(flat.data(), 600u, 3u, 250u, true));
     }
 
+    // ZDT and DTLZ through hip_bfe: bit-exact where only + - * / sqrt round
+    // (ZDT1, ZDT2, ZDT5), else within 1e-12 relative of the UDP
+    {
+        auto check_udp = [](const problem &p, std::size_t n, bool exact) {
+            const auto dvs = random_dvs(p, n, 21);
+            const auto f = hip_bfe{}(p, dvs);
+            const auto nx = p.get_nx(), nf = p.get_nf();
+            bool ok = f.size() == n * nf;
+            for (std::size_t i = 0; i < n && ok; ++i) {
+                const auto fi = p.fitness(vector_double(dvs.begin() + static_cast<std::ptrdiff_t>(nx * i),
+                                                        dvs.begin() + static_cast<std::ptrdiff_t>(nx * (i + 1))));
+                for (std::size_t k = 0; k < nf; ++k) {
+                    ok = ok && (exact ? fi[k] == f[nf * i + k]
+                                      : std::abs(fi[k] - f[nf * i + k]) <= 1e-12 * std::abs(fi[k]) + 1e-300);
+                }
+            }
+            return ok;
+        };
+        for (unsigned id = 1u; id <= 6u; ++id) {
+            CHECK(check_udp(problem{zdt{id, 30u}}, 500, id == 1u || id == 2u || id == 5u));
+        }
+        for (unsigned id = 1u; id <= 7u; ++id) {
+            CHECK(check_udp(problem{dtlz{id, 12u, 3u}}, 500, false));
+        }
+        CHECK(check_udp(problem{dtlz{4u, 10u, 4u, 9u}}, 300, false));
+    }
+
     // memetic local step: a hand-derived trace (rosenbrock at the origin:
     // g = (-2, 0); alpha 0.5 gives (0.5, 0), f = 6.5, rejected; alpha 0.25
     // gives (0.25, 0), f = 0.953125, accepted), then bit-exact parity with the
@@ -329,6 +356,32 @@ static void gpu_tests()
     }
 }
 
+// ZDT / DTLZ descriptions (no device needed): ids from the names, ZDT5's
+// param from nx, DTLZ4's alpha read back from one fitness probe
+static void mo_problem_cpu_tests()
+{
+    pgh_udp u;
+    CHECK(pagmo_hip::describe(problem{zdt{5u, 11u}}, u) && u.kind == PGH_ZDT && u.prob_id == 5u && u.dim == 11u);
+    CHECK(pagmo_hip::describe(problem{zdt{3u, 17u}}, u) && u.prob_id == 3u && u.dim == 17u && u.nobj == 2u);
+    CHECK(pagmo_hip::describe(problem{dtlz{2u, 9u, 4u}}, u) && u.kind == PGH_DTLZ && u.prob_id == 2u && u.dim == 9u
+          && u.nobj == 4u);
+    CHECK(pagmo_hip::describe(problem{dtlz{4u, 7u, 3u, 100u}}, u) && u.alpha == 100u);
+    CHECK(pagmo_hip::describe(problem{dtlz{4u, 7u, 3u, 7u}}, u) && u.alpha == 7u);
+    CHECK(pagmo_hip::describe(problem{dtlz{4u, 7u, 3u, 0u}}, u) && u.alpha == 0u);
+    // an alpha the probe cannot pin falls back to the CPU evaluator, exactly
+    problem big{dtlz{4u, 6u, 3u, 2000u}};
+    CHECK(!pagmo_hip::describe(big, u));
+    const auto dvs = random_dvs(big, 20, 9);
+    const auto f = hip_bfe{}(big, dvs);
+    bool same = f.size() == 60u;
+    for (std::size_t i = 0; i < 20u && same; ++i) {
+        const auto fi = big.fitness(vector_double(dvs.begin() + static_cast<std::ptrdiff_t>(6 * i),
+                                                  dvs.begin() + static_cast<std::ptrdiff_t>(6 * i + 6)));
+        for (std::size_t k = 0; k < 3u; ++k) same = same && fi[k] == f[3 * i + k];
+    }
+    CHECK(same);
+}
+
 // local_descent's argument checks happen before any device work
 static void local_descent_cpu_tests()
 {
@@ -344,6 +397,7 @@ int main(int argc, char **argv)
 {
     const bool gpu = argc > 1 && std::strcmp(argv[1], "gpu") == 0;
     cpu_tests();
+    mo_problem_cpu_tests();
     local_descent_cpu_tests();
     if (gpu) gpu_tests();
     std::printf("%s: %d failure(s)\n", gpu ? "gpu" : "cpu", g_fail);

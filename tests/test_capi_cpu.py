@@ -23,7 +23,8 @@ def test_exports_match_header():
 
 
 def test_abi_version():
-    assert P.lib().pgh_abi_version() == 1
+    assert P.lib().pgh_abi_version() == 2
+    assert C.sizeof(P.pgh_udp) == 40
 
 
 def test_problem_dims():
@@ -132,3 +133,23 @@ def test_local_descent_host_checks():
     assert P.lib().pgh_local_descent_work_len(1000, 30) == 2 * 1000 * 30 + 2 * 1000
     x0, f0 = P.local_descent_host(P.ROSENBROCK, 3, np.zeros((0, 3)), 5, 0.1)
     assert x0.shape == (0, 3) and f0.shape == (0,)
+
+
+
+def test_zdt_dtlz_descriptions(kats):
+    # shapes (zdt.cpp:108-157, dtlz.cpp:116-119) and the constructors' messages
+    assert P.dims(P.ZDT, 30, prob_id=1) == (30, 2)
+    assert P.dims(P.ZDT, 30, prob_id=5) == (175, 2)
+    assert P.dims(P.ZDT, 11, prob_id=5) == ((11 - 1) * 5 + 30, 2)
+    assert P.dims(P.DTLZ, 7, nobj=3, prob_id=5, alpha=100) == (7, 3)
+    for c in kats["zdt_throws"]:
+        with pytest.raises(P.PghError) as e:
+            P.dims(P.ZDT, c["param"], prob_id=c["prob_id"])
+        assert e.value.status == P.INVALID_ARGUMENT and str(e.value) == c["message"]
+    for c in kats["dtlz_throws"]:
+        with pytest.raises(P.PghError) as e:
+            P.dims(P.DTLZ, c["dim"], nobj=c["fdim"], prob_id=c["prob_id"], alpha=100)
+        assert e.value.status == P.INVALID_ARGUMENT and str(e.value) == c["message"]
+    # no device seeding for these two families
+    u = P.make_udp(P.ZDT, 30, prob_id=1)
+    assert P.lib().pgh_batch_random_dv(C.byref(u), 4, 1, None, None) == P.UNSUPPORTED

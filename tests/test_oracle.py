@@ -481,3 +481,53 @@ def test_local_descent_invariants(kind, dim):
     assert np.all((x >= lb) & (x <= ub))
     assert np.array_equal(np.any(x != x0, axis=1), f < f0)
 
+
+
+# ---- ZDT and DTLZ (tests/zdt.cpp, tests/dtlz.cpp) ---------------------------
+
+def kat_x(spec):
+    if "fill" in spec:
+        n, v = spec["fill"]
+        return np.full(n, v)
+    n, v, at, w = spec["fill2"]
+    x = np.full(n, v)
+    x[at:] = w
+    return x
+
+
+def test_zdt_kats(kats):
+    tol = kats["zdt_fitness"]["rel_tol"]
+    for c in kats["zdt_fitness"]["cases"]:
+        f = O.mo_fitness(O.ZDT, c["prob_id"], c["param"], kat_x(c["x"]))[0]
+        for got, want in zip(f, c["f"]):
+            assert abs(got - want) <= tol * abs(want), c
+
+
+def test_dtlz_kats(kats):
+    tol = kats["dtlz_fitness"]["rel_tol"]
+    for c in kats["dtlz_fitness"]["cases"]:
+        f = O.mo_fitness(O.DTLZ, c["prob_id"], c["dim"], np.array(c["x"]), c["fdim"], c["alpha"])[0]
+        for got, want in zip(f, c["f"]):
+            assert abs(got - want) <= tol * abs(want), c
+
+
+def test_zdt_dtlz_shapes_and_bounds():
+    # zdt.cpp:108-132 and tests/zdt.cpp:62; dtlz.cpp:116-119 and tests/dtlz.cpp:184-190
+    assert O.mo_info(O.ZDT, 5, 11)[0] == (11 - 1) * 5 + 30
+    nx, nf, lb, ub = O.mo_info(O.ZDT, 4, 10)
+    assert (nx, nf) == (10, 2) and lb[0] == 0.0 and ub[0] == 1.0 and np.all(lb[1:] == -5.0) and np.all(ub[1:] == 5.0)
+    for pid in range(1, 8):
+        nx, nf, lb, ub = O.mo_info(O.DTLZ, pid, 4, 3)
+        assert (nx, nf) == (4, 3) and np.all(lb == 0.0) and np.all(ub == 1.0)
+
+
+def test_dtlz_closed_forms():
+    # the Pareto-optimal sets: x_M = 0.5 (DTLZ1-5) gives g = 0, where DTLZ1's
+    # objectives sum to 0.5 and DTLZ2's squares sum to 1 (unit sphere)
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        x = np.concatenate([rng.uniform(0, 1, 2), np.full(5, 0.5)])
+        f1 = O.mo_fitness(O.DTLZ, 1, 7, x, 3)[0]
+        assert abs(f1.sum() - 0.5) < 1e-15
+        f2 = O.mo_fitness(O.DTLZ, 2, 7, x, 3)[0]
+        assert abs(np.sum(f2 * f2) - 1.0) < 1e-15
