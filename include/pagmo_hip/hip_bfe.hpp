@@ -46,6 +46,7 @@
 #include <pagmo/problems/rastrigin.hpp>
 #include <pagmo/problems/rosenbrock.hpp>
 #include <pagmo/problems/schwefel.hpp>
+#include <pagmo/problems/translate.hpp>
 #include <pagmo/problems/zdt.hpp>
 #include <pagmo/threading.hpp>
 #include <pagmo/types.hpp>
@@ -113,10 +114,8 @@ inline bool dtlz4_alpha(const pagmo::dtlz &q, std::size_t nx, uint64_t &alpha)
     alpha = cand;
     return true;
 }
-} // namespace detail
-
-// The device description of p, if a kernel exists for it.
-inline bool describe(const pagmo::problem &p, pgh_udp &u)
+// The device description of p itself (no meta-problem), if a kernel exists.
+inline bool describe_udp(const pagmo::problem &p, pgh_udp &u)
 {
     u = pgh_udp{};
     if (const auto *r = p.extract<pagmo::rosenbrock>()) {
@@ -160,6 +159,24 @@ inline bool describe(const pagmo::problem &p, pgh_udp &u)
     }
     u.nobj = 1;
     return pgh_supported(&u) == 1;
+}
+} // namespace detail
+
+// The device description of p, if a kernel exists for it: one of the UDPs
+// above, or a translate (translate.hpp) of one, whose vector the host-buffer
+// entry points subtract on the device (x - t, translate.cpp:414-423).  A
+// translate of a translate rounds twice and is left to the CPU evaluator;
+// decompose keeps its weights and method private (decompose.hpp:219-224,
+// printed rounded by get_extra_info) and is left there too.
+inline bool describe(const pagmo::problem &p, pgh_udp &u)
+{
+    if (const auto *t = p.extract<pagmo::translate>()) {
+        const pagmo::problem &inner = t->get_inner_problem();
+        if (inner.extract<pagmo::translate>() || !detail::describe_udp(inner, u)) return false;
+        u.shift = t->get_translation().data();
+        return pgh_supported(&u) == 1;
+    }
+    return detail::describe_udp(p, u);
 }
 
 class hip_bfe

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define PGH_ABI_VERSION 2
+#define PGH_ABI_VERSION 3
 
 /* Status codes.  The C++ layer (pagmo_hip/hip_bfe.hpp) maps them onto the
  * exception types of the reference: INVALID_ARGUMENT -> std::invalid_argument
@@ -64,7 +64,13 @@ typedef enum pgh_udp_kind {
  *  nobj: MO_SPHERE objectives; DTLZ `fdim`; ignored otherwise.
  *  centers: MO_SPHERE only (nobj x dim, row-major; a device pointer for
  *       device entry points, a host pointer for _host ones).
- *  alpha: DTLZ's `alpha` (used by DTLZ4); ignored otherwise. */
+ *  alpha: DTLZ's `alpha` (used by DTLZ4); ignored otherwise.
+ *  shift: NULL, or the translation vector t (nx values, host memory) of a
+ *       pagmo::translate around the problem: the fitness of x is the problem's
+ *       fitness of x - t, element by element (translate.cpp:101-105,
+ *       414-423).  Only the host-buffer entry points of fitness and gradient
+ *       accept it (they subtract t from their staged copy of dvs); the device
+ *       entry points, which may not modify their input, return UNSUPPORTED. */
 typedef struct pgh_udp {
     int32_t kind;
     uint32_t prob_id;
@@ -72,6 +78,7 @@ typedef struct pgh_udp {
     uint64_t nobj;
     const double *centers;
     uint64_t alpha;
+    const double *shift;
 } pgh_udp;
 
 /* ---- library ------------------------------------------------------------ */

@@ -43,7 +43,7 @@ class PghError(RuntimeError):
 
 class pgh_udp(C.Structure):
     _fields_ = [("kind", C.c_int32), ("prob_id", C.c_uint32), ("dim", C.c_uint64), ("nobj", C.c_uint64),
-                ("centers", C.c_void_p), ("alpha", C.c_uint64)]
+                ("centers", C.c_void_p), ("alpha", C.c_uint64), ("shift", C.c_void_p)]
 
 
 class pgh_pso_params(C.Structure):
@@ -139,14 +139,17 @@ def check(status: int) -> None:
         raise PghError(status, lib().pgh_last_error().decode())
 
 
-def make_udp(kind: int, dim: int, centers=None, nobj: int = 1, prob_id: int = 0, alpha: int = 0) -> pgh_udp:
+def make_udp(kind: int, dim: int, centers=None, nobj: int = 1, prob_id: int = 0, alpha: int = 0,
+             shift=None) -> pgh_udp:
     """centers: a device tensor (device entry points) or a numpy array (host).
     ZDT: dim = the constructor's param, prob_id 1..6; DTLZ: dim, nobj = fdim,
-    prob_id 1..7, alpha."""
+    prob_id 1..7, alpha.  shift: a translation vector (numpy, host memory;
+    the caller keeps it alive during the call)."""
     ptr = None
     if centers is not None:
         ptr = centers.data_ptr() if hasattr(centers, "data_ptr") else centers.ctypes.data
-    return pgh_udp(kind, prob_id, dim, nobj, ptr, alpha)
+    sp = None if shift is None else shift.ctypes.data
+    return pgh_udp(kind, prob_id, dim, nobj, ptr, alpha, sp)
 
 
 def dims(kind: int, dim: int, nobj: int = 1, prob_id: int = 0, alpha: int = 0):
@@ -301,19 +304,21 @@ class Fnds:
 
 # ---- host entry points (numpy) ------------------------------------------------
 
-def bfe_eval_host(kind, dim, dvs: np.ndarray, centers=None, nobj=1, prob_id=0, alpha=0) -> np.ndarray:
+def bfe_eval_host(kind, dim, dvs: np.ndarray, centers=None, nobj=1, prob_id=0, alpha=0, shift=None) -> np.ndarray:
     dvs = np.ascontiguousarray(dvs, np.float64)
     c = None if centers is None else np.ascontiguousarray(centers, np.float64)
-    u = make_udp(kind, dim, c, nobj, prob_id, alpha)
+    t = None if shift is None else np.ascontiguousarray(shift, np.float64)
+    u = make_udp(kind, dim, c, nobj, prob_id, alpha, t)
     nx, nf = dims(kind, dim, nobj, prob_id, alpha)
     out = np.empty((dvs.size // nx, nf))
     check(lib().pgh_bfe_eval_host(C.byref(u), dvs.ctypes.data, dvs.size, out.ctypes.data, out.size))
     return out
 
 
-def batch_gradient_host(kind, dim, dvs: np.ndarray, with_fitness=False):
+def batch_gradient_host(kind, dim, dvs: np.ndarray, with_fitness=False, shift=None):
     dvs = np.ascontiguousarray(dvs, np.float64)
-    u = make_udp(kind, dim)
+    t = None if shift is None else np.ascontiguousarray(shift, np.float64)
+    u = make_udp(kind, dim, shift=t)
     g = np.empty_like(dvs)
     nx = dims(kind, dim)[0]
     f = np.empty(dvs.size // nx) if with_fitness else None

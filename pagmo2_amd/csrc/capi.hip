@@ -170,6 +170,7 @@ pgh_status resolve(const pgh_udp *udp, prob &out)
     out.centers = udp->centers;
     out.prob_id = udp->kind == PGH_ZDT || udp->kind == PGH_DTLZ ? udp->prob_id : 0u;
     out.alpha = udp->kind == PGH_DTLZ ? static_cast<uint32_t>(udp->alpha) : 0u;
+    out.shift = udp->shift;
     return PGH_OK;
 }
 
@@ -215,9 +216,27 @@ pgh_status check_input_len(const prob &p, uint64_t len, uint64_t &n_dvs)
     return PGH_OK;
 }
 
+// A translation is applied only by the host-buffer fitness and gradient entry
+// points, on their staged copy of the decision vectors.
+pgh_status reject_shift(const prob &p, const char *what)
+{
+    if (!p.shift) return PGH_OK;
+    return fail(PGH_UNSUPPORTED, std::string(what) + " does not take a translated problem");
+}
+
+// Subtracts the translation from the staged decision vectors x (n_elems).
+pgh_status apply_shift(const prob &p, double *x, uint64_t n_elems, thread_ctx *ctx)
+{
+    if (!p.shift) return PGH_OK;
+    PGH_TRY(ctx->e.reserve(p.nx * sizeof(double)));
+    PGH_HIP(hipMemcpyAsync(ctx->e.ptr, p.shift, p.nx * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    return launch_shift(x, n_elems, p.nx, static_cast<const double *>(ctx->e.ptr), ctx->stream);
+}
+
 // pgh_local_descent's arguments beyond the problem and the buffers
 pgh_status check_local_descent(const prob &p, const double *lb, const double *ub, double step0)
 {
+    PGH_TRY(reject_shift(p, "the local descent"));
     PGH_TRY(check_has_gradient(p));
     if ((lb == nullptr) != (ub == nullptr)) {
         return fail(PGH_INVALID_ARGUMENT, "the bounds of a local descent are both given or both absent");
@@ -292,6 +311,7 @@ pgh_status pgh_bfe_eval(const pgh_udp *udp, const double *dvs, uint64_t n_dvs, u
 {
     prob p;
     PGH_TRY(resolve(udp, p));
+    PGH_TRY(reject_shift(p, "pgh_bfe_eval"));
     PGH_TRY(check_shape(p, nx, nf));
     if (n_dvs && (!dvs || !fvs)) return fail(PGH_INVALID_ARGUMENT, "null buffer");
     return launch_fitness(p, dvs, n_dvs, fvs, static_cast<hipStream_t>(stream));
@@ -315,6 +335,7 @@ pgh_status pgh_bfe_eval_host(const pgh_udp *udp, const double *dvs, uint64_t dvs
     PGH_TRY(ctx->b.reserve(out_bytes));
     PGH_TRY(stage_centers(p, udp, ctx->c, ctx->stream));
     PGH_HIP(hipMemcpyAsync(ctx->a.ptr, dvs, in_bytes, hipMemcpyHostToDevice, ctx->stream));
+    PGH_TRY(apply_shift(p, static_cast<double *>(ctx->a.ptr), dvs_len, ctx));
     PGH_TRY(launch_fitness(p, static_cast<const double *>(ctx->a.ptr), n_dvs, static_cast<double *>(ctx->b.ptr),
                            ctx->stream));
     PGH_HIP(hipMemcpyAsync(fvs, ctx->b.ptr, out_bytes, hipMemcpyDeviceToHost, ctx->stream));
@@ -327,6 +348,7 @@ pgh_status pgh_batch_gradient(const pgh_udp *udp, const double *dvs, uint64_t n_
 {
     prob p;
     PGH_TRY(resolve(udp, p));
+    PGH_TRY(reject_shift(p, "pgh_batch_gradient"));
     PGH_TRY(check_shape(p, nx, p.nf));
     PGH_TRY(check_has_gradient(p));
     if (n_dvs && (!dvs || !grad)) return fail(PGH_INVALID_ARGUMENT, "null buffer");
@@ -353,6 +375,8 @@ pgh_status pgh_batch_gradient_host(const pgh_udp *udp, const double *dvs, uint64
     PGH_TRY(ctx->b.reserve(bytes));
     PGH_TRY(ctx->d.reserve(n_dvs * sizeof(double)));
     PGH_HIP(hipMemcpyAsync(ctx->a.ptr, dvs, bytes, hipMemcpyHostToDevice, ctx->stream));
+    // translate forwards the gradient at x - t (translate.cpp:245-249)
+    PGH_TRY(apply_shift(p, static_cast<double *>(ctx->a.ptr), dvs_len, ctx));
     PGH_TRY(launch_gradient(p, static_cast<const double *>(ctx->a.ptr), n_dvs, static_cast<double *>(ctx->b.ptr),
                             fvs ? static_cast<double *>(ctx->d.ptr) : nullptr, ctx->stream));
     PGH_HIP(hipMemcpyAsync(grad, ctx->b.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
@@ -435,6 +459,7 @@ pgh_status pgh_batch_random_dv(const pgh_udp *udp, uint64_t n, uint32_t seed, do
 {
     prob p;
     PGH_TRY(resolve(udp, p));
+    PGH_TRY(reject_shift(p, "pgh_batch_random_dv"));
     if (p.kind == PGH_ZDT || p.kind == PGH_DTLZ) {
         // ZDT4's first bound and ZDT5's integer part are not drawn here
         return fail(PGH_UNSUPPORTED, "batch_random_decision_vector on the device covers the continuous problems "

@@ -377,6 +377,25 @@ uint32_t blocks_for(uint64_t count)
 
 } // namespace
 
+namespace
+{
+__global__ __launch_bounds__(kBlock) void k_shift(double *__restrict__ x, uint64_t count, uint32_t nx,
+                                                  const double *__restrict__ t)
+{
+    const uint64_t e = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (e < count) x[e] = x[e] - t[e % nx];
+}
+} // namespace
+
+pgh_status launch_shift(double *x, uint64_t count, uint32_t nx, const double *t, hipStream_t s)
+{
+    if (count == 0u) return PGH_OK;
+    if (count > (uint64_t(1) << 39)) return fail(PGH_OVERFLOW, "batch too large for one launch");
+    k_shift<<<dim3(blocks_for(count)), dim3(kBlock), 0, s>>>(x, count, nx, t);
+    PGH_HIP(hipGetLastError());
+    return PGH_OK;
+}
+
 // Grid sizes stay below 2^31 workgroups: at most 2^31 individuals per call and
 // 2^39 matrix elements for the elementwise gradient kernels.
 pgh_status check_launch_size(uint64_t n, uint32_t nx)

@@ -44,12 +44,15 @@ struct prob {
     const double *centers = nullptr; // MO_SPHERE, device pointer
     uint32_t prob_id = 0;            // ZDT / DTLZ problem number
     uint32_t alpha = 0;              // DTLZ alpha
+    const double *shift = nullptr;   // translate's vector, host pointer
 };
 
 // Validates a pgh_udp; fills `out`.  Does not touch centers' memory.
 pgh_status resolve(const pgh_udp *udp, prob &out);
 
 // ---- launchers (fitness.hip) -----------------------------------------------
+// x[e] -= t[e % nx] for the count elements of a row-major matrix (translate).
+pgh_status launch_shift(double *x, uint64_t count, uint32_t nx, const double *t, hipStream_t s);
 pgh_status launch_fitness(const prob &p, const double *dvs, uint64_t n_dvs, double *fvs, hipStream_t s);
 pgh_status launch_gradient(const prob &p, const double *dvs, uint64_t n_dvs, double *grad, double *fvs,
                            hipStream_t s);
@@ -81,7 +84,7 @@ struct dev_buf {
 // Per-host-thread stream and staging buffers for the _host entry points.
 struct thread_ctx {
     hipStream_t stream = nullptr;
-    dev_buf a, b, c, d;
+    dev_buf a, b, c, d, e;
 };
 pgh_status get_thread_ctx(thread_ctx **ctx);
 

@@ -432,6 +432,7 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
         prob p;
         PGH_TRY(resolve(&sp.udp, p));
         if (p.nf != 1u) return fail(PGH_UNSUPPORTED, "pso_gen needs a single-objective problem");
+        if (p.shift) return fail(PGH_UNSUPPORTED, "the archipelago does not take translated problems");
         probs.push_back(p);
         const pgh_pso_params &q = sp.algo;
         // pso_gen.cpp:83-112 parameter checks

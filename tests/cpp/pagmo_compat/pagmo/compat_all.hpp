@@ -14,6 +14,7 @@
 //   pso_gen with set_bfe (src/algorithms/pso_gen.cpp)
 #pragma once
 
+#include <algorithm>
 #include <atomic>
 #include <functional>
 #include <memory>
@@ -521,6 +522,63 @@ inline std::vector<pop_size_t> select_best_N_mo(const std::vector<vector_double>
 }
 
 // ---- population and pso_gen ------------------------------------------------------
+// ---- translate (translate.hpp / translate.cpp) -----------------------------
+class translate
+{
+public:
+    template <typename T>
+    translate(T udp, const vector_double &translation) : m_problem(std::move(udp)), m_translation(translation)
+    {
+        if (translation.size() != m_problem.get_nx()) {
+            throw std::invalid_argument("Length of shift vector is: " + std::to_string(translation.size())
+                                        + " while the problem dimension is: " + std::to_string(m_problem.get_nx()));
+        }
+    }
+    vector_double fitness(const vector_double &x) const
+    {
+        return m_problem.fitness(translate_back(x));
+    }
+    vector_double gradient(const vector_double &x) const
+    {
+        return m_problem.gradient(translate_back(x));
+    }
+    std::pair<vector_double, vector_double> get_bounds() const
+    {
+        auto b = m_problem.get_bounds();
+        for (std::size_t i = 0; i < b.first.size(); ++i) {
+            b.first[i] += m_translation[i];
+            b.second[i] += m_translation[i];
+        }
+        return b;
+    }
+    std::string get_name() const
+    {
+        return m_problem.get_name() + " [translated]";
+    }
+    thread_safety get_thread_safety() const
+    {
+        return m_problem.get_thread_safety();
+    }
+    const problem &get_inner_problem() const
+    {
+        return m_problem;
+    }
+    const vector_double &get_translation() const
+    {
+        return m_translation;
+    }
+
+private:
+    vector_double translate_back(const vector_double &x) const
+    {
+        vector_double xs(x.size());
+        std::transform(x.begin(), x.end(), m_translation.begin(), xs.begin(), std::minus<double>());
+        return xs;
+    }
+    problem m_problem;
+    vector_double m_translation;
+};
+
 inline oracle::udp to_oracle(const problem &p)
 {
     if (auto q = p.extract<rosenbrock>()) return q->as_udp();

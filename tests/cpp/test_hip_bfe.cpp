@@ -292,6 +292,45 @@ static void gpu_tests()
         CHECK(check_udp(problem{dtlz{4u, 10u, 4u, 9u}}, 300, false));
     }
 
+    // translate around a kernel problem: the vector is subtracted on the
+    // device (x - t rounds exactly as translate_back does), so rosenbrock and
+    // ZDT1 stay bit-exact; the gradient is the inner one at x - t
+    {
+        const auto check_t = [](const problem &p, std::size_t n, bool exact) {
+            const auto dvs = random_dvs(p, n, 5);
+            const auto f = hip_bfe{}(p, dvs);
+            const auto nx = p.get_nx(), nf = p.get_nf();
+            bool ok = f.size() == n * nf;
+            for (std::size_t i = 0; i < n && ok; ++i) {
+                const auto fi = p.fitness(vector_double(dvs.begin() + static_cast<std::ptrdiff_t>(nx * i),
+                                                        dvs.begin() + static_cast<std::ptrdiff_t>(nx * (i + 1))));
+                for (std::size_t k = 0; k < nf; ++k) {
+                    ok = ok && (exact ? fi[k] == f[nf * i + k] : std::abs(fi[k] - f[nf * i + k]) <= 1e-12 * std::abs(fi[k]));
+                }
+            }
+            return ok;
+        };
+        std::mt19937 e(3);
+        std::uniform_real_distribution<double> d(-1., 1.);
+        vector_double t10(10), t50(50), t30(30);
+        for (auto &v : t10) v = d(e);
+        for (auto &v : t50) v = d(e);
+        for (auto &v : t30) v = 0.1 * d(e);
+        CHECK(check_t(problem{translate{rosenbrock{10u}, t10}}, 700, true));
+        CHECK(check_t(problem{translate{rastrigin{50u}, t50}}, 700, false));
+        CHECK(check_t(problem{translate{zdt{1u, 30u}, t30}}, 700, true));
+        problem tr{translate{rosenbrock{10u}, t10}};
+        const auto dvs = random_dvs(tr, 40, 6);
+        const auto g = pagmo_hip::batch_gradient(tr, dvs);
+        bool same = g.size() == dvs.size();
+        for (std::size_t i = 0; i < 40u && same; ++i) {
+            const auto gi = tr.gradient(vector_double(dvs.begin() + static_cast<std::ptrdiff_t>(10 * i),
+                                                      dvs.begin() + static_cast<std::ptrdiff_t>(10 * i + 10)));
+            for (std::size_t j = 0; j < 10u; ++j) same = same && gi[j] == g[10 * i + j];
+        }
+        CHECK(same);
+    }
+
     // memetic local step: a hand-derived trace (rosenbrock at the origin:
     // g = (-2, 0); alpha 0.5 gives (0.5, 0), f = 6.5, rejected; alpha 0.25
     // gives (0.25, 0), f = 0.953125, accepted), then bit-exact parity with the
@@ -382,6 +421,29 @@ static void mo_problem_cpu_tests()
     CHECK(same);
 }
 
+// translate: forwarded with its vector when it wraps a kernel problem; a
+// translate of a translate goes to the CPU evaluator, exactly
+static void translate_cpu_tests()
+{
+    pgh_udp u;
+    const vector_double t{0.5, -0.25, 1.0, 2.0};
+    problem tp{translate{rosenbrock{4u}, t}};
+    CHECK(pagmo_hip::describe(tp, u) && u.kind == PGH_ROSENBROCK && u.dim == 4u && u.shift != nullptr
+          && u.shift[3] == 2.0);
+    problem twice{translate{translate{rosenbrock{4u}, t}, vector_double{-0.5, 0.25, -1.0, -2.0}}};
+    CHECK(!pagmo_hip::describe(twice, u));
+    const auto dvs = random_dvs(twice, 30, 4);
+    const auto f = hip_bfe{}(twice, dvs);
+    bool same = f.size() == 30u;
+    for (std::size_t i = 0; i < 30u && same; ++i) {
+        same = twice.fitness(vector_double(dvs.begin() + static_cast<std::ptrdiff_t>(4 * i),
+                                           dvs.begin() + static_cast<std::ptrdiff_t>(4 * i + 4)))[0]
+               == f[i];
+    }
+    CHECK(same);
+    CHECK(!pagmo_hip::describe(problem{translate{user_udp{}, vector_double{1., 2.}}}, u));
+}
+
 // local_descent's argument checks happen before any device work
 static void local_descent_cpu_tests()
 {
@@ -398,6 +460,7 @@ int main(int argc, char **argv)
     const bool gpu = argc > 1 && std::strcmp(argv[1], "gpu") == 0;
     cpu_tests();
     mo_problem_cpu_tests();
+    translate_cpu_tests();
     local_descent_cpu_tests();
     if (gpu) gpu_tests();
     std::printf("%s: %d failure(s)\n", gpu ? "gpu" : "cpu", g_fail);

@@ -23,8 +23,8 @@ def test_exports_match_header():
 
 
 def test_abi_version():
-    assert P.lib().pgh_abi_version() == 2
-    assert C.sizeof(P.pgh_udp) == 40
+    assert P.lib().pgh_abi_version() == 3
+    assert C.sizeof(P.pgh_udp) == 48
 
 
 def test_problem_dims():
@@ -153,3 +153,19 @@ def test_zdt_dtlz_descriptions(kats):
     # no device seeding for these two families
     u = P.make_udp(P.ZDT, 30, prob_id=1)
     assert P.lib().pgh_batch_random_dv(C.byref(u), 4, 1, None, None) == P.UNSUPPORTED
+
+
+
+def test_translation_only_on_host_buffer_paths():
+    # the device entry points may not modify their input: a translated problem
+    # is refused there, before any device work
+    t = np.full(3, 0.5)
+    u = P.make_udp(P.ROSENBROCK, 3, shift=t)
+    out = np.zeros(2)
+    assert P.lib().pgh_bfe_eval(C.byref(u), 1, 2, 3, out.ctypes.data, 1, None) == P.UNSUPPORTED
+    assert "does not take a translated problem" in P.lib().pgh_last_error().decode()
+    assert P.lib().pgh_batch_gradient(C.byref(u), 1, 2, 3, 1, None, None) == P.UNSUPPORTED
+    assert P.lib().pgh_batch_random_dv(C.byref(u), 2, 1, None, None) == P.UNSUPPORTED
+    with pytest.raises(P.PghError, match="does not take a translated problem"):
+        P.check(P.lib().pgh_local_descent_host(C.byref(u), out.ctypes.data, 0, out.ctypes.data, 0, None, None, 1,
+                                               0.1))

@@ -261,3 +261,25 @@ def test_batch_random_dv_bit_exact(kind, dim, n):
     P.batch_random_dv(kind, dim, n, 42, out)
     torch.cuda.synchronize()
     assert np.array_equal(out.cpu().numpy(), O.batch_random_dv(kind, dim, n, 42))
+
+
+@pytest.mark.parametrize("kind,dim", [(P.ROSENBROCK, 10), (P.RASTRIGIN, 1000), (P.LENNARD_JONES, 13)])
+def test_translated_problem_host_path(kind, dim):
+    # translate (translate.cpp:101-105, 245-249): fitness and gradient of the
+    # inner problem at x - t, with t subtracted on the device
+    x = inputs(kind, dim, 300, 5, lo=-3.0 if kind == P.LENNARD_JONES else None,
+               hi=3.0 if kind == P.LENNARD_JONES else None)
+    t = np.random.default_rng(dim).uniform(-0.5, 0.5, x.shape[1])
+    f = P.bfe_eval_host(kind, dim, x, shift=t)[:, 0]
+    ref = O.bfe(kind, dim, x - t, threads=8)[:, 0]
+    g = P.batch_gradient_host(kind, dim, x, shift=t)
+    gref = O.gradient(kind, dim, x - t)
+    if kind == P.ROSENBROCK:
+        assert np.array_equal(f, ref) and np.array_equal(g, gref)
+    elif kind == P.RASTRIGIN:
+        assert np.all(np.abs(f - ref) <= 1e-12 * np.abs(ref))
+        assert np.allclose(g, gref, rtol=1e-12, atol=1e-12)
+    else:
+        for i in range(x.shape[0]):
+            assert abs(f[i] - ref[i]) <= 1e-12 * lj_abs_scale(x[i] - t)
+            assert np.linalg.norm(g[i] - gref[i]) <= 1e-12 * np.linalg.norm(gref[i])
