@@ -357,6 +357,8 @@ pgh_status launch_lj(const double *dvs, uint64_t n, uint32_t atoms, double *fvs,
                                          + " atoms does not fit the LDS of one workgroup");
     }
     switch (lj_lanes(atoms)) {
+        case 32:
+            return launch_lj_l<32>(dvs, n, atoms, fvs, grad, s);
         case 16:
             return launch_lj_l<16>(dvs, n, atoms, fvs, grad, s);
         case 8:

@@ -95,12 +95,22 @@ __host__ __device__ constexpr int sep_lanes(int kind, uint32_t nx)
     return nx >= 256u ? (kind == ACKLEY ? 32 : 64) : (nx >= 32u ? 16 : 4);
 }
 
-// Lennard-Jones: at least ~8 pair evaluations per lane, at most 16 lanes.
+// Lennard-Jones: lanes per individual.  At least ~8 pair evaluations per
+// lane, at most 16 lanes owning atoms; from 32 atoms on, 32 lanes: two halves
+// of 16 that own the same atoms and split the schedule's steps between them
+// (twice the wavefronts for the same work, to hide latency when a batch fills
+// the device only once).
 __host__ __device__ constexpr int lj_lanes(uint32_t atoms)
 {
     const uint32_t pairs = atoms * (atoms - 1u) / 2u;
     const uint32_t want = pairs / 8u;
-    return want >= 16u ? 16 : (want >= 8u ? 8 : (want >= 4u ? 4 : (want >= 2u ? 2 : 1)));
+    return atoms >= 32u ? 32 : (want >= 16u ? 16 : (want >= 8u ? 8 : (want >= 4u ? 4 : (want >= 2u ? 2 : 1))));
+}
+
+// Lanes of a group that own atoms (the rest repeat them for other steps).
+__host__ __device__ constexpr int lj_atom_lanes(int L)
+{
+    return L >= 32 ? 16 : L;
 }
 
 // Lennard-Jones individuals are staged in LDS with the atoms' (x, y, z) and a
@@ -577,25 +587,31 @@ __device__ __forceinline__ void lj_stage(const double *__restrict__ rows, uint32
 {
     const uint32_t nx = 3u * atoms - 6u;
     const uint32_t stride = lj_stride(atoms, grad);
-    const uint32_t copies = lj_copy_atoms(atoms, grad);
     const uint32_t copy = 3u * atoms;
-    for (uint32_t t = threadIdx.x; t < count * 6u; t += blockDim.x) {
+    const uint32_t copied = 3u * lj_copy_atoms(atoms, grad); // flat positions below this are copied
+    // each wavefront (or the partial one of a small workgroup) copies whole
+    // rows, its lanes taking consecutive elements: no element index is divided
+    const uint32_t wave = threadIdx.x / 64u, lane = threadIdx.x % 64u;
+    const uint32_t waves = (blockDim.x + 63u) / 64u;
+    const uint32_t width = min(64u, blockDim.x - 64u * wave);
+    for (uint32_t g = wave; g < count; g += waves) {
+        double *ind = lds + g * stride;
+        const double *row = rows + static_cast<uint64_t>(g) * nx;
         // the six fixed coordinates: atom 0 (x,y,z), atom 1 (x,y), atom 2 (x)
-        const uint32_t g = t / 6u, f = t % 6u;
-        const uint32_t atom = f < 3u ? 0u : (f < 5u ? 1u : 2u);
-        const uint32_t coord = f < 3u ? f : (f < 5u ? f - 3u : 0u);
-        double *dst = lds + g * stride + 3u * atom + coord;
-        dst[0] = 0.0;
-        if (atom < copies) dst[copy] = 0.0;
-    }
-    for (uint32_t t = threadIdx.x; t < count * nx; t += blockDim.x) {
-        const uint32_t g = t / nx, e = t - g * nx;
-        uint32_t atom, coord;
-        lj_slot(e, atom, coord);
-        const double v = rows[static_cast<uint64_t>(g) * nx + e];
-        double *dst = lds + g * stride + 3u * atom + coord;
-        dst[0] = v;
-        if (atom < copies) dst[copy] = v;
+        // at flat positions 0..4 and 6
+        for (uint32_t z = lane; z < 6u; z += width) {
+            const uint32_t f = z < 5u ? z : 6u;
+            ind[f] = 0.0;
+            if (f < copied) ind[copy + f] = 0.0;
+        }
+        // element e at flat position e + 6, except x[0] (atom 1's z) at 5
+        // (lj_slot)
+        for (uint32_t e = lane; e < nx; e += width) {
+            const uint32_t f = e == 0u ? 5u : e + 6u;
+            const double v = row[e];
+            ind[f] = v;
+            if (f < copied) ind[copy + f] = v;
+        }
     }
 }
 
@@ -771,9 +787,11 @@ __device__ __noinline__ double lj_energy_grad_careful(const double *__restrict__
 // with weight 0, so that no visit needs a mask.
 __host__ __device__ constexpr uint32_t lj_apl(uint32_t atoms, int L)
 {
-    const uint32_t apl = (atoms + static_cast<uint32_t>(L) - 1u) / static_cast<uint32_t>(L);
+    const uint32_t la = static_cast<uint32_t>(lj_atom_lanes(L));
+    const uint32_t apl = (atoms + la - 1u) / la;
     return apl <= 4u ? apl : 0u;
 }
+
 
 template <int L, uint32_t NS>
 struct lj_slots {
@@ -816,29 +834,47 @@ struct lj_pair {
     }
 };
 
+// Steps of one lane: the schedule's T steps are dealt round-robin to the KS
+// halves of a group, half h taking k = h + 1 + i KS.  All halves run the same
+// n = ceil(T / KS) steps: n - 1 steps that are always full, then a last one
+// that is full, the partial step A/2 (A even: only a < A/2 counts) or, past
+// T, a dummy of weight 0.
+struct lj_steps {
+    uint32_t first, stride, n, last, T;
+    bool dummy;
+    __device__ __forceinline__ lj_steps(uint32_t T_, uint32_t h, uint32_t KS)
+    {
+        T = T_;
+        first = h + 1u;
+        stride = KS;
+        n = (T + KS - 1u) / KS;
+        const uint32_t kt = first + (n - 1u) * KS;
+        dummy = kt > T;
+        last = dummy ? T : kt;
+    }
+};
+
 template <int L, uint32_t NS>
 __device__ __forceinline__ void lj_energy_slots(const double *__restrict__ P, uint32_t atoms, uint32_t a0, uint32_t r,
-                                                double &acc)
+                                                const lj_steps &st, double &acc)
 {
     const lj_cycle cyc(atoms);
     lj_slots<L, NS> sl;
     sl.load(P, a0, r, atoms);
-    const uint32_t full = cyc.full_steps();
-    for (uint32_t k = 1u; k <= full; ++k) {
+    for (uint32_t i = 0u; i + 1u < st.n; ++i) {
+        const uint32_t k = st.first + i * st.stride;
 #pragma unroll
         for (uint32_t q = 0; q < NS; ++q) {
             const lj_pair v(sl.x[q], sl.y[q], sl.z[q], sl.p[q] + 3u * k);
             acc = fma(sl.w[q], fma(v.s, v.s, -v.s), acc);
         }
     }
-    if (full < cyc.half) {
-        // A even, k = A/2: only a < A/2 counts
+    const bool partial = !st.dummy && st.last == cyc.half && !(atoms & 1u);
 #pragma unroll
-        for (uint32_t q = 0; q < NS; ++q) {
-            const lj_pair v(sl.x[q], sl.y[q], sl.z[q], sl.p[q] + 3u * cyc.half);
-            const double w = sl.a[q] < cyc.half ? sl.w[q] : 0.0;
-            acc = fma(w, fma(v.s, v.s, -v.s), acc);
-        }
+    for (uint32_t q = 0; q < NS; ++q) {
+        const lj_pair v(sl.x[q], sl.y[q], sl.z[q], sl.p[q] + 3u * st.last);
+        const double w = st.dummy || (partial && sl.a[q] >= cyc.half) ? 0.0 : sl.w[q];
+        acc = fma(w, fma(v.s, v.s, -v.s), acc);
     }
 }
 
@@ -851,12 +887,16 @@ __device__ __forceinline__ void lj_energy_slots(const double *__restrict__ P, ui
 template <int L, uint32_t APL>
 __device__ __forceinline__ double lj_energy(const double *__restrict__ P, uint32_t atoms, int r)
 {
+    constexpr int LA = lj_atom_lanes(L);
+    constexpr uint32_t KS = static_cast<uint32_t>(L / LA);
+    const uint32_t ra = static_cast<uint32_t>(r) % LA, h = static_cast<uint32_t>(r) / LA;
+    const lj_steps st(atoms / 2u, h, KS);
     double acc = 0.;
     if constexpr (APL > 0) {
-        lj_energy_slots<L, APL>(P, atoms, static_cast<uint32_t>(r), static_cast<uint32_t>(r), acc);
+        lj_energy_slots<LA, APL>(P, atoms, ra, ra, st, acc);
     } else {
-        for (uint32_t a0 = static_cast<uint32_t>(r); a0 < atoms; a0 += 4u * L) {
-            lj_energy_slots<L, 4>(P, atoms, a0, static_cast<uint32_t>(r), acc);
+        for (uint32_t a0 = ra; a0 < atoms; a0 += 4u * LA) {
+            lj_energy_slots<LA, 4>(P, atoms, a0, ra, st, acc);
         }
     }
     acc = group_sum<L>(acc);
@@ -864,16 +904,18 @@ __device__ __forceinline__ double lj_energy(const double *__restrict__ P, uint32
     return 4 * acc;
 }
 
-template <int L, uint32_t NS>
+template <int L, int LA, uint32_t NS>
 __device__ __forceinline__ void lj_grad_slots(const double *__restrict__ P, uint32_t atoms, uint32_t a0, uint32_t r,
-                                              double &acc, double &fsum, double *__restrict__ g)
+                                              const lj_steps &st, bool owner, double &acc, double &fsum,
+                                              double *__restrict__ g)
 {
-    lj_slots<L, NS> sl;
+    lj_slots<LA, NS> sl;
     sl.load(P, a0, r, atoms);
     double fx[NS], fy[NS], fz[NS];
 #pragma unroll
     for (uint32_t q = 0; q < NS; ++q) fx[q] = fy[q] = fz[q] = 0.;
-    for (uint32_t k = 1u; k < atoms; ++k) {
+    for (uint32_t i = 0u; i + 1u < st.n; ++i) {
+        const uint32_t k = st.first + i * st.stride;
 #pragma unroll
         for (uint32_t q = 0; q < NS; ++q) {
             const lj_pair v(sl.x[q], sl.y[q], sl.z[q], sl.p[q] + 3u * k);
@@ -885,6 +927,29 @@ __device__ __forceinline__ void lj_grad_slots(const double *__restrict__ P, uint
             acc = fma(sl.w[q], fma(v.s, v.s, -v.s), acc);
         }
     }
+    {
+        // the last step, possibly a dummy: weight 0 for the force and the energy
+        const double wk = st.dummy ? 0.0 : 1.0;
+#pragma unroll
+        for (uint32_t q = 0; q < NS; ++q) {
+            const lj_pair v(sl.x[q], sl.y[q], sl.z[q], sl.p[q] + 3u * st.last);
+            const double coef = ((fma(-48., v.s, 24.) * v.s) * v.inv) * wk;
+            fx[q] = fma(coef, v.dx, fx[q]);
+            fy[q] = fma(coef, v.dy, fy[q]);
+            fz[q] = fma(coef, v.dz, fz[q]);
+            acc = fma(sl.w[q] * wk, fma(v.s, v.s, -v.s), acc);
+        }
+    }
+    if constexpr (L > LA) {
+        // the two halves' partial forces of the same atoms (lanes r, r + LA)
+#pragma unroll
+        for (uint32_t q = 0; q < NS; ++q) {
+            fx[q] += __shfl_xor(fx[q], LA, L);
+            fy[q] += __shfl_xor(fy[q], LA, L);
+            fz[q] += __shfl_xor(fz[q], LA, L);
+        }
+    }
+    if (!owner) return;
 #pragma unroll
     for (uint32_t q = 0; q < NS; ++q) {
         fsum += fabs(fx[q]) + fabs(fy[q]) + fabs(fz[q]); // inf or NaN if a force is
@@ -892,7 +957,9 @@ __device__ __forceinline__ void lj_grad_slots(const double *__restrict__ P, uint
     }
 }
 
-// Energy and gradient: lane r owns atoms a = r, r+L, ... and visits every
+// Energy and gradient: lane r owns atoms a = r, r+LA, ... (LA =
+// lj_atom_lanes(L); with L = 2 LA the two halves take the steps k = 1..A-1 in
+// two ranges and add their forces) and visits every
 // other atom a + k, k = 1..A-1, accumulating the force
 // dE/dp_a = sum_b (p_a - p_b) (24 s - 48 s^2) / r^2 (s = r^-6; 1/r^2 = s r^4)
 // and the energy terms of all its visits (each pair twice, so the sum is
@@ -903,12 +970,16 @@ template <int L, uint32_t APL>
 __device__ __forceinline__ double lj_energy_grad(const double *__restrict__ P, uint32_t atoms, int r,
                                                  double *__restrict__ g)
 {
+    constexpr int LA = lj_atom_lanes(L);
+    constexpr uint32_t KS = static_cast<uint32_t>(L / LA);
+    const uint32_t ra = static_cast<uint32_t>(r) % LA, h = static_cast<uint32_t>(r) / LA;
+    const lj_steps st(atoms - 1u, h, KS);
     double acc = 0., fsum = 0.;
     if constexpr (APL > 0) {
-        lj_grad_slots<L, APL>(P, atoms, static_cast<uint32_t>(r), static_cast<uint32_t>(r), acc, fsum, g);
+        lj_grad_slots<L, LA, APL>(P, atoms, ra, ra, st, h == 0u, acc, fsum, g);
     } else {
-        for (uint32_t a0 = static_cast<uint32_t>(r); a0 < atoms; a0 += 4u * L) {
-            lj_grad_slots<L, 4>(P, atoms, a0, static_cast<uint32_t>(r), acc, fsum, g);
+        for (uint32_t a0 = ra; a0 < atoms; a0 += 4u * LA) {
+            lj_grad_slots<L, LA, 4>(P, atoms, a0, ra, st, h == 0u, acc, fsum, g);
         }
     }
     acc = group_sum<L>(acc);

@@ -305,6 +305,9 @@ __global__ __launch_bounds__(kBlock) void k_eval(const island_desc *__restrict__
             return;
         case LENNARD_JONES:
             switch (di.lanes) {
+                case 32:
+                    eval_lj<32>(di, tl, c, lds);
+                    return;
                 case 16:
                     eval_lj<16>(di, tl, c, lds);
                     return;

@@ -80,8 +80,9 @@ def test_lj_groups_change_no_result(atoms, n, grad, lj_defaults):
 
 
 def test_lj_set_groups_limits(lj_defaults):
-    with pytest.raises(P.PghError, match="at most 16 individuals per workgroup for 48 atoms"):
-        P.lj_set_groups(48, 0, 17)
+    # 48 atoms take 32 lanes each: 8 individuals fill a 256-lane workgroup
+    with pytest.raises(P.PghError, match="at most 8 individuals per workgroup for 48 atoms"):
+        P.lj_set_groups(48, 0, 9)
     with pytest.raises(P.PghError, match="atoms out of range"):
         P.lj_set_groups(2, 0, 1)
     x = dev(lj_inputs(13, 4, 1))
