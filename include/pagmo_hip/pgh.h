@@ -287,6 +287,55 @@ pgh_status pgh_archi_get(pgh_archi *a, uint64_t k, double *X, double *F);
 pgh_status pgh_archi_set_trace(pgh_archi *a, int on);
 pgh_status pgh_archi_get_trace(pgh_archi *a, uint64_t k, double *trace_F);
 
+/* ---- migration between the islands (island::evolve, src/island.cpp:452-642;
+ *      the migrants database, src/archipelago.cpp:658-713).  Once set, every
+ *      pgh_archi_evolve call is one archipelago::evolve(1) round: each island
+ *      takes migrants in (island index order), then every island evolves,
+ *      then each island's select_best emigrants replace its database entry.
+ *      The migrants stay on the device.  Single-objective, unconstrained
+ *      problems with the select_best and fair_replace policies. ------------ */
+enum pgh_migration_type {
+    PGH_MIGRATION_P2P = 0,      /* migration_type::p2p (archipelago.hpp) */
+    PGH_MIGRATION_BROADCAST = 1 /* migration_type::broadcast */
+};
+enum pgh_migrant_handling {
+    PGH_MIGRANTS_PRESERVE = 0, /* migrant_handling::preserve: sources keep their migrants */
+    PGH_MIGRANTS_EVICT = 1     /* migrant_handling::evict: taking migrants empties the source's entry */
+};
+/* A policy's migration rate (detail/base_sr_policy.hpp): `count`
+ * individuals, or with `fractional` set, `fraction` of the input individuals
+ * (truncated). */
+typedef struct pgh_migr_rate {
+    int32_t fractional;
+    uint32_t pad;
+    uint64_t count;
+    double fraction;
+} pgh_migr_rate;
+/* Turns migration on for the archipelago (conn_offsets null turns it off).
+ * Island k's inbound connections are conn_src / conn_weight over
+ * [conn_offsets[k], conn_offsets[k+1]), in topology::get_connections order
+ * (topology.hpp:173, e.g. ring.cpp, fully_connected.cpp); connected islands
+ * must have the same nx.  select_rate / replace_rate: each island's
+ * select_best and fair_replace rates (select_best.cpp:60-116,
+ * fair_replace.cpp:62-129); migr_seeds: the seed of each island's migration
+ * engine (std::mt19937; the reference seeds it from random_device, once per
+ * island::evolve call, island.cpp:466-468).  The engines persist across
+ * pgh_archi_evolve calls until the next pgh_archi_set_migration, so that a
+ * set followed by n evolve calls draws as one island::evolve(n) does.
+ * The database starts empty (archipelago.cpp:511).  Messages of the
+ * reference's checks: edge weights (topology.cpp:58-68), fractional rates
+ * (base_sr_policy.cpp:54-66) and absolute rates above the population size
+ * (select_best.cpp:90-96, fair_replace.cpp:93-99, raised here at set-up).
+ * Islands with pso_gen memory set are UNSUPPORTED. */
+pgh_status pgh_archi_set_migration(pgh_archi *a, int type, int handling, const uint64_t *conn_offsets,
+                                   const uint64_t *conn_src, const double *conn_weight,
+                                   const pgh_migr_rate *select_rate, const pgh_migr_rate *replace_rate,
+                                   const uint32_t *migr_seeds);
+/* Island k's entry of the migrants database (archipelago::get_migrants,
+ * archipelago.cpp:682-694): *count individuals, best first, to host X
+ * (count x nx) and F (count) when non-null.  Synchronous. */
+pgh_status pgh_archi_get_migrants(pgh_archi *a, uint64_t k, uint64_t *count, double *X, double *F);
+
 /* Single-island convenience: pso_gen(prm).evolve(pop) with host X/F in/out. */
 pgh_status pgh_pso_gen_evolve_host(const pgh_udp *udp, const pgh_pso_params *prm, double *X, double *F,
                                    uint64_t pop, uint64_t nx);

@@ -71,6 +71,13 @@ def _declare(L):
         "orc_select_best_N_mo": [_dp, u64, u64, u64, cint, _u64p, C.POINTER(u64)],
         "orc_sort_population_mo": [_dp, u64, u64, cint, _u64p],
         "orc_sort_population_mo_timed": [_dp, u64, u64, u32, C.POINTER(dbl)],
+        "orc_select_best": [u64, u64, _u64p, _dp, _dp, cint, u64, dbl, cint, C.POINTER(u64), _u64p, _dp, _dp],
+        "orc_fair_replace": [u64, u64, _u64p, _dp, _dp, u64, _u64p, _dp, _dp, cint, u64, dbl, cint, C.POINTER(u64),
+                             _u64p, _dp, _dp],
+        "orc_topology": [cint, u64, dbl, _u64p, _u64p, _dp],
+        "orc_archi_migrate": [u64, C.c_void_p, _u64p, _u64p, _dp, _dp, _u32p, _u32p, cint, cint, _u64p, _u64p, _dp,
+                              C.c_void_p, _u64p, _dp, C.c_void_p, _u64p, _dp, _u32p, u32, cint, _dp, _dp, _u64p,
+                              _dp, _dp],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
@@ -335,3 +342,103 @@ def pso_gen_timed(kind: int, dim: int, X: np.ndarray, F: np.ndarray, gen: int, s
     sec = dbl()
     _chk(lib().orc_pso_gen_timed(kind, dim, X.shape[0], X, F, gen, seed, threads, reps, C.byref(sec)))
     return sec.value
+
+
+# ---- migration (select_best, fair_replace, topologies, archipelago rounds) ----
+
+def _rate(r):
+    """A policy rate as pagmo takes it: an int is an absolute count, a float a
+    fraction of the input individuals."""
+    if isinstance(r, (float, np.floating)):
+        return 1, 0, float(r)
+    return 0, int(r), 0.0
+
+
+def _group(ids, X, F):
+    X = np.ascontiguousarray(X, np.float64)
+    n = X.shape[0]
+    ids = np.arange(n, dtype=np.uint64) if ids is None else np.ascontiguousarray(ids, np.uint64)
+    return ids, X, np.ascontiguousarray(F, np.float64).reshape(-1)
+
+
+def select_best(ids, X, F, rate, stable=False):
+    """select_best(rate).select on a single-objective group: (ids, X, F) of
+    the selected individuals, best first."""
+    ids, X, F = _group(ids, X, F)
+    n, nx = X.shape
+    fr, cnt, fra = _rate(rate)
+    no = u64()
+    oi, oX, oF = np.empty(n, np.uint64), np.empty((n, nx)), np.empty(n)
+    _chk(lib().orc_select_best(n, nx, ids, X, F, fr, cnt, fra, int(stable), C.byref(no), oi, oX, oF))
+    k = no.value
+    return oi[:k], oX[:k], oF[:k]
+
+
+def fair_replace(ids, X, F, mids, mX, mF, rate, stable=False):
+    """fair_replace(rate).replace of a single-objective group by migrants."""
+    ids, X, F = _group(ids, X, F)
+    mids, mX, mF = _group(mids, np.asarray(mX, np.float64).reshape(-1, X.shape[1]), mF)
+    n, nx = X.shape
+    fr, cnt, fra = _rate(rate)
+    no = u64()
+    oi, oX, oF = np.empty(n, np.uint64), np.empty((n, nx)), np.empty(n)
+    _chk(lib().orc_fair_replace(n, nx, ids, X, F, mX.shape[0], mids, mX, mF, fr, cnt, fra, int(stable), C.byref(no),
+                                oi, oX, oF))
+    return oi, oX, oF
+
+
+def topology(which: str, n: int, w: float = 1.0):
+    """Inbound connections of ring(n, w) or fully_connected(n, w): a list, per
+    island, of (source, weight) in get_connections order."""
+    off = np.zeros(n + 1, np.uint64)
+    cap = max(n * (n - 1), 1)
+    src, wt = np.zeros(cap, np.uint64), np.zeros(cap)
+    _chk(lib().orc_topology(0 if which == "ring" else 1, n, w, off, src, wt))
+    return [[(int(src[j]), float(wt[j])) for j in range(int(off[i]), int(off[i + 1]))] for i in range(n)]
+
+
+def archi_migrate(islands, conns, mtype, handling, rounds, stable=True):
+    """`rounds` archipelago::evolve(1) rounds with migration.  islands: dicts
+    with kind, dim, X (pop x nx), F, seed (pso_gen), gen, sel, rep (rates) and
+    migr_seed; conns as topology() returns.  mtype 0 p2p / 1 broadcast;
+    handling 0 preserve / 1 evict.  Returns the final (X, F) per island and
+    the migrants database as (X, F) per island."""
+    n = len(islands)
+    kinds = np.array([d["kind"] for d in islands], np.int32)
+    dims = np.array([d["dim"] for d in islands], np.uint64)
+    pops = np.array([np.asarray(d["X"]).shape[0] for d in islands], np.uint64)
+    nxs = [np.asarray(d["X"]).shape[1] for d in islands]
+    X0 = np.concatenate([np.ascontiguousarray(d["X"], np.float64).reshape(-1) for d in islands])
+    F0 = np.concatenate([np.ascontiguousarray(d["F"], np.float64).reshape(-1) for d in islands])
+    seeds = np.array([d["seed"] for d in islands], np.uint32)
+    gens = np.array([d["gen"] for d in islands], np.uint32)
+    off = np.zeros(n + 1, np.uint64)
+    src, wt = [], []
+    for k, c in enumerate(conns):
+        off[k + 1] = off[k] + len(c)
+        src += [s for s, _ in c]
+        wt += [w for _, w in c]
+    src = np.array(src + [0], np.uint64)
+    wt = np.array(wt + [0.0], np.float64)
+    sr = [_rate(d["sel"]) for d in islands]
+    rr = [_rate(d["rep"]) for d in islands]
+    sf = np.array([r[0] for r in sr], np.int32)
+    rf = np.array([r[0] for r in rr], np.int32)
+    mseeds = np.array([d["migr_seed"] for d in islands], np.uint32)
+    Xo, Fo = np.empty_like(X0), np.empty_like(F0)
+    dbc = np.zeros(n, np.uint64)
+    dbX, dbF = np.zeros_like(X0), np.zeros_like(F0)
+    _chk(lib().orc_archi_migrate(n, _cptr(kinds), dims, pops, X0, F0, seeds, gens, mtype, handling, off, src, wt,
+                                 _cptr(sf), np.array([r[1] for r in sr], np.uint64), np.array([r[2] for r in sr]),
+                                 _cptr(rf), np.array([r[1] for r in rr], np.uint64), np.array([r[2] for r in rr]),
+                                 mseeds, rounds, int(stable), Xo, Fo, dbc, dbX, dbF))
+    out, db = [], []
+    xo = fo = 0
+    for k in range(n):
+        p, nx = int(pops[k]), nxs[k]
+        out.append((Xo[xo:xo + p * nx].reshape(p, nx).copy(), Fo[fo:fo + p].copy()))
+        c = int(dbc[k])
+        db.append((dbX[xo:xo + c * nx].reshape(c, nx).copy(), dbF[fo:fo + c].copy()))
+        xo += p * nx
+        fo += p
+    return out, db

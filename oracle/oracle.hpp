@@ -1295,4 +1295,265 @@ private:
     std::vector<vector_double> m_V;
 };
 
+// ---------------------------------------------------------------------------
+// Migration: select_best, fair_replace, topologies and the archipelago's
+// migration step (src/island.cpp:452-642)
+// ---------------------------------------------------------------------------
+
+// A policy's migration rate: an absolute count or a fraction of the input
+// individuals (detail/base_sr_policy.hpp, boost::variant<pop_size_t, double>).
+struct migr_rate {
+    bool fractional = false;
+    std::uint64_t count = 1u;
+    double fraction = 0.;
+};
+
+// src/detail/base_sr_policy.cpp:54-66
+inline void check_migr_rate(const migr_rate &r)
+{
+    if (r.fractional && (!std::isfinite(r.fraction) || r.fraction < 0. || r.fraction > 1.)) {
+        throw std::invalid_argument("Invalid fractional migration rate specified in the constructor of a "
+                                    "replacement/selection policy: the rate must be in the [0., 1.] range, but it is "
+                                    + std::to_string(r.fraction) + " instead");
+    }
+}
+
+// The number of migrants of a rate over inds_size individuals
+// (select_best.cpp:76-97, fair_replace.cpp:79-103 before its min with the
+// migrants' count): numeric_cast truncates the fractional product.
+// `policy` is "'Select best' selection policy" or "'fair_replace'
+// replacement policy".
+inline std::uint64_t migr_count(const migr_rate &r, std::uint64_t inds_size, const std::string &policy)
+{
+    if (r.fractional) {
+        return std::min(static_cast<std::uint64_t>(r.fraction * static_cast<double>(inds_size)), inds_size);
+    }
+    if (r.count > inds_size) {
+        throw std::invalid_argument("The absolute migration rate (" + std::to_string(r.count) + ") in a " + policy
+                                    + " is larger than the number of input individuals ("
+                                    + std::to_string(inds_size) + ")");
+    }
+    return r.count;
+}
+
+// individuals_group_t (IDs, decision vectors, fitness vectors).
+struct ind_group {
+    std::vector<std::uint64_t> id;
+    std::vector<vector_double> x, f;
+};
+
+// Indices of g ordered by less_than_f on the first objective (std::sort as
+// in the reference, or std::stable_sort: see mo_sort).
+inline std::vector<pop_size_t> order_by_f(const ind_group &g, bool stable)
+{
+    std::vector<pop_size_t> idx(g.f.size());
+    std::iota(idx.begin(), idx.end(), pop_size_t(0));
+    mo_sort(idx.begin(), idx.end(), [&g](pop_size_t a, pop_size_t b) { return less_than_f(g.f[a][0], g.f[b][0]); },
+            stable);
+    return idx;
+}
+
+inline void push_ind(ind_group &to, const ind_group &from, pop_size_t i)
+{
+    to.id.push_back(from.id[i]);
+    to.x.push_back(from.x[i]);
+    to.f.push_back(from.f[i]);
+}
+
+// src/s_policies/select_best.cpp:60-116, single-objective unconstrained: the
+// n_migr best individuals, best first.
+inline ind_group select_best(const ind_group &inds, const migr_rate &r, bool stable = false)
+{
+    const auto n_migr = migr_count(r, inds.f.size(), "'Select best' selection policy");
+    const auto order = order_by_f(inds, stable);
+    ind_group out;
+    for (pop_size_t i = 0; i < n_migr; ++i) {
+        push_ind(out, inds, order[i]);
+    }
+    return out;
+}
+
+// src/r_policies/fair_replace.cpp:62-129, single-objective unconstrained: the
+// best n_migr migrants join the population, and the best inds_size of the
+// merged group, best first, are the new population.
+inline ind_group fair_replace(const ind_group &inds, const migr_rate &r, const ind_group &mig, bool stable = false)
+{
+    const auto inds_size = inds.f.size();
+    const auto n_migr = std::min<std::uint64_t>(migr_count(r, inds_size, "'fair_replace' replacement policy"),
+                                                mig.f.size());
+    const auto mig_order = order_by_f(mig, stable);
+    ind_group merged(inds);
+    for (pop_size_t i = 0; i < n_migr; ++i) {
+        push_ind(merged, mig, mig_order[i]);
+    }
+    const auto order = order_by_f(merged, stable);
+    ind_group out;
+    for (pop_size_t i = 0; i < inds_size; ++i) {
+        push_ind(out, merged, order[i]);
+    }
+    return out;
+}
+
+// Islands with an edge towards each island, in the order that
+// topology::get_connections returns them, and the edge weights.
+struct connections {
+    std::vector<std::vector<std::size_t>> src;
+    std::vector<vector_double> w;
+};
+
+// src/topology.cpp:58-68
+inline void check_edge_weight(double w)
+{
+    if (!std::isfinite(w)) {
+        throw std::invalid_argument("invalid weight for the edge of a topology: the value " + std::to_string(w)
+                                    + " is not finite");
+    }
+    if (w < 0. || w > 1.) {
+        throw std::invalid_argument("invalid weight for the edge of a topology: the value " + std::to_string(w)
+                                    + " is not in the [0., 1.] range");
+    }
+}
+
+// ring(n, w): the edges of ring::push_back (src/topologies/ring.cpp:61-100)
+// added and removed vertex by vertex.  base_bgl_topology::get_connections
+// (base_bgl_topology.cpp:214-230) walks inv_adjacent_vertices of a
+// bidirectional vecS adjacency_list (topology.hpp:109-115): each vertex's
+// in-edges in the order they were added, a removal erasing in place.
+inline connections ring_topology(std::size_t n, double w)
+{
+    check_edge_weight(w);
+    connections c;
+    auto add = [&c, w](std::size_t a, std::size_t b) {
+        c.src[b].push_back(a);
+        c.w[b].push_back(w);
+    };
+    auto remove = [&c](std::size_t a, std::size_t b) {
+        for (std::size_t j = 0; j < c.src[b].size(); ++j) {
+            if (c.src[b][j] == a) {
+                c.src[b].erase(c.src[b].begin() + static_cast<std::ptrdiff_t>(j));
+                c.w[b].erase(c.w[b].begin() + static_cast<std::ptrdiff_t>(j));
+                return;
+            }
+        }
+    };
+    for (std::size_t size = 1; size <= n; ++size) {
+        c.src.emplace_back();
+        c.w.emplace_back();
+        if (size == 2u) {
+            add(0, 1);
+            add(1, 0);
+        } else if (size == 3u) {
+            add(1, 2);
+            add(2, 1);
+            add(2, 0);
+            add(0, 2);
+        } else if (size > 3u) {
+            remove(size - 2u, 0);
+            remove(0, size - 2u);
+            add(size - 2u, size - 1u);
+            add(size - 1u, size - 2u);
+            add(0, size - 1u);
+            add(size - 1u, 0);
+        }
+    }
+    return c;
+}
+
+// fully_connected(n, w): every other vertex in ascending order
+// (src/topologies/fully_connected.cpp:86-116).
+inline connections fully_connected_topology(std::size_t n, double w)
+{
+    check_edge_weight(w);
+    connections c;
+    c.src.resize(n);
+    c.w.resize(n);
+    for (std::size_t i = 0; i < n; ++i) {
+        for (std::size_t j = 0; j < n; ++j) {
+            if (j != i) {
+                c.src[i].push_back(j);
+                c.w[i].push_back(w);
+            }
+        }
+    }
+    return c;
+}
+
+// One island of a migrating archipelago: a single-objective problem, its
+// pso_gen and population, the policies' rates and the migration engine.
+struct migr_island {
+    udp prob;
+    pso_gen algo;
+    ind_group pop;
+    migr_rate sel, rep;
+    std::mt19937 eng;
+};
+
+// One archipelago::evolve(1) round in which every island migrates in before
+// any island evolves, and emigrates after every island has evolved -- one of
+// the interleavings the reference's island threads can take; with it every
+// island reads the migrants database as the previous round left it.
+//   immigration (island.cpp:452-597), island by island in index order:
+//     p2p: one source drawn with uniform_int_distribution<size_t>(0, n-1)
+//     and kept if uniform_real_distribution<>{} < its weight; fair_replace
+//     only then.  broadcast: one draw per connection in order, the kept
+//     sources' migrants concatenated; fair_replace always.  The source's
+//     entry is copied (preserve) or taken and emptied (evict,
+//     archipelago.cpp:658-678).
+//   evolution: algo.evolve on the population (pso_gen, per-individual
+//     fitness).
+//   emigration (island.cpp:603-620): db[k] = select_best(population).
+// type: 0 p2p, 1 broadcast; handling: 0 preserve, 1 evict.
+inline void archi_migrate_round(std::vector<migr_island> &isl, const connections &c, int type, int handling,
+                                std::vector<ind_group> &db, bool stable)
+{
+    for (std::size_t k = 0; k < isl.size(); ++k) {
+        const auto &src = c.src[k];
+        if (src.empty()) {
+            continue;
+        }
+        auto take = [&db, handling](std::size_t s) {
+            ind_group g = db[s];
+            if (handling == 1) {
+                db[s] = ind_group{};
+            }
+            return g;
+        };
+        if (type == 0) {
+            const auto j = std::uniform_int_distribution<std::size_t>(0, src.size() - 1u)(isl[k].eng);
+            if (std::uniform_real_distribution<>{}(isl[k].eng) < c.w[k][j]) {
+                const auto mig = take(src[j]);
+                isl[k].pop = fair_replace(isl[k].pop, isl[k].rep, mig, stable);
+            }
+        } else {
+            ind_group mig;
+            for (std::size_t j = 0; j < src.size(); ++j) {
+                if (std::uniform_real_distribution<>{}(isl[k].eng) < c.w[k][j]) {
+                    const auto g = take(src[j]);
+                    for (pop_size_t i = 0; i < g.f.size(); ++i) {
+                        push_ind(mig, g, i);
+                    }
+                }
+            }
+            isl[k].pop = fair_replace(isl[k].pop, isl[k].rep, mig, stable);
+        }
+    }
+    for (auto &is : isl) {
+        const auto nx = is.prob.get_nx();
+        vector_double X, F;
+        for (pop_size_t i = 0; i < is.pop.x.size(); ++i) {
+            X.insert(X.end(), is.pop.x[i].begin(), is.pop.x[i].end());
+            F.push_back(is.pop.f[i][0]);
+        }
+        is.algo.evolve(is.prob, X, F, nullptr);
+        for (pop_size_t i = 0; i < is.pop.x.size(); ++i) {
+            is.pop.x[i].assign(X.begin() + static_cast<std::ptrdiff_t>(i * nx),
+                               X.begin() + static_cast<std::ptrdiff_t>((i + 1u) * nx));
+            is.pop.f[i] = {F[i]};
+        }
+    }
+    for (std::size_t k = 0; k < isl.size(); ++k) {
+        db[k] = select_best(isl[k].pop, isl[k].sel, stable);
+    }
+}
+
 } // namespace oracle

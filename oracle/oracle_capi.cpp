@@ -55,6 +55,37 @@ oracle::udp make_mo_udp(int kind, uint32_t prob_id, uint64_t dim, uint64_t nobj,
     return p;
 }
 
+oracle::migr_rate make_rate(int fractional, uint64_t count, double fraction)
+{
+    oracle::migr_rate r;
+    r.fractional = fractional != 0;
+    r.count = count;
+    r.fraction = fraction;
+    oracle::check_migr_rate(r);
+    return r;
+}
+
+oracle::ind_group make_group(uint64_t n, uint64_t nx, const uint64_t *ids, const double *X, const double *F)
+{
+    oracle::ind_group g;
+    for (uint64_t i = 0; i < n; ++i) {
+        g.id.push_back(ids ? ids[i] : i);
+        g.x.emplace_back(X + i * nx, X + (i + 1u) * nx);
+        g.f.push_back({F[i]});
+    }
+    return g;
+}
+
+void put_group(const oracle::ind_group &g, uint64_t nx, uint64_t *n, uint64_t *ids, double *X, double *F)
+{
+    *n = g.f.size();
+    for (uint64_t i = 0; i < g.f.size(); ++i) {
+        if (ids) ids[i] = g.id[i];
+        std::memcpy(X + i * nx, g.x[i].data(), nx * sizeof(double));
+        F[i] = g.f[i][0];
+    }
+}
+
 } // namespace
 
 extern "C" {
@@ -449,6 +480,100 @@ int orc_pso_gen_timed(int kind, uint64_t dim, uint64_t pop, const double *X, con
         }
         std::sort(t.begin(), t.end());
         *sec = t[t.size() / 2];
+    });
+}
+
+// select_best (src/s_policies/select_best.cpp:60-116); the outputs hold up to n
+// individuals.
+int orc_select_best(uint64_t n, uint64_t nx, const uint64_t *ids, const double *X, const double *F, int frac,
+                    uint64_t count, double fraction, int stable, uint64_t *n_out, uint64_t *ids_out, double *X_out,
+                    double *F_out)
+{
+    return guarded([&] {
+        const auto g = oracle::select_best(make_group(n, nx, ids, X, F), make_rate(frac, count, fraction), stable != 0);
+        put_group(g, nx, n_out, ids_out, X_out, F_out);
+    });
+}
+
+// fair_replace (src/r_policies/fair_replace.cpp:62-129); the outputs hold n
+// individuals.
+int orc_fair_replace(uint64_t n, uint64_t nx, const uint64_t *ids, const double *X, const double *F, uint64_t m,
+                     const uint64_t *mids, const double *mX, const double *mF, int frac, uint64_t count,
+                     double fraction, int stable, uint64_t *n_out, uint64_t *ids_out, double *X_out, double *F_out)
+{
+    return guarded([&] {
+        const auto g = oracle::fair_replace(make_group(n, nx, ids, X, F), make_rate(frac, count, fraction),
+                                            make_group(m, nx, mids, mX, mF), stable != 0);
+        put_group(g, nx, n_out, ids_out, X_out, F_out);
+    });
+}
+
+// Inbound connections of ring(n, w) (which = 0) or fully_connected(n, w)
+// (which = 1) as CSR: offsets (n + 1), sources and weights (n (n - 1) each at
+// most).
+int orc_topology(int which, uint64_t n, double w, uint64_t *offsets, uint64_t *src, double *wout)
+{
+    return guarded([&] {
+        const auto c = which == 0 ? oracle::ring_topology(n, w) : oracle::fully_connected_topology(n, w);
+        uint64_t o = 0;
+        for (uint64_t i = 0; i < n; ++i) {
+            offsets[i] = o;
+            for (std::size_t j = 0; j < c.src[i].size(); ++j, ++o) {
+                src[o] = c.src[i][j];
+                wout[o] = c.w[i][j];
+            }
+        }
+        offsets[n] = o;
+    });
+}
+
+// `rounds` archipelago::evolve(1) rounds with migration (oracle::
+// archi_migrate_round) over n_isl pso_gen islands.  Populations are
+// concatenated row-major (X0/X_out: pops[k] x nx_k after island k-1's rows,
+// F0/F_out: pops[k]); the final migrants database goes to db_count (per
+// island) and dbX/dbF at the same offsets as X_out/F_out.
+int orc_archi_migrate(uint64_t n_isl, const int *kinds, const uint64_t *dims, const uint64_t *pops, const double *X0,
+                      const double *F0, const uint32_t *pso_seeds, const uint32_t *gens, int type, int handling,
+                      const uint64_t *conn_off, const uint64_t *conn_src, const double *conn_w, const int *sel_frac,
+                      const uint64_t *sel_count, const double *sel_fraction, const int *rep_frac,
+                      const uint64_t *rep_count, const double *rep_fraction, const uint32_t *migr_seeds,
+                      uint32_t rounds, int stable, double *X_out, double *F_out, uint64_t *db_count, double *dbX,
+                      double *dbF)
+{
+    return guarded([&] {
+        std::vector<oracle::migr_island> isl;
+        oracle::connections c;
+        uint64_t xo = 0, fo = 0;
+        for (uint64_t k = 0; k < n_isl; ++k) {
+            auto p = make_udp(kinds[k], dims[k], nullptr, 1);
+            const auto nx = p.get_nx();
+            oracle::pso_params prm;
+            prm.gen = gens[k];
+            prm.seed = pso_seeds[k];
+            oracle::migr_island is{p, oracle::pso_gen(prm),
+                                   make_group(pops[k], nx, nullptr, X0 + xo, F0 + fo),
+                                   make_rate(sel_frac[k], sel_count[k], sel_fraction[k]),
+                                   make_rate(rep_frac[k], rep_count[k], rep_fraction[k]),
+                                   std::mt19937(migr_seeds[k])};
+            isl.push_back(std::move(is));
+            xo += pops[k] * nx;
+            fo += pops[k];
+            c.src.emplace_back(conn_src + conn_off[k], conn_src + conn_off[k + 1u]);
+            c.w.emplace_back(conn_w + conn_off[k], conn_w + conn_off[k + 1u]);
+        }
+        std::vector<oracle::ind_group> db(n_isl);
+        for (uint32_t r = 0; r < rounds; ++r) {
+            oracle::archi_migrate_round(isl, c, type, handling, db, stable != 0);
+        }
+        xo = fo = 0;
+        for (uint64_t k = 0; k < n_isl; ++k) {
+            const auto nx = isl[k].prob.get_nx();
+            uint64_t n = 0;
+            put_group(isl[k].pop, nx, &n, nullptr, X_out + xo, F_out + fo);
+            put_group(db[k], nx, &db_count[k], nullptr, dbX + xo, dbF + fo);
+            xo += pops[k] * nx;
+            fo += pops[k];
+        }
     });
 }
 

@@ -31,7 +31,7 @@ EXPORTS = [
     "pgh_mt19937_generate", "pgh_batch_random_dv",
     "pgh_archi_create", "pgh_archi_create_seeded", "pgh_archi_destroy", "pgh_archi_evolve", "pgh_archi_get",
     "pgh_archi_set_trace",
-    "pgh_archi_get_trace", "pgh_pso_gen_evolve_host",
+    "pgh_archi_get_trace", "pgh_archi_set_migration", "pgh_archi_get_migrants", "pgh_pso_gen_evolve_host",
 ]
 
 
@@ -54,6 +54,53 @@ class pgh_pso_params(C.Structure):
 
 class pgh_island_spec(C.Structure):
     _fields_ = [("udp", pgh_udp), ("pop", C.c_uint64), ("algo", pgh_pso_params)]
+
+
+class pgh_migr_rate(C.Structure):
+    _fields_ = [("fractional", C.c_int32), ("pad", C.c_uint32), ("count", C.c_uint64), ("fraction", C.c_double)]
+
+
+MIGRATION_P2P, MIGRATION_BROADCAST = 0, 1
+MIGRANTS_PRESERVE, MIGRANTS_EVICT = 0, 1
+
+
+def migr_rate(r) -> pgh_migr_rate:
+    """A select_best / fair_replace rate as pagmo's constructors take it: an
+    int is an absolute count, a float a fraction of the population."""
+    if isinstance(r, (float, np.floating)):
+        return pgh_migr_rate(1, 0, 0, float(r))
+    return pgh_migr_rate(0, 0, int(r), 0.0)
+
+
+def ring_connections(n: int, w: float = 1.0):
+    """topology::get_connections of ring(n, w) for every island: the edges
+    ring::push_back adds and removes (src/topologies/ring.cpp:61-100), each
+    island's in-edges in the order they were added (base_bgl_topology's
+    bidirectional vecS graph, topology.hpp:109-115).  A list of
+    [(source, weight), ...] per island."""
+    inbound = []
+    for size in range(1, n + 1):
+        inbound.append([])
+        if size == 2:
+            edges_add, edges_rm = [(0, 1), (1, 0)], []
+        elif size == 3:
+            edges_add, edges_rm = [(1, 2), (2, 1), (2, 0), (0, 2)], []
+        elif size > 3:
+            edges_rm = [(size - 2, 0), (0, size - 2)]
+            edges_add = [(size - 2, size - 1), (size - 1, size - 2), (0, size - 1), (size - 1, 0)]
+        else:
+            edges_add, edges_rm = [], []
+        for a, b in edges_rm:
+            inbound[b].remove(a)
+        for a, b in edges_add:
+            inbound[b].append(a)
+    return [[(a, w) for a in c] for c in inbound]
+
+
+def fully_connected_connections(n: int, w: float = 1.0):
+    """fully_connected(n, w): every other island in ascending order
+    (src/topologies/fully_connected.cpp:86-116)."""
+    return [[(j, w) for j in range(n) if j != i] for i in range(n)]
 
 
 def pso_params(gen=1, omega=0.7298, eta1=2.05, eta2=2.05, max_vel=0.5, variant=5, neighb_type=2, neighb_param=4,
@@ -115,6 +162,9 @@ def _declare(L):
         "pgh_archi_get": [vp, u64, vp, vp],
         "pgh_archi_set_trace": [vp, ci],
         "pgh_archi_get_trace": [vp, u64, vp],
+        "pgh_archi_set_migration": [vp, ci, ci, vp, vp, vp, C.POINTER(pgh_migr_rate), C.POINTER(pgh_migr_rate),
+                                    C.POINTER(u32)],
+        "pgh_archi_get_migrants": [vp, u64, C.POINTER(u64), vp, vp],
         "pgh_pso_gen_evolve_host": [udp_p, C.POINTER(pgh_pso_params), vp, vp, u64, u64],
     }
     for name, args in sigs.items():
@@ -422,6 +472,42 @@ class Archipelago:
         T = np.empty((self.gens[k], pop))
         check(lib().pgh_archi_get_trace(self.h, k, T.ctypes.data))
         return T
+
+    def set_migration(self, conns, select_rate=1, replace_rate=1, seeds=None, mtype=MIGRATION_P2P,
+                      handling=MIGRANTS_PRESERVE):
+        """conns: per island, [(source, weight), ...] in get_connections
+        order (ring_connections, fully_connected_connections), or None to turn
+        migration off.  Rates: one for all islands or one per island (int:
+        absolute, float: fraction); seeds: the migration engines' seeds."""
+        n = len(self.shapes)
+        if conns is None:
+            check(lib().pgh_archi_set_migration(self.h, mtype, handling, None, None, None, None, None, None))
+            return
+        off = np.zeros(n + 1, np.uint64)
+        src, wt = [], []
+        for k, c in enumerate(conns):
+            off[k + 1] = off[k] + len(c)
+            src += [a for a, _ in c]
+            wt += [w for _, w in c]
+        src = np.array(src + [0], np.uint64)
+        wt = np.array(wt + [0.0], np.float64)
+        per = lambda r: list(r) if isinstance(r, (list, tuple)) else [r] * n
+        sel = (pgh_migr_rate * n)(*[migr_rate(r) for r in per(select_rate)])
+        rep = (pgh_migr_rate * n)(*[migr_rate(r) for r in per(replace_rate)])
+        sd = (C.c_uint32 * n)(*(seeds if seeds is not None else range(n)))
+        self._migr_keep = [off, src, wt]
+        check(lib().pgh_archi_set_migration(self.h, mtype, handling, off.ctypes.data, src.ctypes.data,
+                                            wt.ctypes.data, sel, rep, sd))
+
+    def migrants(self, k):
+        """Island k's entry of the migrants database: (X, F), best first."""
+        _, nx = self.shapes[k]
+        cnt = C.c_uint64()
+        check(lib().pgh_archi_get_migrants(self.h, k, C.byref(cnt), None, None))
+        X = np.empty((cnt.value, nx))
+        F = np.empty(cnt.value)
+        check(lib().pgh_archi_get_migrants(self.h, k, C.byref(cnt), X.ctypes.data, F.ctypes.data))
+        return X, F
 
 
 def _mat(objs):

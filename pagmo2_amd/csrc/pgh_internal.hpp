@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "pagmo_hip/pgh.h"
 
@@ -72,6 +73,25 @@ pgh_status launch_local_descent(const prob &p, double *x, uint64_t n, double *f,
 // ---- rng (pso.hip) ------------------------------------------------------------
 pgh_status launch_mt19937(uint32_t seed, uint64_t skip, uint64_t n, uint32_t *out, hipStream_t s);
 pgh_status launch_batch_random_dv(const prob &p, uint64_t n, uint32_t seed, double *out, hipStream_t s);
+
+// ---- migration between archipelago islands (migration.hip) ------------------
+struct migr_island {
+    uint32_t nx, pop;
+    uint64_t xoff, foff, eoff; // the island's rows, fitness and first element in the archipelago's arrays
+};
+struct migr_ctx;
+pgh_status migr_create(const std::vector<migr_island> &isl, int type, int handling, const uint64_t *conn_off,
+                       const uint64_t *conn_src, const double *conn_w, const pgh_migr_rate *sel,
+                       const pgh_migr_rate *rep, const uint32_t *seeds, migr_ctx **out);
+void migr_destroy(migr_ctx *m);
+// Immigration of one round from the populations in (sX, sF) into (dX, dF);
+// *moved is false (and nothing is launched) when no island replaces.
+pgh_status migr_immigrate(migr_ctx *m, const double *sX, const double *sF, double *dX, double *dF, bool *moved,
+                          hipStream_t s);
+// Emigration of one round from the populations in (X, F).
+pgh_status migr_emigrate(migr_ctx *m, const double *X, const double *F, hipStream_t s);
+// Island k's database entry to host memory (the caller synchronises first).
+pgh_status migr_get(migr_ctx *m, uint64_t k, uint64_t *count, double *X, double *F);
 
 // ---- device scratch owned by the calling host thread ---------------------
 struct dev_buf {

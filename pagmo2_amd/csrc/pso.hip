@@ -355,10 +355,12 @@ struct pgh_archi {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool graph_draw_v = false, graph_from_lb = false;
+    migr_ctx *migr = nullptr; // set by pgh_archi_set_migration
 
     ~pgh_archi()
     {
         if (last) (void)hipStreamSynchronize(last);
+        if (migr) migr_destroy(migr);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (own) (void)hipStreamDestroy(own);
@@ -597,8 +599,23 @@ pgh_status pgh_archi_evolve(pgh_archi *a, int mode, void *stream)
     // velocities are re-drawn unless memory is set and they exist
     // (pso_gen.cpp:193-200); islands share the memory flag of island 0
     const bool draw_v = !(a->memory[0] && a->v_valid);
-    const bool from_lb = a->evolved;
+    bool from_lb = a->evolved;
     const uint32_t n = static_cast<uint32_t>(a->h_desc.size());
+    auto *X = static_cast<double *>(a->X.ptr);
+    auto *fit = static_cast<double *>(a->fit.ptr);
+    auto *lbX = static_cast<double *>(a->lbX.ptr);
+    auto *lbfit = static_cast<double *>(a->lbfit.ptr);
+    if (a->migr) {
+        // immigration (island.cpp:452-597) out of the buffers that hold the
+        // population into the other pair, from which the evolve call starts
+        bool moved = false;
+        if (a->evolved) {
+            PGH_TRY(migr_immigrate(a->migr, lbX, lbfit, X, fit, &moved, s));
+        } else {
+            PGH_TRY(migr_immigrate(a->migr, X, fit, lbX, lbfit, &moved, s));
+        }
+        if (moved) from_lb = !a->evolved;
+    }
     if (mode == PGH_ARCHI_PER_ISLAND) {
         for (uint32_t k = 0; k < n; ++k) PGH_TRY(launch_range(a, k, k + 1u, draw_v, from_lb, s));
     } else if (mode == PGH_ARCHI_BATCHED) {
@@ -625,10 +642,49 @@ pgh_status pgh_archi_evolve(pgh_archi *a, int mode, void *stream)
     } else {
         return fail(PGH_INVALID_ARGUMENT, "unknown archipelago mode " + std::to_string(mode));
     }
+    // emigration (island.cpp:599-620) from the evolved populations
+    if (a->migr) PGH_TRY(migr_emigrate(a->migr, lbX, lbfit, s));
     a->v_valid = true;
     a->evolved = true;
     a->last = s;
     return PGH_OK;
+}
+
+pgh_status pgh_archi_set_migration(pgh_archi *a, int type, int handling, const uint64_t *conn_offsets,
+                                   const uint64_t *conn_src, const double *conn_weight,
+                                   const pgh_migr_rate *select_rate, const pgh_migr_rate *replace_rate,
+                                   const uint32_t *migr_seeds)
+{
+    if (!a) return fail(PGH_INVALID_ARGUMENT, "null archipelago");
+    if (a->last) PGH_HIP(hipStreamSynchronize(a->last));
+    if (!conn_offsets) {
+        if (a->migr) migr_destroy(a->migr);
+        a->migr = nullptr;
+        return PGH_OK;
+    }
+    std::vector<migr_island> isl;
+    for (size_t k = 0; k < a->h_desc.size(); ++k) {
+        const island_desc &di = a->h_desc[k];
+        if (a->memory[k]) return fail(PGH_UNSUPPORTED, "migration with pso_gen memory is not supported");
+        isl.push_back(migr_island{di.nx, di.pop, di.xoff, di.foff, di.eoff});
+    }
+    migr_ctx *m = nullptr;
+    PGH_TRY(migr_create(isl, type, handling, conn_offsets, conn_src, conn_weight, select_rate, replace_rate,
+                        migr_seeds, &m));
+    if (a->migr) migr_destroy(a->migr);
+    a->migr = m;
+    return PGH_OK;
+}
+
+pgh_status pgh_archi_get_migrants(pgh_archi *a, uint64_t k, uint64_t *count, double *X, double *F)
+{
+    if (!a || k >= a->h_desc.size()) return fail(PGH_INVALID_ARGUMENT, "no such island");
+    if (!a->migr) {
+        if (count) *count = 0u;
+        return PGH_OK;
+    }
+    if (a->last) PGH_HIP(hipStreamSynchronize(a->last));
+    return migr_get(a->migr, k, count, X, F);
 }
 
 pgh_status pgh_archi_get(pgh_archi *a, uint64_t k, double *X, double *F)
