@@ -75,6 +75,91 @@ __global__ __launch_bounds__(kBlock, sep_min_waves(KIND)) void k_separable(const
     if (ind < n && r == 0) fvs[ind] = f;
 }
 
+// Rastrigin fitness and gradient in one pass over 16-byte aligned rows: the
+// fitness is k_separable's (the same routine and reduction, hence the same
+// bits as a bfe call), the gradient is written by the lanes as they add
+// their pairs.  A row with an element beyond the streaming routines' range
+// gets its fitness and its gradient again from the general routines, out of
+// line.
+template <int L>
+__global__ __launch_bounds__(kBlock, sep_min_waves(RASTRIGIN)) void k_rastrigin_fg(const double *__restrict__ dvs,
+                                                                                   uint64_t n, uint32_t nx,
+                                                                                   double *__restrict__ fvs,
+                                                                                   double *__restrict__ grad)
+{
+    constexpr uint32_t G = kBlock / L;
+    const uint64_t ind = static_cast<uint64_t>(blockIdx.x) * G + threadIdx.x / L;
+    const int r = static_cast<int>(threadIdx.x % L);
+    // a trailing partial group evaluates the last row again, writing the same
+    // gradient values to it, and drops its fitness
+    const uint64_t row = ind < n ? ind : n - 1u;
+    double f = separable_eval<RASTRIGIN, L, true, true>(dvs + row * nx, nx, r, grad + row * nx);
+    if (is_recompute_mark(f)) {
+        // group-uniform: every lane of the group holds the mark
+        rastrigin_grad_row(dvs + row * nx, nx, static_cast<uint32_t>(r), L, grad + row * nx);
+        if (r == 0) f = separable_cold<RASTRIGIN, true>(dvs + row * nx, nx);
+    }
+    if (ind < n && r == 0 && fvs) fvs[ind] = f;
+}
+
+// ---- rosenbrock over rows of 64 and more: 8 rows per wavefront ------------
+// A wavefront takes 8 rows through column chunks of 64: for each row the
+// lanes read 64 consecutive elements (contiguous loads), and compute in
+// parallel each element's term of the sum and, with GRAD, its gradient
+// (stored contiguously).  The terms go to the wavefront's LDS tile, where
+// lane r (r < 8) then adds row r's terms in the reference's order
+// (rosenbrock.cpp:62-64), so the sum is the reference's bit for bit.  Each
+// wavefront uses its own tile: no workgroup barrier.
+constexpr uint32_t kRWRows = 8, kRWStride = 65; // odd stride: the 8 summing lanes hit distinct banks
+
+template <bool GRAD>
+__global__ __launch_bounds__(kBlock) void k_rosen_wave(const double *__restrict__ dvs, uint64_t n, uint32_t nx,
+                                                       double *__restrict__ fvs, double *__restrict__ grad)
+{
+    constexpr uint32_t W = 64, kWaves = kBlock / W;
+    __shared__ double terms[kWaves * kRWRows * kRWStride];
+    const uint32_t wave = threadIdx.x / W, lane = threadIdx.x % W;
+    const uint64_t row0 = (static_cast<uint64_t>(blockIdx.x) * kWaves + wave) * kRWRows;
+    if (row0 >= n) return; // wavefront-uniform
+    const uint32_t rows = static_cast<uint32_t>(min(static_cast<uint64_t>(kRWRows), n - row0));
+    double *tile = terms + wave * kRWRows * kRWStride;
+    double acc = 0.;
+    for (uint32_t c0 = 0; c0 < nx; c0 += W) {
+        const uint32_t i = c0 + lane;
+        // Clamped indices: lanes past the row's end take element nx - 1, and
+        // rows past the batch the last row, so that every load and store
+        // stays in the batch with no branch; such a lane stores the very
+        // value that the element's own lane stores.  Its term is not summed.
+        const uint32_t ii = min(i, nx - 1u), in = min(ii + 1u, nx - 1u), im = ii == 0u ? 0u : ii - 1u;
+        double xm[kRWRows], xi[kRWRows], xn[kRWRows];
+#pragma unroll
+        for (uint32_t r = 0; r < kRWRows; ++r) {
+            const double *x = dvs + (row0 + min(r, rows - 1u)) * nx;
+            xi[r] = x[ii];
+            xn[r] = x[in];
+            if constexpr (GRAD) xm[r] = x[im];
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kRWRows; ++r) {
+            const double d = xi[r] * xi[r] - xn[r];
+            const double term = 100. * d * d + (xi[r] - 1) * (xi[r] - 1);
+            tile[r * kRWStride + lane] = term;
+            if constexpr (GRAD) {
+                grad[(row0 + min(r, rows - 1u)) * nx + ii] = rosenbrock_grad3(xm[r], xi[r], xn[r], nx, ii);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        // terms i = c0 .. c0 + cnt - 1 of row `lane`, in order
+        const uint32_t cnt = c0 + 1u < nx ? min(W, nx - 1u - c0) : 0u;
+        if (lane < rows) {
+            const double *t = tile + lane * kRWStride;
+            for (uint32_t j = 0; j < cnt; ++j) acc += t[j];
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (lane < rows && fvs) fvs[row0 + lane] = acc;
+}
+
 // ---- rosenbrock: one lane per individual ------------------------------------
 __global__ __launch_bounds__(kBlock) void k_rosenbrock(const double *__restrict__ dvs, uint64_t n, uint32_t nx,
                                                        double *__restrict__ fvs)
@@ -267,16 +352,6 @@ __global__ __launch_bounds__(kBlock) void k_lj_grad(const double *__restrict__ d
 }
 
 // ---- elementwise gradients ---------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_rosenbrock_grad(const double *__restrict__ dvs, uint64_t n, uint32_t nx,
-                                                            double *__restrict__ grad)
-{
-    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    if (t >= n * nx) return;
-    const uint64_t ind = t / nx;
-    const uint32_t i = static_cast<uint32_t>(t - ind * nx);
-    grad[t] = rosenbrock_grad(dvs + ind * nx, nx, i);
-}
-
 __global__ __launch_bounds__(kBlock) void k_rastrigin_grad(const double *__restrict__ dvs, uint64_t count,
                                                            double *__restrict__ grad)
 {
@@ -314,6 +389,39 @@ pgh_status launch_sep(const double *dvs, uint64_t n, uint32_t nx, double *fvs, h
         default:
             return launch_sep_l<KIND, 4>(dvs, n, nx, fvs, s);
     }
+}
+
+template <int L>
+pgh_status launch_rastrigin_fg_l(const double *dvs, uint64_t n, uint32_t nx, double *fvs, double *grad, hipStream_t s)
+{
+    constexpr uint64_t G = kBlock / L;
+    k_rastrigin_fg<L><<<dim3(static_cast<uint32_t>((n + G - 1u) / G)), dim3(kBlock), 0, s>>>(dvs, n, nx, fvs, grad);
+    PGH_HIP(hipGetLastError());
+    return PGH_OK;
+}
+
+pgh_status launch_rastrigin_fg(const double *dvs, uint64_t n, uint32_t nx, double *fvs, double *grad, hipStream_t s)
+{
+    switch (sep_lanes(RASTRIGIN, nx)) {
+        case 64:
+            return launch_rastrigin_fg_l<64>(dvs, n, nx, fvs, grad, s);
+        case 32:
+            return launch_rastrigin_fg_l<32>(dvs, n, nx, fvs, grad, s);
+        case 16:
+            return launch_rastrigin_fg_l<16>(dvs, n, nx, fvs, grad, s);
+        default:
+            return launch_rastrigin_fg_l<4>(dvs, n, nx, fvs, grad, s);
+    }
+}
+
+pgh_status launch_rosen_rows(const double *dvs, uint64_t n, uint32_t nx, double *fvs, double *grad, hipStream_t s)
+{
+    constexpr uint64_t rows_per_block = kBlock / 64u * kRWRows;
+    const dim3 grid(static_cast<uint32_t>((n + rows_per_block - 1u) / rows_per_block));
+    if (grad) k_rosen_wave<true><<<grid, dim3(kBlock), 0, s>>>(dvs, n, nx, fvs, grad);
+    else k_rosen_wave<false><<<grid, dim3(kBlock), 0, s>>>(dvs, n, nx, fvs, nullptr);
+    PGH_HIP(hipGetLastError());
+    return PGH_OK;
 }
 
 template <int L, uint32_t APL>
@@ -416,6 +524,7 @@ pgh_status launch_fitness(const prob &p, const double *dvs, uint64_t n, double *
     switch (p.kind) {
         case ROSENBROCK:
             if (use_staged(p, dvs)) return launch_rows_staged<ROSENBROCK>(p, dvs, n, fvs, s);
+            if (p.nx >= 2u) return launch_rosen_rows(dvs, n, p.nx, fvs, nullptr, s);
             k_rosenbrock<<<dim3(blocks_for(n)), dim3(kBlock), 0, s>>>(dvs, n, p.nx, fvs);
             PGH_HIP(hipGetLastError());
             return PGH_OK;
@@ -454,10 +563,13 @@ pgh_status launch_gradient(const prob &p, const double *dvs, uint64_t n, double 
     if (check_launch_size(n, p.nx) != PGH_OK) return PGH_OVERFLOW;
     switch (p.kind) {
         case ROSENBROCK:
-            k_rosenbrock_grad<<<dim3(blocks_for(n * p.nx)), dim3(kBlock), 0, s>>>(dvs, n, p.nx, grad);
-            PGH_HIP(hipGetLastError());
-            return fvs ? launch_fitness(p, dvs, n, fvs, s) : PGH_OK;
+            // fitness (when asked) and gradient in one pass
+            return launch_rosen_rows(dvs, n, p.nx, fvs, grad, s);
         case RASTRIGIN:
+            if (p.nx % 2u == 0u && reinterpret_cast<uintptr_t>(dvs) % 16u == 0u &&
+                reinterpret_cast<uintptr_t>(grad) % 16u == 0u) {
+                return launch_rastrigin_fg(dvs, n, p.nx, fvs, grad, s);
+            }
             k_rastrigin_grad<<<dim3(blocks_for(n * p.nx)), dim3(kBlock), 0, s>>>(dvs, n * p.nx, grad);
             PGH_HIP(hipGetLastError());
             return fvs ? launch_fitness(p, dvs, n, fvs, s) : PGH_OK;

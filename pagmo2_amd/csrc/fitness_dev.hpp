@@ -245,10 +245,22 @@ __device__ __forceinline__ void separable_pair(sep_acc<KIND, FAST> &acc, const d
     }
 }
 
-template <int KIND, int L, bool ALIGNED, bool FAST>
-__device__ __forceinline__ double separable_eval_impl(const double *__restrict__ x, uint32_t nx, int r,
-                                                      bool &out_of_range)
+// rastrigin.cpp:103, g = 2 x + 10 omega sin(omega x) with sin_2pi_small:
+// valid for |x| < 2^10.  A row holding a larger element (or inf, NaN) is
+// flagged by the fitness pass, and its gradient is computed again with the
+// general routine.
+__device__ __forceinline__ double rastrigin_grad_fast(double xi)
 {
+    return 2 * xi + 10.0 * kOmega * sin_2pi_small(xi);
+}
+
+// GRAD (rastrigin, 16-byte aligned rows): the same pass also writes the
+// gradient of every element it adds to grad (the row's gradient).
+template <int KIND, int L, bool ALIGNED, bool FAST, bool GRAD = false>
+__device__ __forceinline__ double separable_eval_impl(const double *__restrict__ x, uint32_t nx, int r,
+                                                      bool &out_of_range, double *__restrict__ grad = nullptr)
+{
+    static_assert(!GRAD || (KIND == RASTRIGIN && ALIGNED), "the fused gradient is rastrigin's, on aligned rows");
     sep_acc<KIND, FAST> acc;
     acc.init();
     const uint32_t npairs = (nx + 1u) >> 1;
@@ -260,10 +272,12 @@ __device__ __forceinline__ double separable_eval_impl(const double *__restrict__
         // lane.  The next pair's load is issued before the current pair is
         // used.  Pairs are added in increasing c either way.
         const double2 *row = reinterpret_cast<const double2 *>(x) + c;
+        double2 *grow = GRAD ? reinterpret_cast<double2 *>(grad) + c : nullptr;
         const uint32_t full = npairs / L;
         const auto add_round = [&](const double2 &v, uint32_t j) {
             acc.add(v.x, 2u * (c + j * L));
             acc.add(v.y, 2u * (c + j * L) + 1u);
+            if constexpr (GRAD) grow[j * L] = make_double2(rastrigin_grad_fast(v.x), rastrigin_grad_fast(v.y));
         };
         if (full > 0u) {
             // two rounds per iteration with the loads alternating between a
@@ -294,6 +308,7 @@ __device__ __forceinline__ double separable_eval_impl(const double *__restrict__
             const double2 v = row[full * L];
             acc.add(v.x, 2u * (c + full * L));
             acc.add(v.y, 2u * (c + full * L) + 1u);
+            if constexpr (GRAD) grow[full * L] = make_double2(rastrigin_grad_fast(v.x), rastrigin_grad_fast(v.y));
         }
     } else {
         for (; c < npairs; c += L) separable_pair<KIND, L, ALIGNED, FAST>(acc, x, nx, c);
@@ -345,12 +360,13 @@ __device__ __forceinline__ bool is_recompute_mark(double f)
 // range (|x| >= 2^10, inf, NaN) gets the recompute mark instead, and
 // separable_eval_exact produces its value (kept out of this routine so that
 // its registers do not weigh on the streaming kernel).
-template <int KIND, int L, bool ALIGNED>
-__device__ __forceinline__ double separable_eval(const double *__restrict__ x, uint32_t nx, int r)
+template <int KIND, int L, bool ALIGNED, bool GRAD = false>
+__device__ __forceinline__ double separable_eval(const double *__restrict__ x, uint32_t nx, int r,
+                                                 double *__restrict__ grad = nullptr)
 {
     constexpr bool fast = (KIND == RASTRIGIN || KIND == ACKLEY);
     bool bad = false;
-    const double f = separable_eval_impl<KIND, L, ALIGNED, fast>(x, nx, r, bad);
+    const double f = separable_eval_impl<KIND, L, ALIGNED, fast, GRAD>(x, nx, r, bad, grad);
     if constexpr (fast) {
         if (group_max<L>(bad ? 1 : 0)) return __longlong_as_double(static_cast<long long>(kRecomputeBits));
     }
@@ -382,22 +398,29 @@ __device__ __forceinline__ double rosenbrock_eval(const double *__restrict__ x, 
     return retval;
 }
 
-// rosenbrock.cpp:83-92, element i of the gradient (m_dim >= 2).
-__device__ __forceinline__ double rosenbrock_grad(const double *__restrict__ x, uint32_t m_dim, uint32_t i)
+// rosenbrock.cpp:83-92, element i of m_dim from its neighbours xm = x[i-1]
+// (unused for i = 0), x0 = x[i] and xp = x[i+1] (unused for i = m_dim - 1).
+// Both parts are always evaluated (no branch, so that callers' loads of xm
+// and xp are not sunk into one); the middle elements add them in the
+// reference's association, (a - b) + c.
+__device__ __forceinline__ double rosenbrock_grad3(double xm, double x0, double xp, uint32_t m_dim, uint32_t i)
 {
-    if (i == 0u) {
-        return -400. * x[0] * (x[1] - x[0] * x[0]) - 2. * (1 - x[0]);
-    }
-    if (i == m_dim - 1u) {
-        return 200. * (x[m_dim - 1u] - x[m_dim - 2u] * x[m_dim - 2u]);
-    }
-    return -400. * x[i] * (x[i + 1u] - x[i] * x[i]) - 2. * (1 - x[i]) + 200. * (x[i] - x[i - 1u] * x[i - 1u]);
+    const double right = -400. * x0 * (xp - x0 * x0) - 2. * (1 - x0);
+    const double left = 200. * (x0 - xm * xm);
+    return i == 0u ? right : (i == m_dim - 1u ? left : right + left);
 }
 
 // rastrigin.cpp:103.
 __device__ __forceinline__ double rastrigin_grad(double xi)
 {
     return 2 * xi + 10.0 * kOmega * sin_any(kOmega * xi);
+}
+
+// A row's gradient elements r, r + L, ... with the general routine.
+__device__ __noinline__ void rastrigin_grad_row(const double *__restrict__ x, uint32_t nx, uint32_t r, uint32_t L,
+                                                double *__restrict__ g)
+{
+    for (uint32_t i = r; i < nx; i += L) g[i] = rastrigin_grad(x[i]);
 }
 
 // ---------------------------------------------------------------------------

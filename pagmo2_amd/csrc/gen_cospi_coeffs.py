@@ -1,9 +1,12 @@
-"""Coefficients of the cos(pi u) polynomial in trig_dev.hpp (cos_2pi_small).
+"""Coefficients of the cos(pi u) and sin(pi u) polynomials in trig_dev.hpp
+(cos_2pi_small, sin_2pi_small).
 
 cos(pi u) for |u| <= 1/2 is approximated by p(z), z = u^2 in [0, 1/4], a
 degree-8 polynomial with the smallest maximum absolute error (Remez exchange,
 50 significant digits).  The same polynomial times 10 is the rastrigin set.
-Prints the binary64 coefficients (hex) and the minimax error.
+sin(pi u) is u * q(z) with q the degree-8 minimax polynomial of
+sin(pi sqrt(z)) / sqrt(z) on the same interval.  Prints the binary64
+coefficients (hex) and the minimax errors.
 
     python pagmo2_amd/csrc/gen_cospi_coeffs.py
 """
@@ -14,11 +17,15 @@ DEGREE = 8
 A, B = mp.mpf(0), mp.mpf(1) / 4
 
 
-def target(z):
+def target_cos(z):
     return mp.cos(mp.pi * mp.sqrt(z))
 
 
-def remez(n, iters=30, grid_points=4000):
+def target_sin(z):
+    return mp.pi if z == 0 else mp.sin(mp.pi * mp.sqrt(z)) / mp.sqrt(z)
+
+
+def remez(n, target, iters=30, grid_points=4000):
     xs = [(A + B) / 2 - (B - A) / 2 * mp.cos(mp.pi * i / (n + 1)) for i in range(n + 2)]
     for _ in range(iters):
         m = mp.matrix(n + 2, n + 2)
@@ -46,7 +53,10 @@ def remez(n, iters=30, grid_points=4000):
 
 
 if __name__ == "__main__":
-    coef, level = remez(DEGREE)
+    coef, level = remez(DEGREE, target_cos)
     print(f"degree {DEGREE} in u^2, minimax absolute error {mp.nstr(abs(level), 5)}")
     print("cos(pi u):", ", ".join(float(c).hex() for c in coef))
     print("10 cos(pi u):", ", ".join(float(10 * c).hex() for c in coef))
+    coef, level = remez(DEGREE, target_sin)
+    print(f"sin(pi u) / u: degree {DEGREE} in u^2, minimax absolute error {mp.nstr(abs(level), 5)}")
+    print("sin(pi u) / u:", ", ".join(float(c).hex() for c in coef))

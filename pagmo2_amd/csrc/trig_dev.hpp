@@ -123,6 +123,32 @@ __device__ __forceinline__ double cos_2pi_small(double x)
     return __hiloint2double(static_cast<int>(p_hi), __double2loint(p));
 }
 
+// sin(2 pi x) for moderate x (|x| < 2^10), with the reduction of
+// cos_2pi_small: sin(2 pi x) = (-1)^k sin(pi u), and sin(pi u) = u q(u^2)
+// with q the degree-8 polynomial of smallest maximum error to
+// sin(pi sqrt(z)) / sqrt(z) on [0, 1/4] (gen_cospi_coeffs.py: 6.5e-19 before
+// rounding).  Compared with the reference's sin(omega * x) the result differs
+// by the rounding of omega * x there (at most |2 pi x| * 1.5e-16).
+constexpr double kSinPi[9] = {0x1.921fb54442d18p+1, -0x1.4abbce625be52p+2, 0x1.466bc6775aa7dp+1,
+                              -0x1.32d2cce627c9ep-1, 0x1.50783485523f1p-4,  -0x1.e3074dfb5b971p-8,
+                              0x1.e8f3677c2c58bp-12, -0x1.6f7ad23a648ddp-16, 0x1.9d46b0508cff6p-21};
+
+__device__ __forceinline__ double sin_2pi_small(double x)
+{
+    constexpr double kShift = 0x1.8p52;
+    const double t = fma(x, 2.0, kShift);
+    const double k = t - kShift;
+    const double u = fma(x, 2.0, -k);
+    const double z = u * u;
+    double q = kSinPi[8];
+#pragma unroll
+    for (int i = 7; i >= 0; --i) q = fma(q, z, kSinPi[i]);
+    const double p = u * q;
+    const uint32_t t_lo = static_cast<uint32_t>(__double2loint(t));
+    const uint32_t p_hi = static_cast<uint32_t>(__double2hiint(p)) + (t_lo << 31);
+    return __hiloint2double(static_cast<int>(p_hi), __double2loint(p));
+}
+
 // Any argument: the reduction above when it is accurate, the math library's
 // routine otherwise (|y| >= 2^20, inf, NaN).
 __device__ __forceinline__ double cos_any(double y)

@@ -53,7 +53,7 @@ def lj_abs_scale(x):
 
 
 @pytest.mark.parametrize("n", [1, 3, 64, 257])
-@pytest.mark.parametrize("dim", [2, 3, 10, 37, 500])
+@pytest.mark.parametrize("dim", [2, 3, 10, 37, 63, 64, 65, 500, 1000])
 def test_rosenbrock_bit_exact(n, dim):
     x = inputs(P.ROSENBROCK, dim, n, 100 + dim)
     assert np.array_equal(gpu_bfe(P.ROSENBROCK, dim, x), O.bfe(P.ROSENBROCK, dim, x, threads=8))
@@ -164,7 +164,7 @@ def test_host_entry_point_matches_device():
 
 # ---- gradients ---------------------------------------------------------------
 
-@pytest.mark.parametrize("dim,n", [(2, 3), (10, 64), (501, 17)])
+@pytest.mark.parametrize("dim,n", [(2, 3), (10, 64), (501, 17), (64, 257), (1000, 300), (1001, 513)])
 def test_rosenbrock_gradient_bit_exact(dim, n):
     x = inputs(P.ROSENBROCK, dim, n, dim)
     g = torch.empty((n, dim), dtype=torch.float64, device=DEV)
@@ -175,14 +175,30 @@ def test_rosenbrock_gradient_bit_exact(dim, n):
     assert np.array_equal(f.cpu().numpy(), O.bfe(P.ROSENBROCK, dim, x))
 
 
-def test_rastrigin_gradient():
-    x = inputs(P.RASTRIGIN, 300, 40, 2)
-    g = torch.empty((40, 300), dtype=torch.float64, device=DEV)
-    P.batch_gradient(P.RASTRIGIN, 300, dev(x), g)
+@pytest.mark.parametrize("dim,n", [(300, 40), (1000, 257), (301, 33), (12, 1000), (2, 5)])
+def test_rastrigin_gradient(dim, n):
+    # even dimensions take the fused fitness + gradient pass, odd ones the
+    # element-wise gradient kernel
+    x = inputs(P.RASTRIGIN, dim, n, 2 + dim)
+    if n > 200:
+        # rows that leave the streaming routines' range (|x| >= 2^10, inf,
+        # NaN) are recomputed with the general routines
+        x[3, dim // 2] = 3000.5
+        x[7, 0] = -1e9
+        x[11, dim - 1] = np.inf
+        x[13, 1] = np.nan
+    g = torch.empty((n, dim), dtype=torch.float64, device=DEV)
+    f = torch.empty((n, 1), dtype=torch.float64, device=DEV)
+    P.batch_gradient(P.RASTRIGIN, dim, dev(x), g, f)
     torch.cuda.synchronize()
-    ref = O.gradient(P.RASTRIGIN, 300, x)
+    ref = O.gradient(P.RASTRIGIN, dim, x)
+    got = g.cpu().numpy()
+    fin = np.isfinite(x)
     scale = np.abs(2 * x) + 10 * 2 * np.pi
-    assert np.all(np.abs(g.cpu().numpy() - ref) <= 1e-13 * scale)
+    assert np.all(np.abs(got[fin] - ref[fin]) <= 1e-13 * scale[fin])
+    assert np.array_equal(np.isnan(got), np.isnan(ref))
+    # the fitness equals a bfe call's bit for bit
+    assert np.array_equal(f.cpu().numpy(), gpu_bfe(P.RASTRIGIN, dim, x), equal_nan=True)
 
 
 @pytest.mark.parametrize("atoms,n", [(3, 4), (7, 20), (48, 64), (120, 5)])
