@@ -160,15 +160,6 @@ __global__ __launch_bounds__(kBlock) void k_rosen_wave(const double *__restrict_
     if (lane < rows && fvs) fvs[row0 + lane] = acc;
 }
 
-// ---- rosenbrock: one lane per individual ------------------------------------
-__global__ __launch_bounds__(kBlock) void k_rosenbrock(const double *__restrict__ dvs, uint64_t n, uint32_t nx,
-                                                       double *__restrict__ fvs)
-{
-    const uint64_t ind = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    if (ind >= n) return;
-    fvs[ind] = rosenbrock_eval(dvs + ind * nx, nx);
-}
-
 // ---- multi-objective sphere: one lane per individual ------------------------
 __global__ __launch_bounds__(kBlock) void k_mo_sphere(const double *__restrict__ dvs, uint64_t n, uint32_t nx,
                                                       uint32_t nobj, const double *__restrict__ centers,
@@ -524,10 +515,7 @@ pgh_status launch_fitness(const prob &p, const double *dvs, uint64_t n, double *
     switch (p.kind) {
         case ROSENBROCK:
             if (use_staged(p, dvs)) return launch_rows_staged<ROSENBROCK>(p, dvs, n, fvs, s);
-            if (p.nx >= 2u) return launch_rosen_rows(dvs, n, p.nx, fvs, nullptr, s);
-            k_rosenbrock<<<dim3(blocks_for(n)), dim3(kBlock), 0, s>>>(dvs, n, p.nx, fvs);
-            PGH_HIP(hipGetLastError());
-            return PGH_OK;
+            return launch_rosen_rows(dvs, n, p.nx, fvs, nullptr, s);
         case RASTRIGIN:
             return launch_sep<RASTRIGIN>(dvs, n, p.nx, fvs, s);
         case ACKLEY:
