@@ -74,6 +74,19 @@ pgh_status launch_local_descent(const prob &p, double *x, uint64_t n, double *f,
 pgh_status launch_mt19937(uint32_t seed, uint64_t skip, uint64_t n, uint32_t *out, hipStream_t s);
 pgh_status launch_batch_random_dv(const prob &p, uint64_t n, uint32_t seed, double *out, hipStream_t s);
 
+// ---- std::mt19937 jump-ahead (mt_jump.hip) ------------------------------------
+// Device table of the coefficient lists of z^d mod phi for each distance d
+// (layout: mt_dev.hpp, kMtIdxStride); made on first use per device and kept.
+// Synchronous the first time.
+pgh_status jump_table_device(const std::vector<uint64_t> &dists, const void **out);
+// The same table as bytes, for a caller that keeps its own device copy.
+pgh_status jump_table_bytes(const std::vector<uint64_t> &dists, std::vector<uint8_t> &out);
+// Bytes of the table's header (one 32-bit length per distance, 16-byte padded).
+constexpr size_t jump_table_head(size_t n_dists)
+{
+    return (n_dists * 4u + 15u) / 16u * 16u;
+}
+
 // ---- migration between archipelago islands (migration.hip) ------------------
 struct migr_island {
     uint32_t nx, pop;

@@ -2,11 +2,14 @@
 // of independent islands of uneven size, on the device.
 //
 // Per island the device holds, contiguously: X, V, lbX (pop x nx), fit, lbfit
-// (pop), the island's std::mt19937 state and a word buffer for one generation
-// (pop x nx x 4 words: r1 and r2 of pso_gen.cpp:300-301 in draw order).
-// A generation is three launches:
-//   k_rng     one workgroup per island hands out the generation's words,
-//             exactly the draws of the reference's sequential loop
+// (pop), the island's std::mt19937 state and a word buffer for a batch of
+// kGenBatch generations (pop x nx x 4 words each: r1 and r2 of
+// pso_gen.cpp:300-301 in draw order).  A generation is up to three launches:
+//   k_rng     (every kGenBatch generations) the island's generator split in
+//             segments, one workgroup each, each handing out whole
+//             generations of the batch and reaching its own by jump-ahead
+//             (mt_dev.hpp, mt_jump): exactly the draws of the reference's
+//             sequential loop
 //   k_update  one lane per (island, particle, dimension): best neighbour
 //             (pso_gen.cpp:593-622), velocity (variant 5, :302-305), clamp and
 //             position (:336-361) -- arithmetic in the reference's order
@@ -42,6 +45,17 @@ struct island_desc {
     uint64_t woff; // word buffer offset (u32)
     uint64_t toff; // trace offset (doubles), gens x pop
     uint64_t eoff; // cumulative pop*nx of the islands before this one
+    // the generator split in `segs` segments (k_rng): of every batch of
+    // kGenBatch generations, segment s hands out the words of generations
+    // s gps .. (s + 1) gps - 1; seg0 is the first segment's index; j_* index
+    // the archipelago's jump table (z^n0, z^(gps W 2^b) for b = 0..,
+    // z^((segs - 1) gps W))
+    uint32_t segs, seg0, gps;
+    uint32_t j_n0, j_tree, j_main;
+};
+
+struct seg_desc {
+    uint32_t island, s;
 };
 
 struct tile {
@@ -52,6 +66,11 @@ namespace
 {
 
 constexpr uint32_t kBlock = 256;
+// k_rng hands out the words of kGenBatch generations at a time; an island's
+// generator is split in segments (k_rng) so that each hands out at most
+// kSegWords words of a batch, or a single generation's
+constexpr uint32_t kGenBatch = 32;
+constexpr uint64_t kSegWords = uint64_t(1) << 20;
 
 // "a <= b" in the sense of less_than_f(a, b) || equal_to_f(a, b)
 // (custom_comparisons.hpp:54-100): NaN is larger than everything but itself.
@@ -76,17 +95,89 @@ __device__ __forceinline__ uint32_t find_island(const island_desc *__restrict__ 
     return lo;
 }
 
-// phase 0: the velocity draws of the evolve call (2 words per element,
-// pso_gen.cpp:193-200); phase 1: a generation (4 words per element).
-__global__ __launch_bounds__(kBlock) void k_rng(const island_desc *__restrict__ d, mt_state *__restrict__ st,
-                                                uint32_t *__restrict__ words, uint32_t i0, uint32_t phase, uint32_t gen)
+struct jump_tab {
+    const uint32_t *lens;
+    const uint16_t *lists; // kMtIdxStride values per distance
+};
+
+__device__ __forceinline__ void load_state(const mt_state *__restrict__ src, uint32_t *ring, uint64_t &K, uint64_t &C)
+{
+    for (uint32_t i = threadIdx.x; i < kMtRing; i += blockDim.x) ring[i] = src->ring[i];
+    K = src->K;
+    C = src->C;
+    __syncthreads();
+}
+
+__device__ __forceinline__ void store_state(mt_state *__restrict__ dst, const uint32_t *ring, uint64_t K, uint64_t C)
+{
+    for (uint32_t i = threadIdx.x; i < kMtRing; i += blockDim.x) dst->ring[i] = ring[i];
+    if (threadIdx.x == 0) {
+        dst->K = K;
+        dst->C = C;
+    }
+}
+
+// The island's std::mt19937 draws, by its segments (one workgroup each).
+// phase 0, the start of an evolve call: every segment takes the island's
+// canonical generator (everything handed out so far); segment 0 hands out
+// the velocity draws (2 words per element, pso_gen.cpp:193-200) when
+// draw_v, and every segment moves to the start of its first generation, s gps
+// (jumps z^n0 and z^(gps W 2^b) over the bits of s).  phase 1, the batch of
+// generations g0 .. g0 + kGenBatch - 1 (4 words per element and generation,
+// r1 and r2 of pso_gen.cpp:300-301): segment s hands out the words of its
+// gps generations (those before the island's last) into their slots of the
+// word buffer, then jumps over the other segments' generations to its share
+// of the next batch.  The segment that hands out the island's last
+// generation stops there and becomes the canonical generator.  The words are
+// those of one sequential generator, in order.
+__global__ __launch_bounds__(kMtJumpBlock) void k_rng(const island_desc *__restrict__ d,
+                                                      const seg_desc *__restrict__ sd, mt_state *__restrict__ st,
+                                                      mt_state *__restrict__ segst, jump_tab jt,
+                                                      uint32_t *__restrict__ words, uint32_t sg0, uint32_t phase,
+                                                      uint32_t g0, uint32_t draw_v)
 {
     __shared__ uint32_t ring[kMtRing];
-    const uint32_t k = i0 + blockIdx.x;
-    const island_desc di = d[k];
-    if (phase == 1u && gen >= di.gens) return;
-    const uint64_t count = static_cast<uint64_t>(di.pop) * di.nx * (phase == 0u ? 2u : 4u);
-    mt_generate(&st[k], ring, words + di.woff, count);
+    __shared__ uint4 lds_idx[kMtIdxStride / 8u];
+    const seg_desc sg = sd[sg0 + blockIdx.x];
+    const island_desc di = d[sg.island];
+    const uint64_t elems = static_cast<uint64_t>(di.pop) * di.nx;
+    const uint64_t W = 4u * elems;
+    const auto jump = [&](uint64_t &K, uint64_t &C, uint64_t J, uint32_t j) {
+        mt_jump(ring, K, C, J, jt.lists + static_cast<uint64_t>(j) * kMtIdxStride, jt.lens[j], lds_idx);
+    };
+    uint64_t K, C;
+    mt_state *mine = &segst[di.seg0 + sg.s];
+    if (phase == 0u) {
+        load_state(&st[sg.island], ring, K, C);
+        const uint64_t n0 = draw_v ? 2u * elems : 0u;
+        if (sg.s == 0u) {
+            if (n0) {
+                mt_run(ring, K, C, n0, words + di.woff);
+                C += n0;
+            }
+            if (di.gens == 0u) store_state(&st[sg.island], ring, K, C);
+        } else {
+            if (n0) jump(K, C, n0, di.j_n0);
+            for (uint32_t b = 0; (sg.s >> b) != 0u; ++b) {
+                if ((sg.s >> b) & 1u) jump(K, C, (di.gps * W) << b, di.j_tree + b);
+            }
+        }
+        store_state(mine, ring, K, C);
+        return;
+    }
+    const uint32_t first = g0 + sg.s * di.gps;
+    if (first >= di.gens) return; // no generation of this batch, nor of a later one
+    const uint32_t last = min(first + di.gps, di.gens); // one past
+    load_state(mine, ring, K, C);
+    const uint64_t n = (last - first) * W;
+    mt_run(ring, K, C, n, words + di.woff + static_cast<uint64_t>(first - g0) * W);
+    C += n;
+    if (last == di.gens) {
+        store_state(&st[sg.island], ring, K, C);
+    } else if (first + kGenBatch < di.gens) {
+        if (di.segs > 1u) jump(K, C, static_cast<uint64_t>(di.segs - 1u) * di.gps * W, di.j_main);
+        store_state(mine, ring, K, C);
+    }
 }
 
 // Start of an evolve call (pso_gen.cpp:183-200): V = U(minv, maxv) per
@@ -151,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const island_desc *__restrict
         const uint32_t q = p + j >= S ? p + j - S : p + j;
         if (le_f(lbf[q], lbf[bn])) bn = q;
     }
-    const uint32_t *w = words + di.woff + 4u * m;
+    const uint32_t *w = words + di.woff + static_cast<uint64_t>(gen % kGenBatch) * (4ull * di.pop * di.nx) + 4u * m;
     const double r1 = mt_canonical(w[0], w[1]);
     const double r2 = mt_canonical(w[2], w[3]);
     const uint64_t xi = di.xoff + m;
@@ -349,7 +440,9 @@ struct pgh_archi {
     bool v_valid = false;
     bool evolved = false; // the population lives in lbX/lbfit once evolved
     std::vector<uint32_t> memory;
-    dev_buf desc, tiles, X, V, lbX, fit, lbfit, words, states, trace_buf;
+    dev_buf desc, tiles, X, V, lbX, fit, lbfit, words, states, trace_buf, segs, seg_states, jump_table;
+    uint32_t n_segs = 0;
+    jump_tab jt{nullptr, nullptr};
     hipStream_t own = nullptr;
     hipStream_t last = nullptr;
     hipGraph_t graph = nullptr;
@@ -402,18 +495,20 @@ pgh_status launch_range(pgh_archi *a, uint32_t i0, uint32_t i1, bool draw_v, boo
     const uint64_t e1 = i1 < a->h_desc.size() ? a->h_desc[i1].eoff : a->total_elems;
     const uint64_t total = e1 - e0;
     const uint32_t t0 = a->tile_begin[i0], nt = a->tile_begin[i1] - t0;
-    const uint32_t nisl = i1 - i0;
     uint32_t gens = 0;
     for (uint32_t k = i0; k < i1; ++k) gens = std::max(gens, a->h_desc[k].gens);
-    if (draw_v) {
-        k_rng<<<nisl, kBlock, 0, s>>>(d, st, w, i0, 0u, 0u);
-        PGH_HIP(hipGetLastError());
-    }
+    const uint32_t sg0 = a->h_desc[i0].seg0;
+    const uint32_t ng = (i1 < a->h_desc.size() ? a->h_desc[i1].seg0 : a->n_segs) - sg0;
+    const auto *sd = static_cast<const seg_desc *>(a->segs.ptr);
+    auto *segst = static_cast<mt_state *>(a->seg_states.ptr);
+    const jump_tab jt = a->jt;
+    k_rng<<<ng, kMtJumpBlock, 0, s>>>(d, sd, st, segst, jt, w, sg0, 0u, 0u, draw_v ? 1u : 0u);
+    PGH_HIP(hipGetLastError());
     k_vinit<<<blocks_for(total), kBlock, 0, s>>>(d, i0, i1, e0, total, w, X, V, lbX, fit, lbfit, draw_v ? 1u : 0u,
                                                  from_lb ? 1u : 0u);
     PGH_HIP(hipGetLastError());
     for (uint32_t g = 0; g < gens; ++g) {
-        k_rng<<<nisl, kBlock, 0, s>>>(d, st, w, i0, 1u, g);
+        if (g % kGenBatch == 0u) k_rng<<<ng, kMtJumpBlock, 0, s>>>(d, sd, st, segst, jt, w, sg0, 1u, g, 0u);
         k_update<<<blocks_for(total), kBlock, 0, s>>>(d, i0, i1, e0, total, w, X, V, lbX, lbfit, g);
         k_eval<<<nt, kBlock, a->lds_bytes, s>>>(d, static_cast<const tile *>(a->tiles.ptr), t0, make_ctx(a, g));
         PGH_HIP(hipGetLastError());
@@ -431,6 +526,8 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
     auto a = std::make_unique<pgh_archi>();
     std::vector<prob> probs;
     uint64_t xoff = 0, foff = 0, woff = 0, toff = 0, eoff = 0;
+    std::vector<seg_desc> seg_list;
+    std::vector<uint64_t> dists;
     a->tile_begin.push_back(0);
     for (uint64_t k = 0; k < n_islands; ++k) {
         const pgh_island_spec &sp = islands[k];
@@ -491,7 +588,7 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
         di.eoff = eoff;
         xoff += align_up(elems, 32u); // 256-byte aligned rows blocks
         foff += align_up(sp.pop, 32u);
-        woff += align_up(elems * 4u, 64u);
+        woff += align_up(elems * 4u * kGenBatch, 64u); // the words of a batch of generations
         toff += static_cast<uint64_t>(q.gen) * sp.pop;
         eoff += elems;
         a->max_gens = std::max(a->max_gens, q.gen);
@@ -509,6 +606,30 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
             per = 256u / di.lanes;
         } else {
             di.lanes = 1u;
+        }
+        // the generator's segments (k_rng): as few as keep each segment's
+        // share of a batch within kSegWords words (a power of two, at most
+        // one segment per generation of the batch)
+        {
+            const uint64_t W = 4u * elems;
+            const uint32_t batch = std::min<uint32_t>(kGenBatch, std::max<uint32_t>(q.gen, 1u));
+            di.seg0 = static_cast<uint32_t>(seg_list.size());
+            di.segs = 1u;
+            while (di.segs < batch && ((batch + di.segs - 1u) / di.segs) * W > kSegWords) di.segs *= 2u;
+            // gps * segs = kGenBatch when a call has a batch or more of
+            // generations; otherwise the single batch's generations
+            di.gps = (batch + di.segs - 1u) / di.segs;
+            di.j_n0 = di.j_tree = di.j_main = 0u;
+            if (di.segs > 1u) {
+                const uint64_t share = static_cast<uint64_t>(di.gps) * W;
+                di.j_n0 = static_cast<uint32_t>(dists.size());
+                dists.push_back(2u * elems);
+                di.j_tree = static_cast<uint32_t>(dists.size());
+                for (uint32_t b = 0; (di.segs - 1u) >> b; ++b) dists.push_back(share << b);
+                di.j_main = static_cast<uint32_t>(dists.size());
+                dists.push_back((di.segs - 1u) * share);
+            }
+            for (uint32_t sgi = 0; sgi < di.segs; ++sgi) seg_list.push_back(seg_desc{static_cast<uint32_t>(k), sgi});
         }
         for (uint64_t f = 0; f < sp.pop; f += per) {
             a->h_tiles.push_back(tile{static_cast<uint32_t>(k), static_cast<uint32_t>(f),
@@ -530,6 +651,19 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
     PGH_TRY(a->lbfit.reserve(foff * 8u));
     PGH_TRY(a->words.reserve(std::max<uint64_t>(woff, 1u) * 4u));
     PGH_TRY(a->states.reserve(n_islands * sizeof(mt_state)));
+    a->n_segs = static_cast<uint32_t>(seg_list.size());
+    PGH_TRY(a->segs.reserve(seg_list.size() * sizeof(seg_desc)));
+    PGH_HIP(hipMemcpy(a->segs.ptr, seg_list.data(), seg_list.size() * sizeof(seg_desc), hipMemcpyHostToDevice));
+    PGH_TRY(a->seg_states.reserve(seg_list.size() * sizeof(mt_state)));
+    {
+        std::vector<uint8_t> tab;
+        PGH_TRY(jump_table_bytes(dists, tab));
+        PGH_TRY(a->jump_table.reserve(std::max<size_t>(tab.size(), 16u)));
+        if (!tab.empty()) PGH_HIP(hipMemcpy(a->jump_table.ptr, tab.data(), tab.size(), hipMemcpyHostToDevice));
+        a->jt.lens = static_cast<const uint32_t *>(a->jump_table.ptr);
+        a->jt.lists = reinterpret_cast<const uint16_t *>(static_cast<const char *>(a->jump_table.ptr) +
+                                                         jump_table_head(dists.size()));
+    }
     PGH_HIP(hipMemcpy(a->desc.ptr, a->h_desc.data(), a->h_desc.size() * sizeof(island_desc), hipMemcpyHostToDevice));
     PGH_HIP(hipMemcpy(a->tiles.ptr, a->h_tiles.data(), a->h_tiles.size() * sizeof(tile), hipMemcpyHostToDevice));
     std::vector<mt_state> states(n_islands);

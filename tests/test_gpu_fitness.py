@@ -262,7 +262,9 @@ def test_lennard_jones_gradient_finite_differences():
 
 # ---- RNG ----------------------------------------------------------------------
 
-@pytest.mark.parametrize("seed,skip,n", [(5489, 0, 10000), (42, 0, 1), (42, 1000, 5000), (7, 623, 700)])
+# beyond 2^21 words the stream is split across workgroups by jump-ahead
+@pytest.mark.parametrize("seed,skip,n", [(5489, 0, 10000), (42, 0, 1), (42, 1000, 5000), (7, 623, 700),
+                                         (5489, 0, (1 << 21) + 1), (42, 0, 5_000_000), (9, 0, (1 << 24) + 12345)])
 def test_mt19937_bit_exact(seed, skip, n):
     out = torch.empty(n, dtype=torch.int32, device=DEV)
     P.mt19937_generate(seed, n, out, skip=skip)
@@ -270,7 +272,8 @@ def test_mt19937_bit_exact(seed, skip, n):
     assert np.array_equal(out.cpu().numpy().view(np.uint32), O.mt19937_u32(seed, n, skip=skip))
 
 
-@pytest.mark.parametrize("kind,dim,n", [(P.ROSENBROCK, 10, 64), (P.RASTRIGIN, 1000, 33), (P.LENNARD_JONES, 48, 20)])
+@pytest.mark.parametrize("kind,dim,n", [(P.ROSENBROCK, 10, 64), (P.RASTRIGIN, 1000, 33), (P.LENNARD_JONES, 48, 20),
+                                        (P.RASTRIGIN, 1000, 2000), (P.ROSENBROCK, 10, 150_001)])
 def test_batch_random_dv_bit_exact(kind, dim, n):
     nx, _ = P.dims(kind, dim)
     out = torch.empty((n, nx), dtype=torch.float64, device=DEV)

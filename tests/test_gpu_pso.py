@@ -145,3 +145,33 @@ def test_device_seeded_archipelago():
 def test_device_seeded_archipelago_checks():
     with pytest.raises(P.PghError, match="at least one island"):
         P.Archipelago.seeded([])
+
+
+@pytest.mark.parametrize("gens", [0, 1, 3])
+def test_split_generators_match_the_oracle(gens):
+    """Islands drawing 2^19 words or more per generation split their
+    std::mt19937 into segments reached by jump-ahead (k_rng): the words, and
+    so the trajectories, are those of the one sequential generator, over two
+    evolve calls, in batched and per-island launches."""
+    spec = [(P.ROSENBROCK, 300, 1000), (P.RASTRIGIN, 100, 2000), (P.LENNARD_JONES, 30, 2000), (P.ROSENBROCK, 10, 50)]
+    isl = []
+    for k, (kind, dim, pop) in enumerate(spec):
+        X0, F0 = O.population_init(kind, dim, pop, 70 + k)
+        isl.append(dict(kind=kind, dim=dim, X0=X0, F0=F0, algo=P.pso_params(gen=gens, seed=80 + k)))
+    res = {}
+    for mode in (P.ARCHI_BATCHED, P.ARCHI_PER_ISLAND):
+        a = P.Archipelago(isl)
+        a.evolve(mode)
+        a.evolve(mode)
+        res[mode] = [a.get(k) for k in range(len(isl))]
+        a.close()
+    for k, d in enumerate(isl):
+        xb, fb = res[P.ARCHI_BATCHED][k]
+        xp, fp = res[P.ARCHI_PER_ISLAND][k]
+        assert np.array_equal(xb, xp) and np.array_equal(fb, fp), k
+        xo, fo = O.pso_gen_evolve(d["kind"], d["dim"], d["X0"], d["F0"], gens, 80 + k, calls=2)
+        assert np.array_equal(xb, xo), k
+        if d["kind"] == P.ROSENBROCK:
+            assert np.array_equal(fb, fo), k
+        else:
+            np.testing.assert_allclose(fb, fo, rtol=1e-9, atol=1e-9)
