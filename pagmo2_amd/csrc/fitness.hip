@@ -445,8 +445,23 @@ pgh_status launch_lj_l(const double *dvs, uint64_t n, uint32_t atoms, double *fv
         case 4:
             return launch_lj_la<L, 4>(dvs, n, atoms, fvs, grad, s);
         default:
-            return launch_lj_la<L, 0>(dvs, n, atoms, fvs, grad, s);
+            break;
     }
+    if constexpr (L >= 32) {
+        switch (lj_apl(atoms, L)) {
+            case 5:
+                return launch_lj_la<L, 5>(dvs, n, atoms, fvs, grad, s);
+            case 6:
+                return launch_lj_la<L, 6>(dvs, n, atoms, fvs, grad, s);
+            case 7:
+                return launch_lj_la<L, 7>(dvs, n, atoms, fvs, grad, s);
+            case 8:
+                return launch_lj_la<L, 8>(dvs, n, atoms, fvs, grad, s);
+            default:
+                break;
+        }
+    }
+    return launch_lj_la<L, 0>(dvs, n, atoms, fvs, grad, s);
 }
 
 pgh_status launch_lj(const double *dvs, uint64_t n, uint32_t atoms, double *fvs, double *grad, hipStream_t s)

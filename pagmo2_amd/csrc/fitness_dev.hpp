@@ -812,7 +812,9 @@ __host__ __device__ constexpr uint32_t lj_apl(uint32_t atoms, int L)
 {
     const uint32_t la = static_cast<uint32_t>(lj_atom_lanes(L));
     const uint32_t apl = (atoms + la - 1u) / la;
-    return apl <= 4u ? apl : 0u;
+    // up to 4 slots for every lane count; 32-lane groups (from 32 atoms on)
+    // up to 8, i.e. 128 atoms
+    return apl <= (L >= 32 ? 8u : 4u) ? apl : 0u;
 }
 
 

@@ -363,8 +363,27 @@ __device__ __forceinline__ void eval_lj(const island_desc &di, const tile &tl, c
             eval_lj_a<L, 4>(di, tl, c, lds);
             return;
         default:
-            eval_lj_a<L, 0>(di, tl, c, lds);
+            break;
     }
+    if constexpr (L >= 32) {
+        switch (lj_apl(di.dim, L)) {
+            case 5:
+                eval_lj_a<L, 5>(di, tl, c, lds);
+                return;
+            case 6:
+                eval_lj_a<L, 6>(di, tl, c, lds);
+                return;
+            case 7:
+                eval_lj_a<L, 7>(di, tl, c, lds);
+                return;
+            case 8:
+                eval_lj_a<L, 8>(di, tl, c, lds);
+                return;
+            default:
+                break;
+        }
+    }
+    eval_lj_a<L, 0>(di, tl, c, lds);
 }
 
 __global__ __launch_bounds__(kBlock) void k_eval(const island_desc *__restrict__ d, const tile *__restrict__ tiles,
