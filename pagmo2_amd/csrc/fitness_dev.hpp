@@ -429,6 +429,17 @@ __device__ __noinline__ void rastrigin_grad_row(const double *__restrict__ x, ui
 // ZDT2 and ZDT5 (bit-exact); ZDT3, ZDT4 and ZDT6 call the device sin, cos,
 // exp and pow.
 // ---------------------------------------------------------------------------
+// 10 cos(4 pi x) (zdt.cpp:281-296) as cos_2pi_small at the exact 2x for
+// |2x| < 2^10 (the error against the reference's cos(4 pi x) is the rounding
+// of 4 pi x there, about |4 pi x| 1e-16 absolute; ZDT4's g >= 1 sums such
+// terms); the math library's cosine of the reference's argument beyond.
+__device__ __forceinline__ double zdt4_cos10(double x)
+{
+    const double y = 2. * x;
+    if (fabs(y) < 0x1p10) return cos_2pi_small<true>(y);
+    return 10. * cos(4. * kPi * x);
+}
+
 __device__ __forceinline__ void zdt_eval(uint32_t id, const double *__restrict__ x, uint32_t N, double &f0,
                                          double &f1)
 {
@@ -453,7 +464,7 @@ __device__ __forceinline__ void zdt_eval(uint32_t id, const double *__restrict__
     if (id == 4u) {
         // zdt.cpp:281-296
         double g = 1 + 10 * static_cast<double>(N - 1u);
-        for (uint32_t i = 1u; i < N; ++i) g += x[i] * x[i] - 10. * cos(4. * kPi * x[i]);
+        for (uint32_t i = 1u; i < N; ++i) g += x[i] * x[i] - zdt4_cos10(x[i]);
         f0 = x0;
         f1 = g * (1. - sqrt(x0 / g));
         return;
@@ -483,6 +494,16 @@ __device__ __forceinline__ void zdt_eval(uint32_t id, const double *__restrict__
 // distance variables x_M = x[M-1 .. nx-1].  pow(t, 2) is written t * t, the
 // correctly rounded square (glibc's pow returns it too, to within 0.52 ulp).
 // ---------------------------------------------------------------------------
+// cos(20 pi t) (dtlz.cpp:207-214) as cos(2 pi (10 t)) through cos_2pi_small
+// for |10 t| < 2^10 (about |20 pi t| 1e-16 absolute from the reference's
+// rounded argument), the math library's cosine beyond.
+__device__ __forceinline__ double dtlz_cos20pi(double t)
+{
+    const double y = 10. * t;
+    if (fabs(y) < 0x1p10) return cos_2pi_small<false>(y);
+    return cos(20. * kPi * t);
+}
+
 __device__ __forceinline__ double dtlz_g(uint32_t id, const double *__restrict__ xm, uint32_t K)
 {
     double y = 0.;
@@ -491,7 +512,7 @@ __device__ __forceinline__ double dtlz_g(uint32_t id, const double *__restrict__
         case 3u: // dtlz.cpp:207-214
             for (uint32_t i = 0u; i < K; ++i) {
                 const double t = xm[i] - 0.5;
-                y += t * t - cos(20. * kPi * (xm[i] - 0.5));
+                y += t * t - dtlz_cos20pi(t);
             }
             return 100. * (y + static_cast<double>(K));
         case 6u: // dtlz.cpp:225-232
