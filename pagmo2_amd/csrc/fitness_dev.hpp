@@ -672,6 +672,17 @@ __device__ __forceinline__ int lj_pair_index(uint32_t a, uint32_t b, uint32_t at
     return static_cast<int>(lj_row_start(i, atoms) + (j - i - 1u));
 }
 
+// The careful paths below are out of line, so the compiler no longer sees
+// that their staged individual lies in LDS; they address it through an LDS
+// pointer (LDS loads instead of generic ones).  Their callers always pass a
+// pointer into the workgroup's LDS block.
+using lds_double = const __attribute__((address_space(3))) double;
+
+__device__ __forceinline__ lds_double *lds_ptr(const double *p)
+{
+    return (lds_double *)p;
+}
+
 // 1 / d by v_rcp_f64 refined with two Newton steps: accurate to about an ulp
 // for d in [2^-1000, 2^1000]; outside that range (0, subnormal, huge, inf,
 // NaN) the result is not 1 / d, and callers that can meet such d use lj_recip.
@@ -727,7 +738,7 @@ __device__ __noinline__ double lj_zero_rule(const double *__restrict__ P, uint32
     for (uint32_t k = 1u; k <= cyc.half; ++k) {
         for (uint32_t a = static_cast<uint32_t>(r); a < atoms; a += L) {
             if (!cyc.counts(a, k)) continue;
-            const double *pa = P + 3u * a, *pb = pa + 3u * k;
+            lds_double *pa = lds_ptr(P) + 3u * a, *pb = pa + 3u * k;
             const double dx = pa[0] - pb[0], dy = pa[1] - pb[1], dz = pa[2] - pb[2];
             const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
             if (r2 == 0.0) continue;
@@ -740,32 +751,46 @@ __device__ __noinline__ double lj_zero_rule(const double *__restrict__ P, uint32
     return nlast > zlast ? __builtin_nan("") : __builtin_inf();
 }
 
-// Energy with every special case: lj_recip's full range, zero distances
-// (excluded from the sum and recorded for lj_zero_rule) and masked visits.
+// Energy with every special case: lj_recip's full range, zero distances and
+// the reference's rule for them (see lj_zero_rule), in one pass.  Four owned
+// atoms per pass, each with its own partial sum, so that a lane has four
+// independent chains in flight (a group that lands here runs alone on its
+// wavefront while the rest of the workgroup waits).
 template <int L>
 __device__ __noinline__ double lj_energy_careful(const double *__restrict__ P, uint32_t atoms, int r)
 {
     const lj_cycle cyc(atoms);
-    double acc = 0.;
-    int zlast = -1;
-    for (uint32_t k = 1u; k <= cyc.half; ++k) {
-        for (uint32_t a = static_cast<uint32_t>(r); a < atoms; a += L) {
-            if (!cyc.counts(a, k)) continue;
-            const double *pa = P + 3u * a, *pb = pa + 3u * k;
-            const double dx = pa[0] - pb[0], dy = pa[1] - pb[1], dz = pa[2] - pb[2];
-            const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-            if (r2 == 0.0) {
-                zlast = max(zlast, lj_pair_index(a, cyc.partner(a, k), atoms));
-                continue;
+    lds_double *Q = lds_ptr(P);
+    double acc[4] = {0., 0., 0., 0.};
+    int zlast = -1, nlast = -1;
+    for (uint32_t a0 = static_cast<uint32_t>(r); a0 < atoms; a0 += 4u * L) {
+        for (uint32_t k = 1u; k <= cyc.half; ++k) {
+#pragma unroll
+            for (uint32_t q = 0; q < 4u; ++q) {
+                const uint32_t a = a0 + q * L;
+                if (a >= atoms || !cyc.counts(a, k)) continue;
+                lds_double *pa = Q + 3u * a, *pb = pa + 3u * k;
+                const double dx = pa[0] - pb[0], dy = pa[1] - pb[1], dz = pa[2] - pb[2];
+                const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                if (r2 == 0.0) {
+                    zlast = max(zlast, lj_pair_index(a, cyc.partner(a, k), atoms));
+                    continue;
+                }
+                const double s = lj_recip(r2 * r2 * r2);
+                const double t = fma(s, s, -s);
+                if (t != t) nlast = max(nlast, lj_pair_index(a, cyc.partner(a, k), atoms));
+                acc[q] += t;
             }
-            const double s = lj_recip(r2 * r2 * r2);
-            acc += fma(s, s, -s);
         }
     }
-    acc = group_sum<L>(acc);
+    const double sum = group_sum<L>((acc[0] + acc[1]) + (acc[2] + acc[3]));
     zlast = group_max<L>(zlast);
-    if (zlast >= 0) return lj_zero_rule<L>(P, atoms, r, zlast);
-    return 4 * acc;
+    nlast = group_max<L>(nlast);
+    // lennard_jones.cpp:81-83,89: after a zero distance f is DBL_MAX plus the
+    // later terms (each >= -1/4), so 4 f overflows: +inf, or NaN if a NaN term
+    // follows the last zero-distance pair in the reference's pair order
+    if (zlast >= 0) return nlast > zlast ? __builtin_nan("") : __builtin_inf();
+    return 4 * sum;
 }
 
 // Writes the gradient components of atom a (its force f) into the gradient row
@@ -795,10 +820,10 @@ __device__ __noinline__ double lj_energy_grad_careful(const double *__restrict__
     double acc = 0.;
     int zlast = -1;
     for (uint32_t a = static_cast<uint32_t>(r); a < atoms; a += L) {
-        const double *pa = P + 3u * a;
+        lds_double *pa = lds_ptr(P) + 3u * a;
         double fx = 0., fy = 0., fz = 0.;
         for (uint32_t k = 1u; k < atoms; ++k) {
-            const double *pb = pa + 3u * k;
+            lds_double *pb = pa + 3u * k;
             const double dx = pa[0] - pb[0], dy = pa[1] - pb[1], dz = pa[2] - pb[2];
             const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
             const double s = lj_recip(r2 * r2 * r2);
