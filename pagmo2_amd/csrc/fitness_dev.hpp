@@ -724,6 +724,26 @@ struct lj_cycle {
     }
 };
 
+// Steps of one lane: the schedule's T steps are dealt round-robin to the KS
+// halves of a group, half h taking k = h + 1 + i KS.  All halves run the same
+// n = ceil(T / KS) steps: n - 1 steps that are always full, then a last one
+// that is full, the partial step A/2 (A even: only a < A/2 counts) or, past
+// T, a dummy of weight 0.
+struct lj_steps {
+    uint32_t first, stride, n, last, T;
+    bool dummy;
+    __device__ __forceinline__ lj_steps(uint32_t T_, uint32_t h, uint32_t KS)
+    {
+        T = T_;
+        first = h + 1u;
+        stride = KS;
+        n = (T + KS - 1u) / KS;
+        const uint32_t kt = first + (n - 1u) * KS;
+        dummy = kt > T;
+        last = dummy ? T : kt;
+    }
+};
+
 // ---- careful paths: any input, the reference's special cases -----------------
 
 // The reference's dist == 0 rule (lennard_jones.cpp:81-83,89): the result is
@@ -752,34 +772,69 @@ __device__ __noinline__ double lj_zero_rule(const double *__restrict__ P, uint32
 }
 
 // Energy with every special case: lj_recip's full range, zero distances and
-// the reference's rule for them (see lj_zero_rule), in one pass.  Four owned
-// atoms per pass, each with its own partial sum, so that a lane has four
-// independent chains in flight (a group that lands here runs alone on its
-// wavefront while the rest of the workgroup waits).
+// the reference's rule for them (see lj_zero_rule), in one pass over the
+// streaming kernel's own schedule (lj_steps over the group's KS halves, four
+// owned atoms per pass), so that a lane keeps four independent chains and
+// the group's lanes share the pairs as in lj_energy.
 template <int L>
 __device__ __noinline__ double lj_energy_careful(const double *__restrict__ P, uint32_t atoms, int r)
 {
+    constexpr int LA = lj_atom_lanes(L);
+    constexpr uint32_t KS = static_cast<uint32_t>(L / LA);
+    const uint32_t ra = static_cast<uint32_t>(r) % LA, h = static_cast<uint32_t>(r) / LA;
     const lj_cycle cyc(atoms);
+    const lj_steps st(cyc.half, h, KS);
     lds_double *Q = lds_ptr(P);
     double acc[4] = {0., 0., 0., 0.};
     int zlast = -1, nlast = -1;
-    for (uint32_t a0 = static_cast<uint32_t>(r); a0 < atoms; a0 += 4u * L) {
-        for (uint32_t k = 1u; k <= cyc.half; ++k) {
+    for (uint32_t a0 = ra; a0 < atoms; a0 += 4u * LA) {
+        uint32_t a[4], aa[4];
+        double x[4], y[4], z[4];
+#pragma unroll
+        for (uint32_t q = 0; q < 4u; ++q) {
+            a[q] = a0 + q * LA;
+            aa[q] = a[q] < atoms ? a[q] : ra; // a slot past the last atom repeats atom ra, uncounted
+            x[q] = Q[3u * aa[q]];
+            y[q] = Q[3u * aa[q] + 1u];
+            z[q] = Q[3u * aa[q] + 2u];
+        }
+        for (uint32_t i = 0u; i < st.n; ++i) {
+            const bool last = i + 1u == st.n;
+            const uint32_t k = last ? st.last : st.first + i * st.stride;
+            double r2[4], d[4], s[4];
+            bool cnt[4], odd = false;
 #pragma unroll
             for (uint32_t q = 0; q < 4u; ++q) {
-                const uint32_t a = a0 + q * L;
-                if (a >= atoms || !cyc.counts(a, k)) continue;
-                lds_double *pa = Q + 3u * a, *pb = pa + 3u * k;
-                const double dx = pa[0] - pb[0], dy = pa[1] - pb[1], dz = pa[2] - pb[2];
-                const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                if (r2 == 0.0) {
-                    zlast = max(zlast, lj_pair_index(a, cyc.partner(a, k), atoms));
-                    continue;
+                lds_double *pb = Q + 3u * (aa[q] + k);
+                const double dx = x[q] - pb[0], dy = y[q] - pb[1], dz = z[q] - pb[2];
+                r2[q] = fma(dz, dz, fma(dy, dy, dx * dx));
+                d[q] = r2[q] * r2[q] * r2[q];
+                cnt[q] = a[q] < atoms && !(last && st.dummy) && cyc.counts(a[q], k);
+                s[q] = lj_recip_fast(d[q]);
+                odd |= cnt[q] && !(d[q] > 0x1p-1000 && d[q] < 0x1p1000);
+            }
+            // rare: a counted visit outside lj_recip_fast's range (zero,
+            // subnormal, huge, inf or NaN r^6) takes the IEEE reciprocal;
+            // zero distances and NaN terms are recorded by pair index
+            if (odd) {
+#pragma unroll
+                for (uint32_t q = 0; q < 4u; ++q) {
+                    if (!cnt[q] || (d[q] > 0x1p-1000 && d[q] < 0x1p1000)) continue;
+                    const int pi = lj_pair_index(a[q], cyc.partner(a[q], k), atoms);
+                    if (r2[q] == 0.0) {
+                        zlast = max(zlast, pi);
+                        cnt[q] = false;
+                        continue;
+                    }
+                    s[q] = 1.0 / d[q];
+                    const double t = fma(s[q], s[q], -s[q]);
+                    if (t != t) nlast = max(nlast, pi);
                 }
-                const double s = lj_recip(r2 * r2 * r2);
-                const double t = fma(s, s, -s);
-                if (t != t) nlast = max(nlast, lj_pair_index(a, cyc.partner(a, k), atoms));
-                acc[q] += t;
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < 4u; ++q) {
+                const double t = fma(s[q], s[q], -s[q]);
+                acc[q] += cnt[q] ? t : 0.0;
             }
         }
     }
@@ -902,26 +957,6 @@ struct lj_pair {
         const double r = __builtin_amdgcn_rcp(r2);
         inv = fma(r, fma(-r2, r, 1.0), r);
         s = (inv * inv) * inv;
-    }
-};
-
-// Steps of one lane: the schedule's T steps are dealt round-robin to the KS
-// halves of a group, half h taking k = h + 1 + i KS.  All halves run the same
-// n = ceil(T / KS) steps: n - 1 steps that are always full, then a last one
-// that is full, the partial step A/2 (A even: only a < A/2 counts) or, past
-// T, a dummy of weight 0.
-struct lj_steps {
-    uint32_t first, stride, n, last, T;
-    bool dummy;
-    __device__ __forceinline__ lj_steps(uint32_t T_, uint32_t h, uint32_t KS)
-    {
-        T = T_;
-        first = h + 1u;
-        stride = KS;
-        n = (T + KS - 1u) / KS;
-        const uint32_t kt = first + (n - 1u) * KS;
-        dummy = kt > T;
-        last = dummy ? T : kt;
     }
 };
 
