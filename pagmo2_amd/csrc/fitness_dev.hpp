@@ -773,9 +773,11 @@ __device__ __noinline__ double lj_zero_rule(const double *__restrict__ P, uint32
 
 // Energy with every special case: lj_recip's full range, zero distances and
 // the reference's rule for them (see lj_zero_rule), in one pass over the
-// streaming kernel's own schedule (lj_steps over the group's KS halves, four
-// owned atoms per pass), so that a lane keeps four independent chains and
-// the group's lanes share the pairs as in lj_energy.
+// streaming kernel's own schedule (lj_steps over the group's KS halves,
+// kLjCarefulChains owned atoms per pass, each its own accumulation chain), so
+// the group's lanes share the pairs as in lj_energy.  Two chains, not more:
+// the kernels that call this function reserve its registers as well.
+constexpr uint32_t kLjCarefulChains = 2;
 template <int L>
 __device__ __noinline__ double lj_energy_careful(const double *__restrict__ P, uint32_t atoms, int r)
 {
@@ -785,13 +787,13 @@ __device__ __noinline__ double lj_energy_careful(const double *__restrict__ P, u
     const lj_cycle cyc(atoms);
     const lj_steps st(cyc.half, h, KS);
     lds_double *Q = lds_ptr(P);
-    double acc[4] = {0., 0., 0., 0.};
+    double acc[kLjCarefulChains] = {};
     int zlast = -1, nlast = -1;
-    for (uint32_t a0 = ra; a0 < atoms; a0 += 4u * LA) {
-        uint32_t a[4], aa[4];
-        double x[4], y[4], z[4];
+    for (uint32_t a0 = ra; a0 < atoms; a0 += kLjCarefulChains * LA) {
+        uint32_t a[kLjCarefulChains], aa[kLjCarefulChains];
+        double x[kLjCarefulChains], y[kLjCarefulChains], z[kLjCarefulChains];
 #pragma unroll
-        for (uint32_t q = 0; q < 4u; ++q) {
+        for (uint32_t q = 0; q < kLjCarefulChains; ++q) {
             a[q] = a0 + q * LA;
             aa[q] = a[q] < atoms ? a[q] : ra; // a slot past the last atom repeats atom ra, uncounted
             x[q] = Q[3u * aa[q]];
@@ -801,10 +803,10 @@ __device__ __noinline__ double lj_energy_careful(const double *__restrict__ P, u
         for (uint32_t i = 0u; i < st.n; ++i) {
             const bool last = i + 1u == st.n;
             const uint32_t k = last ? st.last : st.first + i * st.stride;
-            double r2[4], d[4], s[4];
-            bool cnt[4], odd = false;
+            double r2[kLjCarefulChains], d[kLjCarefulChains], s[kLjCarefulChains];
+            bool cnt[kLjCarefulChains], odd = false;
 #pragma unroll
-            for (uint32_t q = 0; q < 4u; ++q) {
+            for (uint32_t q = 0; q < kLjCarefulChains; ++q) {
                 lds_double *pb = Q + 3u * (aa[q] + k);
                 const double dx = x[q] - pb[0], dy = y[q] - pb[1], dz = z[q] - pb[2];
                 r2[q] = fma(dz, dz, fma(dy, dy, dx * dx));
@@ -818,7 +820,7 @@ __device__ __noinline__ double lj_energy_careful(const double *__restrict__ P, u
             // zero distances and NaN terms are recorded by pair index
             if (odd) {
 #pragma unroll
-                for (uint32_t q = 0; q < 4u; ++q) {
+                for (uint32_t q = 0; q < kLjCarefulChains; ++q) {
                     if (!cnt[q] || (d[q] > 0x1p-1000 && d[q] < 0x1p1000)) continue;
                     const int pi = lj_pair_index(a[q], cyc.partner(a[q], k), atoms);
                     if (r2[q] == 0.0) {
@@ -832,13 +834,13 @@ __device__ __noinline__ double lj_energy_careful(const double *__restrict__ P, u
                 }
             }
 #pragma unroll
-            for (uint32_t q = 0; q < 4u; ++q) {
+            for (uint32_t q = 0; q < kLjCarefulChains; ++q) {
                 const double t = fma(s[q], s[q], -s[q]);
                 acc[q] += cnt[q] ? t : 0.0;
             }
         }
     }
-    const double sum = group_sum<L>((acc[0] + acc[1]) + (acc[2] + acc[3]));
+    const double sum = group_sum<L>(acc[0] + acc[1]);
     zlast = group_max<L>(zlast);
     nlast = group_max<L>(nlast);
     // lennard_jones.cpp:81-83,89: after a zero distance f is DBL_MAX plus the
