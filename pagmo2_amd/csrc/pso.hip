@@ -884,7 +884,10 @@ pgh_status pgh_pso_gen_evolve_host(const pgh_udp *udp, const pgh_pso_params *prm
     const double *fs[1] = {F};
     PGH_TRY(pgh_archi_create(&a, &sp, 1u, xs, fs));
     std::unique_ptr<pgh_archi> guard(a);
-    PGH_TRY(pgh_archi_evolve(a, PGH_ARCHI_BATCHED, nullptr));
+    // one island, evolved once: direct launches (the same kernels a batched
+    // evolve captures; capturing and instantiating a graph that runs once
+    // costs more than it saves)
+    PGH_TRY(pgh_archi_evolve(a, PGH_ARCHI_PER_ISLAND, nullptr));
     return pgh_archi_get(a, 0u, X, F);
 }
 
