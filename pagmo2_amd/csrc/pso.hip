@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstring>
 #include <memory>
 #include <string>
 #include <vector>
@@ -443,6 +444,13 @@ uint64_t align_up(uint64_t v, uint64_t a)
     return (v + a - 1u) / a * a;
 }
 
+// A buffer inside an archipelago's arena: its byte offset while the layout
+// is planned, its device address afterwards.
+struct arena_view {
+    void *ptr = nullptr;
+    uint64_t off = 0, bytes = 0;
+};
+
 } // namespace
 } // namespace pgh
 
@@ -459,7 +467,11 @@ struct pgh_archi {
     bool v_valid = false;
     bool evolved = false; // the population lives in lbX/lbfit once evolved
     std::vector<uint32_t> memory;
-    dev_buf desc, tiles, X, V, lbX, fit, lbfit, words, states, trace_buf, segs, seg_states, jump_table;
+    // one device allocation (arena) holds every buffer below but trace_buf;
+    // the small tables (desc, tiles, segs, jump_table, states) come first
+    // and go up in one copy
+    dev_buf arena, trace_buf;
+    arena_view desc, tiles, segs, jump_table, states, X, V, lbX, fit, lbfit, words, seg_states;
     uint32_t n_segs = 0;
     jump_tab jt{nullptr, nullptr};
     hipStream_t own = nullptr;
@@ -661,33 +673,53 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
     a->total_words = woff;
     a->total_trace = toff;
     PGH_HIP(hipStreamCreateWithFlags(&a->own, hipStreamNonBlocking));
-    PGH_TRY(a->desc.reserve(a->h_desc.size() * sizeof(island_desc)));
-    PGH_TRY(a->tiles.reserve(a->h_tiles.size() * sizeof(tile)));
-    PGH_TRY(a->X.reserve(xoff * 8u));
-    PGH_TRY(a->V.reserve(xoff * 8u));
-    PGH_TRY(a->lbX.reserve(xoff * 8u));
-    PGH_TRY(a->fit.reserve(foff * 8u));
-    PGH_TRY(a->lbfit.reserve(foff * 8u));
-    PGH_TRY(a->words.reserve(std::max<uint64_t>(woff, 1u) * 4u));
-    PGH_TRY(a->states.reserve(n_islands * sizeof(mt_state)));
     a->n_segs = static_cast<uint32_t>(seg_list.size());
-    PGH_TRY(a->segs.reserve(seg_list.size() * sizeof(seg_desc)));
-    PGH_HIP(hipMemcpy(a->segs.ptr, seg_list.data(), seg_list.size() * sizeof(seg_desc), hipMemcpyHostToDevice));
-    PGH_TRY(a->seg_states.reserve(seg_list.size() * sizeof(mt_state)));
-    {
-        std::vector<uint8_t> tab;
-        PGH_TRY(jump_table_bytes(dists, tab));
-        PGH_TRY(a->jump_table.reserve(std::max<size_t>(tab.size(), 16u)));
-        if (!tab.empty()) PGH_HIP(hipMemcpy(a->jump_table.ptr, tab.data(), tab.size(), hipMemcpyHostToDevice));
-        a->jt.lens = static_cast<const uint32_t *>(a->jump_table.ptr);
-        a->jt.lists = reinterpret_cast<const uint16_t *>(static_cast<const char *>(a->jump_table.ptr) +
-                                                         jump_table_head(dists.size()));
-    }
-    PGH_HIP(hipMemcpy(a->desc.ptr, a->h_desc.data(), a->h_desc.size() * sizeof(island_desc), hipMemcpyHostToDevice));
-    PGH_HIP(hipMemcpy(a->tiles.ptr, a->h_tiles.data(), a->h_tiles.size() * sizeof(tile), hipMemcpyHostToDevice));
+    std::vector<uint8_t> tab;
+    PGH_TRY(jump_table_bytes(dists, tab));
     std::vector<mt_state> states(n_islands);
+    for (uint64_t k = 0; k < n_islands; ++k) mt_seed_host(islands[k].algo.seed, states[k]);
+    // arena layout, every buffer on a 256-byte boundary
+    uint64_t arena_bytes = 0;
+    auto place = [&](arena_view &v, uint64_t bytes) {
+        v.off = arena_bytes;
+        v.bytes = bytes;
+        arena_bytes = align_up(arena_bytes + std::max<uint64_t>(bytes, 1u), 256u);
+    };
+    place(a->desc, a->h_desc.size() * sizeof(island_desc));
+    place(a->tiles, a->h_tiles.size() * sizeof(tile));
+    place(a->segs, seg_list.size() * sizeof(seg_desc));
+    place(a->jump_table, std::max<size_t>(tab.size(), 16u));
+    place(a->states, n_islands * sizeof(mt_state));
+    const uint64_t head_bytes = arena_bytes;
+    place(a->X, xoff * 8u);
+    place(a->V, xoff * 8u);
+    place(a->lbX, xoff * 8u);
+    place(a->fit, foff * 8u);
+    place(a->lbfit, foff * 8u);
+    place(a->words, std::max<uint64_t>(woff, 1u) * 4u);
+    place(a->seg_states, seg_list.size() * sizeof(mt_state));
+    PGH_TRY(a->arena.reserve(arena_bytes));
+    auto *base = static_cast<uint8_t *>(a->arena.ptr);
+    for (arena_view *v : {&a->desc, &a->tiles, &a->segs, &a->jump_table, &a->states, &a->X, &a->V, &a->lbX, &a->fit,
+                          &a->lbfit, &a->words, &a->seg_states}) {
+        v->ptr = base + v->off;
+    }
+    {
+        std::vector<uint8_t> head(head_bytes, 0u);
+        auto put = [&](const arena_view &v, const void *src, uint64_t bytes) {
+            if (bytes) std::memcpy(head.data() + v.off, src, bytes);
+        };
+        put(a->desc, a->h_desc.data(), a->h_desc.size() * sizeof(island_desc));
+        put(a->tiles, a->h_tiles.data(), a->h_tiles.size() * sizeof(tile));
+        put(a->segs, seg_list.data(), seg_list.size() * sizeof(seg_desc));
+        put(a->jump_table, tab.data(), tab.size());
+        put(a->states, states.data(), n_islands * sizeof(mt_state));
+        PGH_HIP(hipMemcpy(base, head.data(), head_bytes, hipMemcpyHostToDevice));
+    }
+    a->jt.lens = static_cast<const uint32_t *>(a->jump_table.ptr);
+    a->jt.lists = reinterpret_cast<const uint16_t *>(static_cast<const char *>(a->jump_table.ptr) +
+                                                     jump_table_head(dists.size()));
     for (uint64_t k = 0; k < n_islands; ++k) {
-        mt_seed_host(islands[k].algo.seed, states[k]);
         const island_desc &di = a->h_desc[k];
         double *Xk = static_cast<double *>(a->X.ptr) + di.xoff;
         double *Fk = static_cast<double *>(a->fit.ptr) + di.foff;
@@ -702,7 +734,6 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
             PGH_HIP(hipMemcpy(Fk, F0[k], static_cast<uint64_t>(di.pop) * 8u, hipMemcpyHostToDevice));
         }
     }
-    PGH_HIP(hipMemcpy(a->states.ptr, states.data(), n_islands * sizeof(mt_state), hipMemcpyHostToDevice));
     if (seeds) PGH_HIP(hipStreamSynchronize(a->own));
     *out = a.release();
     return PGH_OK;
