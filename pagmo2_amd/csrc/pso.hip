@@ -215,19 +215,12 @@ __global__ __launch_bounds__(kBlock) void k_vinit(const island_desc *__restrict_
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_update(const island_desc *__restrict__ d, uint32_t i0, uint32_t i1,
-                                                   uint64_t e0, uint64_t total, const uint32_t *__restrict__ words,
-                                                   double *__restrict__ X, double *__restrict__ V,
-                                                   const double *__restrict__ lbX, const double *__restrict__ lbfit,
-                                                   uint32_t gen)
+// Velocity and position of element m of island di in generation gen.
+__device__ __forceinline__ void update_elem(const island_desc &di, uint64_t m, const uint32_t *__restrict__ words,
+                                            double *__restrict__ X, double *__restrict__ V,
+                                            const double *__restrict__ lbX, const double *__restrict__ lbfit,
+                                            uint32_t gen)
 {
-    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    if (t >= total) return;
-    const uint64_t e = e0 + t;
-    const uint32_t k = find_island(d, i0, i1, e);
-    const island_desc di = d[k];
-    if (gen >= di.gens) return;
-    const uint64_t m = e - di.eoff;
     const uint32_t p = static_cast<uint32_t>(m / di.nx);
     const uint32_t dd = static_cast<uint32_t>(m - static_cast<uint64_t>(p) * di.nx);
     const uint32_t S = di.pop, R = di.radius;
@@ -269,6 +262,21 @@ __global__ __launch_bounds__(kBlock) void k_update(const island_desc *__restrict
     }
     X[xi] = nxv;
     V[xi] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_update(const island_desc *__restrict__ d, uint32_t i0, uint32_t i1,
+                                                   uint64_t e0, uint64_t total, const uint32_t *__restrict__ words,
+                                                   double *__restrict__ X, double *__restrict__ V,
+                                                   const double *__restrict__ lbX, const double *__restrict__ lbfit,
+                                                   uint32_t gen)
+{
+    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= total) return;
+    const uint64_t e = e0 + t;
+    const uint32_t k = find_island(d, i0, i1, e);
+    const island_desc di = d[k];
+    if (gen >= di.gens) return;
+    update_elem(di, e - di.eoff, words, X, V, lbX, lbfit, gen);
 }
 
 struct eval_ctx {
@@ -387,13 +395,10 @@ __device__ __forceinline__ void eval_lj(const island_desc &di, const tile &tl, c
     eval_lj_a<L, 0>(di, tl, c, lds);
 }
 
-__global__ __launch_bounds__(kBlock) void k_eval(const island_desc *__restrict__ d, const tile *__restrict__ tiles,
-                                                 uint32_t t0, eval_ctx c)
+// Fitness and memory update of the individuals of tile tl (all threads of
+// the workgroup call it).
+__device__ __forceinline__ void eval_tile(const island_desc &di, const tile &tl, const eval_ctx &c, double *lds)
 {
-    extern __shared__ double lds[];
-    const tile tl = tiles[t0 + blockIdx.x];
-    const island_desc di = d[tl.island];
-    if (c.gen >= di.gens) return;
     switch (di.kind) {
         case ROSENBROCK:
             if (threadIdx.x < tl.count) {
@@ -435,6 +440,41 @@ __global__ __launch_bounds__(kBlock) void k_eval(const island_desc *__restrict__
                     eval_lj<1>(di, tl, c, lds);
                     return;
             }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_eval(const island_desc *__restrict__ d, const tile *__restrict__ tiles,
+                                                 uint32_t t0, eval_ctx c)
+{
+    extern __shared__ double lds[];
+    const tile tl = tiles[t0 + blockIdx.x];
+    const island_desc di = d[tl.island];
+    if (c.gen >= di.gens) return;
+    eval_tile(di, tl, c, lds);
+}
+
+// Generations g0 .. g1 - 1 (within one batch of random words) of one island
+// that is a single evaluation tile, in one workgroup: k_update's and k_eval's
+// work in turn, with workgroup barriers where their launches would end.
+// Only this workgroup touches the island's X, V, lbX, fit and lbfit, so the
+// barriers order every write before the reads of the next phase.
+__global__ __launch_bounds__(kBlock) void k_gens_one_tile(const island_desc *__restrict__ d,
+                                                          const tile *__restrict__ tiles, uint32_t island,
+                                                          uint32_t tl_index, const uint32_t *__restrict__ words,
+                                                          double *V, eval_ctx c, uint32_t g0, uint32_t g1)
+{
+    extern __shared__ double lds[];
+    const tile tl = tiles[tl_index];
+    const island_desc di = d[island];
+    const uint64_t elems = static_cast<uint64_t>(di.pop) * di.nx;
+    const uint32_t end = min(g1, di.gens);
+    for (uint32_t g = g0; g < end; ++g) {
+        for (uint64_t m = threadIdx.x; m < elems; m += kBlock) update_elem(di, m, words, c.X, V, c.lbX, c.lbfit, g);
+        __syncthreads();
+        eval_ctx cg = c;
+        cg.gen = g;
+        eval_tile(di, tl, cg, lds);
+        __syncthreads();
     }
 }
 
@@ -480,6 +520,7 @@ struct pgh_archi {
     hipGraphExec_t exec = nullptr;
     bool graph_draw_v = false, graph_from_lb = false;
     migr_ctx *migr = nullptr; // set by pgh_archi_set_migration
+    bool one_shot = false;    // pgh_pso_gen_evolve_host's own archipelago
 
     ~pgh_archi()
     {
@@ -538,8 +579,19 @@ pgh_status launch_range(pgh_archi *a, uint32_t i0, uint32_t i1, bool draw_v, boo
     k_vinit<<<blocks_for(total), kBlock, 0, s>>>(d, i0, i1, e0, total, w, X, V, lbX, fit, lbfit, draw_v ? 1u : 0u,
                                                  from_lb ? 1u : 0u);
     PGH_HIP(hipGetLastError());
+    // a single island of one tile, evolved by the one-shot entry point: a
+    // batch of generations per launch (k_gens_one_tile)
+    const bool fused = a->one_shot && i1 == i0 + 1u && nt == 1u;
+    const auto *tl = static_cast<const tile *>(a->tiles.ptr);
     for (uint32_t g = 0; g < gens; ++g) {
-        if (g % kGenBatch == 0u) k_rng<<<ng, kMtJumpBlock, 0, s>>>(d, sd, st, segst, jt, w, sg0, 1u, g, 0u);
+        if (g % kGenBatch == 0u) {
+            k_rng<<<ng, kMtJumpBlock, 0, s>>>(d, sd, st, segst, jt, w, sg0, 1u, g, 0u);
+            if (fused) {
+                k_gens_one_tile<<<1, kBlock, a->lds_bytes, s>>>(d, tl, i0, t0, w, V, make_ctx(a, g), g, g + kGenBatch);
+                PGH_HIP(hipGetLastError());
+            }
+        }
+        if (fused) continue;
         k_update<<<blocks_for(total), kBlock, 0, s>>>(d, i0, i1, e0, total, w, X, V, lbX, lbfit, g);
         k_eval<<<nt, kBlock, a->lds_bytes, s>>>(d, static_cast<const tile *>(a->tiles.ptr), t0, make_ctx(a, g));
         PGH_HIP(hipGetLastError());
@@ -915,9 +967,10 @@ pgh_status pgh_pso_gen_evolve_host(const pgh_udp *udp, const pgh_pso_params *prm
     const double *fs[1] = {F};
     PGH_TRY(pgh_archi_create(&a, &sp, 1u, xs, fs));
     std::unique_ptr<pgh_archi> guard(a);
-    // one island, evolved once: direct launches (the same kernels a batched
-    // evolve captures; capturing and instantiating a graph that runs once
-    // costs more than it saves)
+    // one island, evolved once: direct launches (capturing and instantiating
+    // a graph that runs once costs more than it saves), a batch of
+    // generations per launch when the island is one evaluation tile
+    a->one_shot = true;
     PGH_TRY(pgh_archi_evolve(a, PGH_ARCHI_PER_ISLAND, nullptr));
     return pgh_archi_get(a, 0u, X, F);
 }

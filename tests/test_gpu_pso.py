@@ -39,6 +39,32 @@ def test_host_entry_point():
     assert np.array_equal(Xg, Xo) and np.array_equal(Fg, Fo)
 
 
+@pytest.mark.parametrize("kind,dim,pop,gens", [
+    (P.ROSENBROCK, 10, 64, 100),    # one tile: a batch of generations per launch, batches of 32, 32, 32, 4
+    (P.ROSENBROCK, 7, 256, 33),     # one full tile
+    (P.ROSENBROCK, 7, 300, 40),     # two tiles: one launch per phase and generation
+    (P.RASTRIGIN, 12, 16, 70),
+    (P.ACKLEY, 9, 8, 64),
+    (P.LENNARD_JONES, 5, 12, 45),
+    (P.LENNARD_JONES, 40, 6, 33),
+])
+def test_host_entry_point_matches_archipelago(kind, dim, pop, gens):
+    """The one-shot entry point runs a single-tile island's generations in one
+    workgroup per batch of random words; its result equals the archipelago's
+    launch-per-phase evolve bit for bit (and the oracle's for rosenbrock)."""
+    X0, F0 = O.population_init(kind, dim, pop, 11)
+    prm = P.pso_params(gen=gens, seed=5)
+    Xh, Fh = P.pso_gen_evolve_host(kind, dim, X0, F0, prm)
+    a = P.Archipelago([dict(kind=kind, dim=dim, X0=X0, F0=F0, algo=P.pso_params(gen=gens, seed=5))])
+    a.evolve()
+    Xa, Fa = a.get(0)
+    a.close()
+    assert np.array_equal(Xh, Xa) and np.array_equal(Fh.reshape(-1), Fa.reshape(-1))
+    if kind == P.ROSENBROCK:
+        Xo, Fo = O.pso_gen_evolve(kind, dim, X0, F0, gens, 5)
+        assert np.array_equal(Xh, Xo) and np.array_equal(Fh, Fo)
+
+
 def test_two_evolve_calls_continue_the_stream():
     """A second evolve call re-draws velocities from the same engine and
     starts from the evolved population (pso_gen.cpp:193-200, :526-528)."""
