@@ -457,24 +457,71 @@ __global__ __launch_bounds__(kBlock) void k_eval(const island_desc *__restrict__
 // that is a single evaluation tile, in one workgroup: k_update's and k_eval's
 // work in turn, with workgroup barriers where their launches would end.
 // Only this workgroup touches the island's X, V, lbX, fit and lbfit, so the
-// barriers order every write before the reads of the next phase.
+// barriers order every write before the reads of the next phase.  STAGE: the
+// island is held in LDS for the batch, at byte offset stage_off of the
+// dynamic LDS (after the evaluation's own staging area), read at the start
+// and written back at the end; the arithmetic is the same.
+template <bool STAGE>
 __global__ __launch_bounds__(kBlock) void k_gens_one_tile(const island_desc *__restrict__ d,
                                                           const tile *__restrict__ tiles, uint32_t island,
                                                           uint32_t tl_index, const uint32_t *__restrict__ words,
-                                                          double *V, eval_ctx c, uint32_t g0, uint32_t g1)
+                                                          double *V, eval_ctx c, uint32_t g0, uint32_t g1,
+                                                          uint32_t stage_off)
 {
     extern __shared__ double lds[];
     const tile tl = tiles[tl_index];
     const island_desc di = d[island];
     const uint64_t elems = static_cast<uint64_t>(di.pop) * di.nx;
     const uint32_t end = min(g1, di.gens);
+    if (g0 >= end) return;
+    island_desc dl = di;
+    eval_ctx cl = c;
+    double *Vl = V;
+    double *sX = nullptr, *sV = nullptr, *sL = nullptr, *sF = nullptr, *sLf = nullptr;
+    if constexpr (STAGE) {
+        sX = lds + stage_off / 8u;
+        sV = sX + elems;
+        sL = sV + elems;
+        sF = sL + elems;
+        sLf = sF + di.pop;
+        for (uint64_t m = threadIdx.x; m < elems; m += kBlock) {
+            sX[m] = c.X[di.xoff + m];
+            sV[m] = V[di.xoff + m];
+            sL[m] = c.lbX[di.xoff + m];
+        }
+        for (uint32_t p = threadIdx.x; p < di.pop; p += kBlock) {
+            sF[p] = c.fit[di.foff + p];
+            sLf[p] = c.lbfit[di.foff + p];
+        }
+        __syncthreads();
+        dl.xoff = 0u;
+        dl.foff = 0u;
+        dl.toff = 0u;
+        cl.X = sX;
+        cl.lbX = sL;
+        cl.fit = sF;
+        cl.lbfit = sLf;
+        cl.trace = c.trace ? c.trace + di.toff : nullptr;
+        Vl = sV;
+    }
     for (uint32_t g = g0; g < end; ++g) {
-        for (uint64_t m = threadIdx.x; m < elems; m += kBlock) update_elem(di, m, words, c.X, V, c.lbX, c.lbfit, g);
+        for (uint64_t m = threadIdx.x; m < elems; m += kBlock) update_elem(dl, m, words, cl.X, Vl, cl.lbX, cl.lbfit, g);
         __syncthreads();
-        eval_ctx cg = c;
+        eval_ctx cg = cl;
         cg.gen = g;
-        eval_tile(di, tl, cg, lds);
+        eval_tile(dl, tl, cg, lds);
         __syncthreads();
+    }
+    if constexpr (STAGE) {
+        for (uint64_t m = threadIdx.x; m < elems; m += kBlock) {
+            c.X[di.xoff + m] = sX[m];
+            V[di.xoff + m] = sV[m];
+            c.lbX[di.xoff + m] = sL[m];
+        }
+        for (uint32_t p = threadIdx.x; p < di.pop; p += kBlock) {
+            c.fit[di.foff + p] = sF[p];
+            c.lbfit[di.foff + p] = sLf[p];
+        }
     }
 }
 
@@ -582,12 +629,26 @@ pgh_status launch_range(pgh_archi *a, uint32_t i0, uint32_t i1, bool draw_v, boo
     // a single island of one tile, evolved by the one-shot entry point: a
     // batch of generations per launch (k_gens_one_tile)
     const bool fused = a->one_shot && i1 == i0 + 1u && nt == 1u;
+    // the island in LDS when it fits beside the evaluation's staging area
+    // within 64 KiB: X, V, lbX (pop x nx) and fit, lbfit (pop)
+    const uint64_t stage_off = align_up(a->lds_bytes, 256u);
+    uint64_t stage_bytes = 0u;
+    if (fused) {
+        const island_desc &di = a->h_desc[i0];
+        const uint64_t need = (3u * static_cast<uint64_t>(di.pop) * di.nx + 2u * di.pop) * 8u;
+        if (stage_off + need <= 65536u) stage_bytes = need;
+    }
     const auto *tl = static_cast<const tile *>(a->tiles.ptr);
     for (uint32_t g = 0; g < gens; ++g) {
         if (g % kGenBatch == 0u) {
             k_rng<<<ng, kMtJumpBlock, 0, s>>>(d, sd, st, segst, jt, w, sg0, 1u, g, 0u);
-            if (fused) {
-                k_gens_one_tile<<<1, kBlock, a->lds_bytes, s>>>(d, tl, i0, t0, w, V, make_ctx(a, g), g, g + kGenBatch);
+            if (fused && stage_bytes) {
+                k_gens_one_tile<true><<<1, kBlock, stage_off + stage_bytes, s>>>(d, tl, i0, t0, w, V, make_ctx(a, g), g,
+                                                                                 g + kGenBatch, stage_off);
+                PGH_HIP(hipGetLastError());
+            } else if (fused) {
+                k_gens_one_tile<false><<<1, kBlock, a->lds_bytes, s>>>(d, tl, i0, t0, w, V, make_ctx(a, g), g,
+                                                                       g + kGenBatch, 0u);
                 PGH_HIP(hipGetLastError());
             }
         }
