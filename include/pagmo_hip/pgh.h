@@ -336,7 +336,10 @@ pgh_status pgh_archi_set_migration(pgh_archi *a, int type, int handling, const u
  * (count x nx) and F (count) when non-null.  Synchronous. */
 pgh_status pgh_archi_get_migrants(pgh_archi *a, uint64_t k, uint64_t *count, double *X, double *F);
 
-/* Single-island convenience: pso_gen(prm).evolve(pop) with host X/F in/out. */
+/* Single-island convenience: pso_gen(prm).evolve(pop) with host X/F in/out
+ * (pso_gen.cpp:120-530).  Results equal an archipelago of that one island
+ * evolved once; an island that is one evaluation tile runs a batch of 32
+ * generations per launch in one workgroup.  Synchronous. */
 pgh_status pgh_pso_gen_evolve_host(const pgh_udp *udp, const pgh_pso_params *prm, double *X, double *F,
                                    uint64_t pop, uint64_t nx);
 
