@@ -461,17 +461,42 @@ __global__ __launch_bounds__(kBlock) void k_eval(const island_desc *__restrict__
 // island is held in LDS for the batch, at byte offset stage_off of the
 // dynamic LDS (after the evaluation's own staging area), read at the start
 // and written back at the end; the arithmetic is the same.
+//
+// With a second workgroup (gridDim.x == 2, an island whose generator is one
+// segment), that workgroup hands out the next batch's words (g1 ..) into
+// words_next, as k_rng's phase 1 would after this launch: the two workgroups
+// share no data, and the next launch reads those words.
 template <bool STAGE>
 __global__ __launch_bounds__(kBlock) void k_gens_one_tile(const island_desc *__restrict__ d,
                                                           const tile *__restrict__ tiles, uint32_t island,
                                                           uint32_t tl_index, const uint32_t *__restrict__ words,
                                                           double *V, eval_ctx c, uint32_t g0, uint32_t g1,
-                                                          uint32_t stage_off)
+                                                          uint32_t stage_off, uint32_t *__restrict__ words_next,
+                                                          mt_state *__restrict__ st, mt_state *__restrict__ segst)
 {
     extern __shared__ double lds[];
     const tile tl = tiles[tl_index];
     const island_desc di = d[island];
     const uint64_t elems = static_cast<uint64_t>(di.pop) * di.nx;
+    if (blockIdx.x == 1u) {
+        __shared__ uint32_t ring[kMtRing];
+        const uint32_t first = g1;
+        if (first >= di.gens) return;
+        const uint32_t last = min(first + di.gps, di.gens);
+        const uint64_t W = 4u * elems;
+        mt_state *mine = &segst[di.seg0];
+        uint64_t K, C;
+        load_state(mine, ring, K, C);
+        const uint64_t n = (last - first) * W;
+        mt_run(ring, K, C, n, words_next + di.woff);
+        C += n;
+        if (last == di.gens) {
+            store_state(&st[island], ring, K, C);
+        } else if (first + kGenBatch < di.gens) {
+            store_state(mine, ring, K, C);
+        }
+        return;
+    }
     const uint32_t end = min(g1, di.gens);
     if (g0 >= end) return;
     island_desc dl = di;
@@ -568,6 +593,7 @@ struct pgh_archi {
     bool graph_draw_v = false, graph_from_lb = false;
     migr_ctx *migr = nullptr; // set by pgh_archi_set_migration
     bool one_shot = false;    // pgh_pso_gen_evolve_host's own archipelago
+    uint64_t words_half = 0;  // offset of a lone island's second batch of words (0: none)
 
     ~pgh_archi()
     {
@@ -639,16 +665,24 @@ pgh_status launch_range(pgh_archi *a, uint32_t i0, uint32_t i1, bool draw_v, boo
         if (stage_off + need <= 65536u) stage_bytes = need;
     }
     const auto *tl = static_cast<const tile *>(a->tiles.ptr);
+    // ... and the next batch's words handed out beside it, into the other
+    // half of the word buffer, when the generator is one segment
+    const bool overlap = fused && a->words_half != 0u && a->h_desc[i0].segs == 1u;
     for (uint32_t g = 0; g < gens; ++g) {
         if (g % kGenBatch == 0u) {
-            k_rng<<<ng, kMtJumpBlock, 0, s>>>(d, sd, st, segst, jt, w, sg0, 1u, g, 0u);
-            if (fused && stage_bytes) {
-                k_gens_one_tile<true><<<1, kBlock, stage_off + stage_bytes, s>>>(d, tl, i0, t0, w, V, make_ctx(a, g), g,
-                                                                                 g + kGenBatch, stage_off);
-                PGH_HIP(hipGetLastError());
-            } else if (fused) {
-                k_gens_one_tile<false><<<1, kBlock, a->lds_bytes, s>>>(d, tl, i0, t0, w, V, make_ctx(a, g), g,
-                                                                       g + kGenBatch, 0u);
+            const uint32_t b = g / kGenBatch;
+            if (!overlap || g == 0u) k_rng<<<ng, kMtJumpBlock, 0, s>>>(d, sd, st, segst, jt, w, sg0, 1u, g, 0u);
+            if (fused) {
+                const uint32_t *wc = w + (overlap ? (b % 2u) * a->words_half : 0u);
+                uint32_t *wn = w + (overlap ? ((b + 1u) % 2u) * a->words_half : 0u);
+                const uint32_t grid = overlap && g + kGenBatch < gens ? 2u : 1u;
+                if (stage_bytes) {
+                    k_gens_one_tile<true><<<grid, kBlock, stage_off + stage_bytes, s>>>(
+                        d, tl, i0, t0, wc, V, make_ctx(a, g), g, g + kGenBatch, stage_off, wn, st, segst);
+                } else {
+                    k_gens_one_tile<false><<<grid, kBlock, a->lds_bytes, s>>>(d, tl, i0, t0, wc, V, make_ctx(a, g), g,
+                                                                              g + kGenBatch, 0u, wn, st, segst);
+                }
                 PGH_HIP(hipGetLastError());
             }
         }
@@ -732,7 +766,13 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
         di.eoff = eoff;
         xoff += align_up(elems, 32u); // 256-byte aligned rows blocks
         foff += align_up(sp.pop, 32u);
-        woff += align_up(elems * 4u * kGenBatch, 64u); // the words of a batch of generations
+        // the words of a batch of generations; a lone island of up to 2^22
+        // words per batch gets room for a second batch, which the one-shot
+        // evolve fills while it evolves the first (k_gens_one_tile)
+        const uint64_t wbatch = align_up(elems * 4u * kGenBatch, 64u);
+        const bool second = n_islands == 1u && wbatch <= (uint64_t(1) << 22);
+        woff += second ? 2u * wbatch : wbatch;
+        if (second) a->words_half = wbatch;
         toff += static_cast<uint64_t>(q.gen) * sp.pop;
         eoff += elems;
         a->max_gens = std::max(a->max_gens, q.gen);
