@@ -1,9 +1,12 @@
 // capi.hip -- the extern "C" entry points of libpagmo_hip.so: validation,
 // error reporting, host staging and dispatch to the kernel launchers.
+#include <algorithm>
 #include <cfloat>
 #include <cstring>
 #include <limits>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "pgh_internal.hpp"
 
@@ -55,6 +58,80 @@ dev_buf::~dev_buf()
     // Buffers live until their host thread ends; at process exit the HIP
     // runtime may already be gone, so the release is best effort.
     if (ptr) (void)hipFree(ptr);
+}
+
+pgh_status pinned_buf::reserve(size_t bytes)
+{
+    if (!done[0]) {
+        PGH_HIP(hipEventCreateWithFlags(&done[0], hipEventDisableTiming));
+        PGH_HIP(hipEventCreateWithFlags(&done[1], hipEventDisableTiming));
+    }
+    if (bytes <= cap) return PGH_OK;
+    if (ptr) {
+        (void)hipHostFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+    PGH_HIP(hipHostMalloc(&ptr, bytes, hipHostMallocDefault));
+    cap = bytes;
+    return PGH_OK;
+}
+
+pinned_buf::~pinned_buf()
+{
+    // best effort, as for dev_buf
+    if (ptr) (void)hipHostFree(ptr);
+    if (done[0]) (void)hipEventDestroy(done[0]);
+    if (done[1]) (void)hipEventDestroy(done[1]);
+}
+
+namespace
+{
+// chunk of a pinned upload and the size from which uploads use them
+constexpr size_t kPinChunk = size_t(16) << 20, kPinMin = size_t(16) << 20;
+
+// memcpy split over up to `threads` host threads
+void parallel_copy(void *dst, const void *src, size_t bytes, unsigned threads)
+{
+    if (threads <= 1u || bytes < (size_t(1) << 20)) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t part = (bytes / threads + 63u) & ~size_t(63);
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < threads; ++t) {
+        const size_t off = t * part;
+        if (off >= bytes) break;
+        const size_t len = std::min(part, bytes - off);
+        pool.emplace_back([=] { std::memcpy(static_cast<char *>(dst) + off, static_cast<const char *>(src) + off, len); });
+    }
+    std::memcpy(dst, src, std::min(part, bytes));
+    for (auto &th : pool) th.join();
+}
+} // namespace
+
+pgh_status upload_host(thread_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    if (bytes < kPinMin) {
+        PGH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        return PGH_OK;
+    }
+    PGH_TRY(ctx->pin.reserve(2u * kPinChunk));
+    const unsigned hw = std::thread::hardware_concurrency();
+    const unsigned threads = std::max(1u, std::min(8u, hw / 2u));
+    auto *stage = static_cast<char *>(ctx->pin.ptr);
+    size_t i = 0;
+    for (size_t off = 0; off < bytes; off += kPinChunk, ++i) {
+        const size_t len = std::min(kPinChunk, bytes - off);
+        char *half = stage + (i % 2u) * kPinChunk;
+        // the DMA that last read this half (in this call or an earlier one)
+        // has finished; an event never recorded is complete
+        PGH_HIP(hipEventSynchronize(ctx->pin.done[i % 2u]));
+        parallel_copy(half, static_cast<const char *>(src) + off, len, threads);
+        PGH_HIP(hipMemcpyAsync(static_cast<char *>(dst) + off, half, len, hipMemcpyHostToDevice, ctx->stream));
+        PGH_HIP(hipEventRecord(ctx->pin.done[i % 2u], ctx->stream));
+    }
+    return PGH_OK;
 }
 
 pgh_status get_thread_ctx(thread_ctx **out)
@@ -334,7 +411,7 @@ pgh_status pgh_bfe_eval_host(const pgh_udp *udp, const double *dvs, uint64_t dvs
     PGH_TRY(ctx->a.reserve(in_bytes));
     PGH_TRY(ctx->b.reserve(out_bytes));
     PGH_TRY(stage_centers(p, udp, ctx->c, ctx->stream));
-    PGH_HIP(hipMemcpyAsync(ctx->a.ptr, dvs, in_bytes, hipMemcpyHostToDevice, ctx->stream));
+    PGH_TRY(upload_host(ctx, ctx->a.ptr, dvs, in_bytes));
     PGH_TRY(apply_shift(p, static_cast<double *>(ctx->a.ptr), dvs_len, ctx));
     PGH_TRY(launch_fitness(p, static_cast<const double *>(ctx->a.ptr), n_dvs, static_cast<double *>(ctx->b.ptr),
                            ctx->stream));
@@ -374,7 +451,7 @@ pgh_status pgh_batch_gradient_host(const pgh_udp *udp, const double *dvs, uint64
     PGH_TRY(ctx->a.reserve(bytes));
     PGH_TRY(ctx->b.reserve(bytes));
     PGH_TRY(ctx->d.reserve(n_dvs * sizeof(double)));
-    PGH_HIP(hipMemcpyAsync(ctx->a.ptr, dvs, bytes, hipMemcpyHostToDevice, ctx->stream));
+    PGH_TRY(upload_host(ctx, ctx->a.ptr, dvs, bytes));
     // translate forwards the gradient at x - t (translate.cpp:245-249)
     PGH_TRY(apply_shift(p, static_cast<double *>(ctx->a.ptr), dvs_len, ctx));
     PGH_TRY(launch_gradient(p, static_cast<const double *>(ctx->a.ptr), n_dvs, static_cast<double *>(ctx->b.ptr),
@@ -440,7 +517,7 @@ pgh_status pgh_local_descent_host(const pgh_udp *udp, double *dvs, uint64_t dvs_
     }
     double *x = static_cast<double *>(ctx->a.ptr);
     double *f = static_cast<double *>(ctx->d.ptr);
-    PGH_HIP(hipMemcpyAsync(x, dvs, x_bytes, hipMemcpyHostToDevice, ctx->stream));
+    PGH_TRY(upload_host(ctx, x, dvs, x_bytes));
     PGH_TRY(launch_local_descent(p, x, n_dvs, f, dlb, dub, iters, step0, static_cast<double *>(ctx->b.ptr),
                                  ctx->stream));
     PGH_HIP(hipMemcpyAsync(dvs, x, x_bytes, hipMemcpyDeviceToHost, ctx->stream));

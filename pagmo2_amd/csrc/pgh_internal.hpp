@@ -114,11 +114,26 @@ struct dev_buf {
     ~dev_buf();
 };
 
+// Grow-only pinned host memory (hipHostMalloc) with two events, one per half.
+struct pinned_buf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    hipEvent_t done[2] = {nullptr, nullptr};
+    pgh_status reserve(size_t bytes);
+    ~pinned_buf();
+};
+
 // Per-host-thread stream and staging buffers for the _host entry points.
 struct thread_ctx {
     hipStream_t stream = nullptr;
     dev_buf a, b, c, d, e;
+    pinned_buf pin;
 };
+
+// Host-to-device copy of a host buffer on the context's stream.  Large
+// buffers go through two pinned chunks filled by host threads, each sent by
+// DMA while the other fills; small ones are one hipMemcpyAsync.
+pgh_status upload_host(thread_ctx *ctx, void *dst, const void *src, size_t bytes);
 pgh_status get_thread_ctx(thread_ctx **ctx);
 
 } // namespace pgh

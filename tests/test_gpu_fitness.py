@@ -162,6 +162,28 @@ def test_host_entry_point_matches_device():
     assert np.array_equal(h, gpu_bfe(P.ACKLEY, 100, x))
 
 
+@pytest.mark.parametrize("rows", [2048, 5243, 9000])
+def test_large_host_batch_goes_through_pinned_chunks(rows):
+    """Batches of 16 MiB and more are uploaded through two pinned 16 MiB
+    chunks filled by host threads (one chunk exactly, a ragged last chunk,
+    and more than two chunks so that both halves are reused); the result is
+    the device-resident evaluation's, and a second call reuses the buffers."""
+    x = np.random.default_rng(rows).uniform(-5.12, 5.12, (rows, 1024))
+    want = gpu_bfe(P.RASTRIGIN, 1024, x)
+    assert np.array_equal(P.bfe_eval_host(P.RASTRIGIN, 1024, x), want)
+    y = x[::-1].copy()
+    assert np.array_equal(P.bfe_eval_host(P.RASTRIGIN, 1024, y), want[::-1])
+
+
+def test_large_host_gradient_batch_goes_through_pinned_chunks():
+    x = np.random.default_rng(4).uniform(-2.0, 2.0, (5000, 1000))
+    d = dev(x)
+    g = torch.empty_like(d)
+    P.batch_gradient(P.ROSENBROCK, 1000, d, g)
+    torch.cuda.synchronize()
+    assert np.array_equal(P.batch_gradient_host(P.ROSENBROCK, 1000, x), g.cpu().numpy())
+
+
 # ---- gradients ---------------------------------------------------------------
 
 @pytest.mark.parametrize("dim,n", [(2, 3), (10, 64), (501, 17), (64, 257), (1000, 300), (1001, 513)])
