@@ -3,8 +3,10 @@
 // once per device and set of distances and kept for the life of the process.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <map>
+#include <thread>
 #include <mutex>
 #include <utility>
 #include <vector>
@@ -49,6 +51,7 @@ pgh_status jump_table_bytes(const std::vector<uint64_t> &dists, std::vector<uint
     if (!mtj::the_field().ok) return fail(PGH_DEVICE_ERROR, "mt19937 characteristic polynomial not found");
     const size_t head = jump_table_head(dists.size());
     h.assign(head + dists.size() * dev::kMtIdxStride * 2u, 0u);
+    mtj::prefetch_jump_polys(dists, std::min(8u, std::thread::hardware_concurrency()));
     std::vector<uint16_t> lst;
     for (size_t k = 0; k < dists.size(); ++k) {
         PGH_TRY(index_list(dists[k], lst));

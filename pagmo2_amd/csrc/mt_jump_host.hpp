@@ -21,10 +21,13 @@
 // 19967 left over are reduced one by one.
 #pragma once
 
+#include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <map>
 #include <mutex>
 #include <random>
+#include <thread>
 #include <vector>
 
 #if defined(__x86_64__)
@@ -302,15 +305,54 @@ inline const field &the_field()
     return f;
 }
 
+struct poly_cache {
+    std::mutex mu;
+    std::map<uint64_t, poly> map; // references stay valid as entries are added
+};
+
+inline poly_cache &the_cache()
+{
+    static poly_cache c;
+    return c;
+}
+
 // z^J mod phi, memoised (the same few distances recur).
 inline const poly &jump_poly(uint64_t J)
 {
-    static std::mutex mu;
-    static std::map<uint64_t, poly> cache;
-    std::lock_guard<std::mutex> lk(mu);
-    auto it = cache.find(J);
-    if (it != cache.end()) return it->second;
-    return cache.emplace(J, the_field().zpow(J)).first->second;
+    poly_cache &c = the_cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto it = c.map.find(J);
+    if (it != c.map.end()) return it->second;
+    return c.map.emplace(J, the_field().zpow(J)).first->second;
+}
+
+// Computes the polynomials of the distances not yet memoised on up to
+// `threads` host threads (each z^J is independent), so that the jump_poly
+// calls that follow find them.
+inline void prefetch_jump_polys(const std::vector<uint64_t> &Js, unsigned threads)
+{
+    const field &f = the_field();
+    poly_cache &c = the_cache();
+    std::vector<uint64_t> todo;
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        for (uint64_t J : Js) {
+            if (!c.map.count(J) && std::find(todo.begin(), todo.end(), J) == todo.end()) todo.push_back(J);
+        }
+    }
+    if (todo.size() < 2u || threads < 2u) return; // jump_poly computes them as they come
+    std::vector<poly> res(todo.size());
+    std::atomic<size_t> next{0};
+    auto work = [&] {
+        for (size_t i = next++; i < todo.size(); i = next++) res[i] = f.zpow(todo[i]);
+    };
+    std::vector<std::thread> pool;
+    const size_t n = std::min<size_t>(threads, todo.size());
+    for (size_t t = 1; t < n; ++t) pool.emplace_back(work);
+    work();
+    for (auto &th : pool) th.join();
+    std::lock_guard<std::mutex> lk(c.mu);
+    for (size_t i = 0; i < todo.size(); ++i) c.map.emplace(todo[i], std::move(res[i]));
 }
 
 } // namespace mtj

@@ -112,6 +112,15 @@ int main()
         std::printf("FAIL large powers\n");
         ++failures;
     }
+    // distances computed on several threads equal those computed one by one
+    const std::vector<uint64_t> Jp = {(1ull << 33) + 7u, 3ull << 30, 123456789012ull, 3ull << 30, 98765432109ull};
+    pgh::mtj::prefetch_jump_polys(Jp, 4u);
+    for (uint64_t J : Jp) {
+        if (pgh::mtj::jump_poly(J) != f.zpow(J)) {
+            std::printf("FAIL prefetched z^%llu\n", (unsigned long long)J);
+            ++failures;
+        }
+    }
     if (failures) return 1;
     std::printf("ok\n");
     return 0;
