@@ -27,6 +27,7 @@
 // batched gradient for a population between two evolve() calls.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <stdexcept>
@@ -230,19 +231,41 @@ private:
     }
 };
 
-// Gradients of a whole population: n_dvs x nx (the dense pattern of
-// problem.cpp:132-140 with nf == 1), row-major.  rosenbrock and rastrigin
-// follow their UDP's gradient(); lennard_jones uses the library's adjoint
-// kernel.  Like problem::gradient for a UDP without one, other problems throw.
+// Gradients of a whole population: one problem::gradient vector per
+// decision vector of dvs (row-major; gradient_sparsity() order, which is the
+// dense 1 x nx pattern of problem.cpp:132-140 for the device kernels).
+//   - rosenbrock and rastrigin (their UDP's gradient()) and lennard_jones
+//     (the library's adjoint kernel; pagmo's UDP has none), also inside a
+//     translate, run on the device.  The problem's gradient counter is not
+//     advanced there: pagmo's problem offers no public way to do so.
+//   - every other problem goes through problem::gradient row by row, with
+//     its semantics (src/problem.cpp:456-467): the length check, one gevals
+//     per row, and not_implemented_error for a UDP without a gradient.
 inline pagmo::vector_double batch_gradient(const pagmo::problem &p, const pagmo::vector_double &dvs)
 {
-    pgh_udp u;
-    if (!describe(p, u)) {
-        pagmo_throw(std::invalid_argument, "batch_gradient: no device gradient for the problem '" + p.get_name() + "'");
+    const auto nx = p.get_nx();
+    if (nx == 0u || dvs.size() % nx) {
+        pagmo_throw(std::invalid_argument, "Invalid argument for a batch gradient evaluation: the length of the "
+                                           "vector representing the decision vectors, "
+                                               + std::to_string(dvs.size())
+                                               + ", is not an exact multiple of the dimension of the problem, "
+                                               + std::to_string(nx));
     }
-    pagmo::vector_double g(dvs.size());
-    detail::check(pgh_batch_gradient_host(&u, dvs.data(), dvs.size(), g.data(), g.size(), nullptr));
-    return g;
+    pgh_udp u;
+    if (describe(p, u)
+        && (u.kind == PGH_ROSENBROCK || u.kind == PGH_RASTRIGIN || u.kind == PGH_LENNARD_JONES)) {
+        pagmo::vector_double g(dvs.size());
+        detail::check(pgh_batch_gradient_host(&u, dvs.data(), dvs.size(), g.data(), g.size(), nullptr));
+        return g;
+    }
+    pagmo::vector_double out, row(nx);
+    for (std::size_t i = 0; i < dvs.size() / nx; ++i) {
+        std::copy(dvs.begin() + static_cast<std::ptrdiff_t>(i * nx),
+                  dvs.begin() + static_cast<std::ptrdiff_t>((i + 1u) * nx), row.begin());
+        const auto g = p.gradient(row);
+        out.insert(out.end(), g.begin(), g.end());
+    }
+    return out;
 }
 
 // Memetic local step: `iters` rounds of the projected gradient descent of
@@ -306,15 +329,10 @@ inline pagmo::fnds_return_type fast_non_dominated_sorting(const std::vector<pagm
     const auto n = points.size();
     const auto m = n ? points[0].size() : 0u;
     const auto flat = detail::flatten_objectives(points, m);
+    // the calling thread's workspace: kept between calls, so a generation
+    // loop (nsga2.cpp:183) allocates device memory and a stream once
     pgh_fnds_ws *ws = nullptr;
-    detail::check(pgh_fnds_create(&ws));
-    struct guard {
-        pgh_fnds_ws *w;
-        ~guard()
-        {
-            pgh_fnds_destroy(w);
-        }
-    } g{ws};
+    detail::check(pgh_fnds_thread_workspace(&ws));
     detail::check(pgh_fnds_run_host(ws, flat.empty() ? nullptr : flat.data(), n, m));
     uint64_t n_fronts = 0, n_dom = 0;
     detail::check(pgh_fnds_sizes(ws, &n_fronts, &n_dom));

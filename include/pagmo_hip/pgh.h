@@ -157,6 +157,11 @@ pgh_status pgh_local_descent_host(const pgh_udp *udp, double *dvs, uint64_t dvs_
  *      run (async) -> sizes (synchronises) -> copy (to host or device). ---- */
 typedef struct pgh_fnds_ws pgh_fnds_ws;
 pgh_status pgh_fnds_create(pgh_fnds_ws **ws);
+/* The calling host thread's own workspace, made on its first call and kept
+ * until the thread ends (never at process exit), so that a caller sorting
+ * every generation (nsga2.cpp:183, nspso.cpp:162,377, maco.cpp:176) allocates
+ * once per thread.  pgh_fnds_destroy refuses it. */
+pgh_status pgh_fnds_thread_workspace(pgh_fnds_ws **ws);
 pgh_status pgh_fnds_destroy(pgh_fnds_ws *ws);
 /* INVALID_ARGUMENT when n < 2, with the reference's message
  * (multi_objective.cpp:203-207). */

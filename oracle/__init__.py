@@ -16,7 +16,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# PGH_ORACLE_LIB names another build of the same sources (the sanitized one
+# of tests/test_sanitize_cpu.py)
+LIB_PATH = os.environ.get("PGH_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 ROSENBROCK, RASTRIGIN, ACKLEY, SCHWEFEL, GRIEWANK, LENNARD_JONES, MO_SPHERE, ZDT, DTLZ = range(1, 10)
 

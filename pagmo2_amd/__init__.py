@@ -2,8 +2,9 @@
 
 The product is libpagmo_hip.so (HIP kernels for gfx950 behind the C ABI of
 include/pagmo_hip/pgh.h) and the C++ drop-in UDBFE include/pagmo_hip/hip_bfe.hpp.
-This Python package only loads the library (pagmo2_amd._capi) and builds it
-(pagmo2_amd.build).
+This Python package only loads the library (pagmo2_amd._capi), builds it
+(pagmo2_amd.build) and describes the uneven-archipelago workload
+(pagmo2_amd.workloads).
 """
 from . import _capi  # noqa: F401
 from ._capi import *  # noqa: F401,F403

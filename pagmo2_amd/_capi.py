@@ -9,11 +9,14 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpagmo_hip.so")
+# PGH_LIB names another build of the same sources (the host-sanitized one of
+# tests/test_sanitize_cpu.py)
+LIB_PATH = os.environ.get("PGH_LIB") or os.path.join(HERE, "libpagmo_hip.so")
 
 OK, INVALID_ARGUMENT, OVERFLOW, UNSUPPORTED, DEVICE_ERROR, NO_DEVICE, OUT_OF_MEMORY = range(7)
 ROSENBROCK, RASTRIGIN, ACKLEY, SCHWEFEL, GRIEWANK, LENNARD_JONES, MO_SPHERE, ZDT, DTLZ = range(1, 10)
@@ -23,7 +26,7 @@ ARCHI_BATCHED, ARCHI_PER_ISLAND = 0, 1
 EXPORTS = [
     "pgh_last_error", "pgh_abi_version", "pgh_device_available", "pgh_problem_dims", "pgh_supported",
     "pgh_bfe_eval", "pgh_bfe_eval_host", "pgh_batch_gradient", "pgh_batch_gradient_host",
-    "pgh_fnds_create", "pgh_fnds_destroy", "pgh_fnds_run", "pgh_fnds_run_host", "pgh_fnds_sizes",
+    "pgh_fnds_create", "pgh_fnds_thread_workspace", "pgh_fnds_destroy", "pgh_fnds_run", "pgh_fnds_run_host", "pgh_fnds_sizes",
     "pgh_fnds_copy", "pgh_fnds_set_peel_blocks", "pgh_fnds_tune", "pgh_lj_set_groups", "pgh_lj_tune",
     "pgh_crowding_distance", "pgh_crowding_distance_host", "pgh_sort_population_mo", "pgh_sort_population_mo_host",
     "pgh_select_best_N_mo", "pgh_select_best_N_mo_host",
@@ -138,6 +141,7 @@ def _declare(L):
         "pgh_local_descent": [udp_p, vp, u64, u64, vp, vp, vp, u32, C.c_double, vp, u64, vp],
         "pgh_local_descent_host": [udp_p, vp, u64, vp, u64, vp, vp, u32, C.c_double],
         "pgh_fnds_create": [C.POINTER(vp)],
+        "pgh_fnds_thread_workspace": [C.POINTER(vp)],
         "pgh_fnds_destroy": [vp],
         "pgh_fnds_run": [vp, vp, u64, u64, vp],
         "pgh_fnds_run_host": [vp, vp, u64, u64],
@@ -296,7 +300,11 @@ class Fnds:
             lib().pgh_fnds_destroy(self.h)
             self.h = None
 
-    __del__ = close
+    def __del__(self):
+        # at interpreter exit the process end reclaims device memory; no HIP
+        # call is made from the finalizer then
+        if not sys.is_finalizing():
+            self.close()
 
     def run(self, objs, stream=None):
         n, m = objs.shape
@@ -455,7 +463,10 @@ class Archipelago:
             lib().pgh_archi_destroy(self.h)
             self.h = None
 
-    __del__ = close
+    def __del__(self):
+        # as for Fnds: nothing is released while the interpreter finalizes
+        if not sys.is_finalizing():
+            self.close()
 
     def evolve(self, mode=ARCHI_BATCHED, stream=None):
         check(lib().pgh_archi_evolve(self.h, mode, _stream(stream) if stream is not None else None))

@@ -58,10 +58,51 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+# Host code with AddressSanitizer and UndefinedBehaviorSanitizer (device code
+# as in the product build): libpagmo_hip_san.so, for the CPU-only validation
+# runs of tests/test_sanitize_cpu.py, which preload clang's ASan runtime.
+SAN_OBJ = os.path.join(HERE, "_build_san")
+SAN_LIB = os.path.join(HERE, "libpagmo_hip_san.so")
+SAN_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer"]
+CLANGXX = os.environ.get("PGH_CLANGXX", "/opt/rocm/llvm/bin/clang++")
+
+
+def _compile_san(src: str) -> str:
+    obj = os.path.join(SAN_OBJ, os.path.basename(src) + ".o")
+    flags = [f if f != "-O3" else "-O1" for f in HIPFLAGS]
+    cmd = [HIPCC, *flags, "-g", *SAN_HOST, "-c", src, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc (sanitized host) failed for {src}:\n{r.stdout}\n{r.stderr}")
+    return obj
+
+
+def build_sanitized(verbose: bool = False) -> str:
+    os.makedirs(SAN_OBJ, exist_ok=True)
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        objs = list(ex.map(_compile_san, srcs))
+    tmp = SAN_LIB + ".tmp"
+    # the host link: clang++ with the sanitizers' shared runtime (preloaded
+    # by the process that loads the library)
+    cmd = [CLANGXX, "-shared", "-fPIC", "-fsanitize=address,undefined", "-shared-libsan", "-o", tmp, *objs,
+           "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"sanitized link failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, SAN_LIB)
+    if verbose:
+        print(f"built {SAN_LIB}")
+    return SAN_LIB
+
+
 def clean() -> None:
     shutil.rmtree(OBJ, ignore_errors=True)
-    if os.path.exists(LIB):
-        os.remove(LIB)
+    shutil.rmtree(SAN_OBJ, ignore_errors=True)
+    for lib in (LIB, SAN_LIB):
+        if os.path.exists(lib):
+            os.remove(lib)
 
 
 if __name__ == "__main__":

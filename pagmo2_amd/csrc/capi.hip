@@ -1,10 +1,15 @@
 // capi.hip -- the extern "C" entry points of libpagmo_hip.so: validation,
 // error reporting, host staging and dispatch to the kernel launchers.
+#include <pthread.h>
+
 #include <algorithm>
 #include <cfloat>
 #include <cstring>
+#include <exception>
 #include <limits>
+#include <new>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -27,6 +32,24 @@ pgh_status fail(pgh_status st, const std::string &msg)
 {
     g_last_error = msg;
     return st;
+}
+
+pgh_status api_exception() noexcept
+{
+    try {
+        throw;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host memory exhausted";
+        return PGH_OUT_OF_MEMORY;
+    } catch (const std::exception &e) {
+        try {
+            g_last_error = std::string("host error: ") + e.what();
+        } catch (...) {
+        }
+        return PGH_DEVICE_ERROR;
+    } catch (...) {
+        return PGH_DEVICE_ERROR;
+    }
 }
 
 pgh_status hip_fail(hipError_t e, const char *what)
@@ -55,8 +78,8 @@ pgh_status dev_buf::reserve(size_t bytes)
 
 dev_buf::~dev_buf()
 {
-    // Buffers live until their host thread ends; at process exit the HIP
-    // runtime may already be gone, so the release is best effort.
+    // Runs when the owner is released explicitly (an archipelago, a
+    // workspace) or when a thread_ctx's thread ends, never from exit().
     if (ptr) (void)hipFree(ptr);
 }
 
@@ -79,7 +102,7 @@ pgh_status pinned_buf::reserve(size_t bytes)
 
 pinned_buf::~pinned_buf()
 {
-    // best effort, as for dev_buf
+    // as for dev_buf
     if (ptr) (void)hipHostFree(ptr);
     if (done[0]) (void)hipEventDestroy(done[0]);
     if (done[1]) (void)hipEventDestroy(done[1]);
@@ -98,14 +121,23 @@ void parallel_copy(void *dst, const void *src, size_t bytes, unsigned threads)
         return;
     }
     const size_t part = (bytes / threads + 63u) & ~size_t(63);
+    const auto copy_part = [=](size_t off) {
+        if (off < bytes) {
+            std::memcpy(static_cast<char *>(dst) + off, static_cast<const char *>(src) + off,
+                        std::min(part, bytes - off));
+        }
+    };
     std::vector<std::thread> pool;
-    for (unsigned t = 1; t < threads; ++t) {
-        const size_t off = t * part;
-        if (off >= bytes) break;
-        const size_t len = std::min(part, bytes - off);
-        pool.emplace_back([=] { std::memcpy(static_cast<char *>(dst) + off, static_cast<const char *>(src) + off, len); });
+    unsigned t = 1;
+    try {
+        pool.reserve(threads);
+        for (; t < threads && t * part < bytes; ++t) pool.emplace_back(copy_part, t * part);
+    } catch (...) {
+        // no more threads (or no memory for them): this thread copies the
+        // parts that have none
     }
-    std::memcpy(dst, src, std::min(part, bytes));
+    for (unsigned u = t; u < threads; ++u) copy_part(u * part);
+    copy_part(0);
     for (auto &th : pool) th.join();
 }
 } // namespace
@@ -134,17 +166,59 @@ pgh_status upload_host(thread_ctx *ctx, void *dst, const void *src, size_t bytes
     return PGH_OK;
 }
 
+thread_ctx::~thread_ctx()
+{
+    // the owning thread has ended (pthread key destructor): the runtime is
+    // alive, and nothing of this thread is in flight after the synchronise
+    if (fnds) fnds_free(fnds);
+    if (stream) {
+        (void)hipStreamSynchronize(stream);
+        (void)hipStreamDestroy(stream);
+    }
+}
+
+namespace
+{
+// The contexts hang off a pthread key rather than a C++ thread_local: key
+// destructors run when a thread ends, but not for the thread that calls
+// exit(), whose context the process end reclaims.  A thread_local's
+// destructor would run inside exit()'s handlers, where HIP calls are not safe
+// (the runtime and a profiler's tool library tear down there too).
+pthread_key_t g_ctx_key;
+int g_ctx_key_status = -1;
+pthread_once_t g_ctx_once = PTHREAD_ONCE_INIT;
+
+void release_ctx(void *p)
+{
+    delete static_cast<thread_ctx *>(p);
+}
+
+void make_ctx_key()
+{
+    g_ctx_key_status = pthread_key_create(&g_ctx_key, release_ctx);
+}
+} // namespace
+
 pgh_status get_thread_ctx(thread_ctx **out)
 {
-    thread_local thread_ctx ctx;
-    if (!ctx.stream) {
+    (void)pthread_once(&g_ctx_once, make_ctx_key);
+    if (g_ctx_key_status != 0) return fail(PGH_DEVICE_ERROR, "pthread_key_create failed");
+    auto *ctx = static_cast<thread_ctx *>(pthread_getspecific(g_ctx_key));
+    if (!ctx) {
+        ctx = new thread_ctx();
+        if (pthread_setspecific(g_ctx_key, ctx) != 0) {
+            delete ctx;
+            return fail(PGH_OUT_OF_MEMORY, "pthread_setspecific failed");
+        }
+    }
+    if (!ctx->stream) {
         int n = 0;
         if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
             return fail(PGH_NO_DEVICE, "no HIP device is available");
         }
-        PGH_HIP(hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking));
+        PGH_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     }
-    *out = &ctx;
+    *out = ctx;
     return PGH_OK;
 }
 
@@ -366,36 +440,42 @@ int pgh_device_available(void)
 }
 
 pgh_status pgh_problem_dims(const pgh_udp *udp, uint64_t *nx, uint64_t *nf)
-{
+try {
     prob p;
     PGH_TRY(resolve(udp, p));
     if (nx) *nx = p.nx;
     if (nf) *nf = p.nf;
     return PGH_OK;
+} catch (...) {
+    return api_exception();
 }
 
 int pgh_supported(const pgh_udp *udp)
-{
+try {
     const std::string saved = g_last_error;
     prob p;
     const bool ok = resolve(udp, p) == PGH_OK;
     g_last_error = saved;
     return ok ? 1 : 0;
+} catch (...) {
+    return 0;
 }
 
 pgh_status pgh_bfe_eval(const pgh_udp *udp, const double *dvs, uint64_t n_dvs, uint64_t nx, double *fvs, uint64_t nf,
                         void *stream)
-{
+try {
     prob p;
     PGH_TRY(resolve(udp, p));
     PGH_TRY(reject_shift(p, "pgh_bfe_eval"));
     PGH_TRY(check_shape(p, nx, nf));
     if (n_dvs && (!dvs || !fvs)) return fail(PGH_INVALID_ARGUMENT, "null buffer");
     return launch_fitness(p, dvs, n_dvs, fvs, static_cast<hipStream_t>(stream));
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_bfe_eval_host(const pgh_udp *udp, const double *dvs, uint64_t dvs_len, double *fvs, uint64_t fvs_len)
-{
+try {
     prob p;
     PGH_TRY(resolve(udp, p));
     uint64_t n_dvs = 0;
@@ -418,11 +498,13 @@ pgh_status pgh_bfe_eval_host(const pgh_udp *udp, const double *dvs, uint64_t dvs
     PGH_HIP(hipMemcpyAsync(fvs, ctx->b.ptr, out_bytes, hipMemcpyDeviceToHost, ctx->stream));
     PGH_HIP(hipStreamSynchronize(ctx->stream));
     return PGH_OK;
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_batch_gradient(const pgh_udp *udp, const double *dvs, uint64_t n_dvs, uint64_t nx, double *grad,
                               double *fvs, void *stream)
-{
+try {
     prob p;
     PGH_TRY(resolve(udp, p));
     PGH_TRY(reject_shift(p, "pgh_batch_gradient"));
@@ -430,11 +512,13 @@ pgh_status pgh_batch_gradient(const pgh_udp *udp, const double *dvs, uint64_t n_
     PGH_TRY(check_has_gradient(p));
     if (n_dvs && (!dvs || !grad)) return fail(PGH_INVALID_ARGUMENT, "null buffer");
     return launch_gradient(p, dvs, n_dvs, grad, fvs, static_cast<hipStream_t>(stream));
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_batch_gradient_host(const pgh_udp *udp, const double *dvs, uint64_t dvs_len, double *grad,
                                    uint64_t grad_len, double *fvs)
-{
+try {
     prob p;
     PGH_TRY(resolve(udp, p));
     PGH_TRY(check_has_gradient(p));
@@ -462,6 +546,8 @@ pgh_status pgh_batch_gradient_host(const pgh_udp *udp, const double *dvs, uint64
     }
     PGH_HIP(hipStreamSynchronize(ctx->stream));
     return PGH_OK;
+} catch (...) {
+    return api_exception();
 }
 
 uint64_t pgh_local_descent_work_len(uint64_t n_dvs, uint64_t nx)
@@ -472,7 +558,7 @@ uint64_t pgh_local_descent_work_len(uint64_t n_dvs, uint64_t nx)
 pgh_status pgh_local_descent(const pgh_udp *udp, double *dvs, uint64_t n_dvs, uint64_t nx, double *fvs,
                              const double *lb, const double *ub, uint32_t iters, double step0, double *work,
                              uint64_t work_len, void *stream)
-{
+try {
     prob p;
     PGH_TRY(resolve(udp, p));
     PGH_TRY(check_shape(p, nx, p.nf));
@@ -485,11 +571,13 @@ pgh_status pgh_local_descent(const pgh_udp *udp, double *dvs, uint64_t n_dvs, ui
                                               + std::to_string(need) + " are needed");
     }
     return launch_local_descent(p, dvs, n_dvs, fvs, lb, ub, iters, step0, work, static_cast<hipStream_t>(stream));
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_local_descent_host(const pgh_udp *udp, double *dvs, uint64_t dvs_len, double *fvs, uint64_t fvs_len,
                                   const double *lb, const double *ub, uint32_t iters, double step0)
-{
+try {
     prob p;
     PGH_TRY(resolve(udp, p));
     PGH_TRY(check_local_descent(p, lb, ub, step0));
@@ -524,16 +612,20 @@ pgh_status pgh_local_descent_host(const pgh_udp *udp, double *dvs, uint64_t dvs_
     PGH_HIP(hipMemcpyAsync(fvs, f, n_dvs * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     PGH_HIP(hipStreamSynchronize(ctx->stream));
     return PGH_OK;
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_mt19937_generate(uint32_t seed, uint64_t skip, uint64_t n, uint32_t *out, void *stream)
-{
+try {
     if (n && !out) return fail(PGH_INVALID_ARGUMENT, "null buffer");
     return launch_mt19937(seed, skip, n, out, static_cast<hipStream_t>(stream));
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_batch_random_dv(const pgh_udp *udp, uint64_t n, uint32_t seed, double *out, void *stream)
-{
+try {
     prob p;
     PGH_TRY(resolve(udp, p));
     PGH_TRY(reject_shift(p, "pgh_batch_random_dv"));
@@ -544,6 +636,8 @@ pgh_status pgh_batch_random_dv(const pgh_udp *udp, uint64_t n, uint32_t seed, do
     }
     if (n && !out) return fail(PGH_INVALID_ARGUMENT, "null buffer");
     return launch_batch_random_dv(p, n, seed, out, static_cast<hipStream_t>(stream));
+} catch (...) {
+    return api_exception();
 }
 
 } // extern "C"

@@ -589,7 +589,7 @@ using namespace pgh;
 extern "C" {
 
 pgh_status pgh_lj_set_groups(uint64_t atoms, int grad, uint32_t groups)
-{
+try {
     if (atoms < 3u || atoms > 4096u) return fail(PGH_INVALID_ARGUMENT, "atoms out of range: " + std::to_string(atoms));
     const uint32_t a = static_cast<uint32_t>(atoms);
     const uint32_t most = lj_group_count(a, grad != 0);
@@ -601,11 +601,13 @@ pgh_status pgh_lj_set_groups(uint64_t atoms, int grad, uint32_t groups)
     if (groups == 0u) g_lj_groups.erase(2u * a + (grad ? 1u : 0u));
     else g_lj_groups[2u * a + (grad ? 1u : 0u)] = groups;
     return PGH_OK;
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_lj_tune(const pgh_udp *udp, const double *dvs, uint64_t n_dvs, double *fvs, double *grad,
                        uint32_t reps, void *stream, uint32_t *chosen)
-{
+try {
     prob p;
     PGH_TRY(resolve(udp, p));
     if (p.kind != PGH_LENNARD_JONES) return fail(PGH_INVALID_ARGUMENT, "pgh_lj_tune needs a lennard_jones problem");
@@ -653,6 +655,8 @@ pgh_status pgh_lj_tune(const pgh_udp *udp, const double *dvs, uint64_t n_dvs, do
     PGH_TRY(pgh_lj_set_groups(p.dim, with_grad, best_g));
     if (chosen) *chosen = best_g;
     return launch_lj(dvs, n_dvs, p.dim, fvs, grad, s);
+} catch (...) {
+    return api_exception();
 }
 
 } // extern "C"

@@ -603,6 +603,27 @@ pgh_status launch_grid_peel(pgh_fnds_ws *ws, hipStream_t s, uint32_t n32, uint32
 }
 
 } // namespace
+
+void fnds_free(pgh_fnds_ws *ws)
+{
+    if (!ws) return;
+    if (ws->stream) (void)hipStreamSynchronize(ws->stream);
+    if (ws->own) (void)hipStreamDestroy(ws->own);
+    delete ws;
+}
+
+pgh_status thread_fnds_ws(pgh_fnds_ws **ws)
+{
+    thread_ctx *ctx = nullptr;
+    PGH_TRY(get_thread_ctx(&ctx));
+    if (!ctx->fnds) {
+        ctx->fnds = new pgh_fnds_ws();
+        ctx->fnds->thread_owned = true;
+    }
+    *ws = ctx->fnds;
+    return PGH_OK;
+}
+
 } // namespace pgh
 
 using namespace pgh;
@@ -610,22 +631,35 @@ using namespace pgh;
 extern "C" {
 
 pgh_status pgh_fnds_create(pgh_fnds_ws **ws)
-{
+try {
     if (!ws) return fail(PGH_INVALID_ARGUMENT, "null workspace pointer");
     *ws = new pgh_fnds_ws();
     return PGH_OK;
+} catch (...) {
+    return api_exception();
+}
+
+pgh_status pgh_fnds_thread_workspace(pgh_fnds_ws **ws)
+try {
+    if (!ws) return fail(PGH_INVALID_ARGUMENT, "null workspace pointer");
+    return thread_fnds_ws(ws);
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_fnds_destroy(pgh_fnds_ws *ws)
-{
-    if (ws && ws->stream) (void)hipStreamSynchronize(ws->stream);
-    if (ws && ws->own) (void)hipStreamDestroy(ws->own);
-    delete ws;
+try {
+    if (ws && ws->thread_owned) {
+        return fail(PGH_INVALID_ARGUMENT, "a thread's own fnds workspace is released with its thread");
+    }
+    fnds_free(ws);
     return PGH_OK;
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_fnds_run_host(pgh_fnds_ws *ws, const double *objs, uint64_t n, uint64_t m)
-{
+try {
     if (!ws) return fail(PGH_INVALID_ARGUMENT, "null workspace");
     if (n < 2u) {
         ws->sized = false;
@@ -641,10 +675,12 @@ pgh_status pgh_fnds_run_host(pgh_fnds_ws *ws, const double *objs, uint64_t n, ui
     PGH_TRY(ws->staged.reserve(std::max<uint64_t>(bytes, 8u)));
     if (bytes) PGH_HIP(hipMemcpyAsync(ws->staged.ptr, objs, bytes, hipMemcpyHostToDevice, ws->own));
     return pgh_fnds_run(ws, static_cast<const double *>(ws->staged.ptr), n, m, ws->own);
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_fnds_run(pgh_fnds_ws *ws, const double *objs, uint64_t n, uint64_t m, void *stream)
-{
+try {
     if (!ws) return fail(PGH_INVALID_ARGUMENT, "null workspace");
     ws->sized = false;
     if (n < 2u) {
@@ -725,10 +761,12 @@ pgh_status pgh_fnds_run(pgh_fnds_ws *ws, const double *objs, uint64_t n, uint64_
     }
     PGH_HIP(hipGetLastError());
     return PGH_OK;
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_fnds_set_peel_blocks(pgh_fnds_ws *ws, uint32_t blocks)
-{
+try {
     if (!ws) return fail(PGH_INVALID_ARGUMENT, "null workspace");
     if (blocks > kMaxPeelBlocks) {
         return fail(PGH_INVALID_ARGUMENT, "at most " + std::to_string(kMaxPeelBlocks) + " peel workgroups, got "
@@ -736,11 +774,13 @@ pgh_status pgh_fnds_set_peel_blocks(pgh_fnds_ws *ws, uint32_t blocks)
     }
     ws->peel_blocks = blocks;
     return PGH_OK;
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_fnds_tune(pgh_fnds_ws *ws, const double *objs, uint64_t n, uint64_t m, uint32_t reps, void *stream,
                          uint32_t *chosen)
-{
+try {
     if (!ws) return fail(PGH_INVALID_ARGUMENT, "null workspace");
     if (reps == 0u) return fail(PGH_INVALID_ARGUMENT, "the tuner needs at least one timed repetition");
     if (chosen) *chosen = 0u;
@@ -801,10 +841,12 @@ pgh_status pgh_fnds_tune(pgh_fnds_ws *ws, const double *objs, uint64_t n, uint64
     if (chosen) *chosen = best_b;
     // leave the workspace holding a result computed with the chosen grid
     return pgh_fnds_run(ws, objs, n, m, stream);
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_fnds_sizes(pgh_fnds_ws *ws, uint64_t *n_fronts, uint64_t *n_dom_items)
-{
+try {
     if (!ws || !ws->n) return fail(PGH_INVALID_ARGUMENT, "fnds workspace has no result");
     if (!ws->sized) {
         PGH_HIP(hipMemcpyAsync(ws->h_sizes, ws->scalars.ptr, 24u, hipMemcpyDeviceToHost, ws->stream));
@@ -822,11 +864,13 @@ pgh_status pgh_fnds_sizes(pgh_fnds_ws *ws, uint64_t *n_fronts, uint64_t *n_dom_i
     if (n_fronts) *n_fronts = ws->h_sizes[0];
     if (n_dom_items) *n_dom_items = ws->h_sizes[1];
     return PGH_OK;
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_fnds_copy(pgh_fnds_ws *ws, uint64_t *dom_count, uint64_t *rank, uint64_t *front_offsets,
                          uint64_t *front_items, uint64_t *dom_offsets, uint64_t *dom_items, int to_host)
-{
+try {
     uint64_t nfr = 0, ndi = 0;
     PGH_TRY(pgh_fnds_sizes(ws, &nfr, &ndi));
     const hipMemcpyKind kind = to_host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
@@ -847,6 +891,8 @@ pgh_status pgh_fnds_copy(pgh_fnds_ws *ws, uint64_t *dom_count, uint64_t *rank, u
     }
     if (to_host) PGH_HIP(hipStreamSynchronize(ws->stream));
     return PGH_OK;
+} catch (...) {
+    return api_exception();
 }
 
 } // extern "C"

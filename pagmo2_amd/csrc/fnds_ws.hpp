@@ -38,4 +38,7 @@ struct pgh_fnds_ws {
     // mo_sort.hip: sort keys and values (two of each), crowding, radix-sort
     // scratch, and a staged output
     pgh::dev_buf sk0, sk1, sv0, sv1, crowd, sort_tmp, out_staged;
+    // a host thread's own workspace (pgh_fnds_thread_workspace): released
+    // with its thread, never by pgh_fnds_destroy
+    bool thread_owned = false;
 };

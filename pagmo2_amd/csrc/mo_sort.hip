@@ -228,19 +228,14 @@ pgh_status fnds_fronts(pgh_fnds_ws *ws, const double *objs, uint64_t n, uint64_t
     return PGH_OK;
 }
 
-// Host-buffer forms: a workspace, its own stream and staged copies.
+// Host-buffer forms: the calling thread's own workspace (kept between calls,
+// so a generation loop allocates once), its stream and a staged copy.
 struct host_call {
     pgh_fnds_ws *ws = nullptr;
-    ~host_call()
-    {
-        if (ws) pgh_fnds_destroy(ws);
-    }
     pgh_status begin(const double *objs, uint64_t n, uint64_t m, const double **dev)
     {
-        PGH_TRY(pgh_fnds_create(&ws));
-        int count = 0;
-        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(PGH_NO_DEVICE, "no HIP device is available");
-        PGH_HIP(hipStreamCreateWithFlags(&ws->own, hipStreamNonBlocking));
+        PGH_TRY(thread_fnds_ws(&ws));
+        if (!ws->own) PGH_HIP(hipStreamCreateWithFlags(&ws->own, hipStreamNonBlocking));
         const uint64_t bytes = n * m * sizeof(double);
         PGH_TRY(ws->staged.reserve(std::max<uint64_t>(bytes, 8u)));
         if (bytes) PGH_HIP(hipMemcpyAsync(ws->staged.ptr, objs, bytes, hipMemcpyHostToDevice, ws->own));
@@ -258,7 +253,7 @@ extern "C" {
 
 pgh_status pgh_crowding_distance(pgh_fnds_ws *ws, const double *points, uint64_t n, uint64_t m, double *out,
                                  void *stream)
-{
+try {
     if (!ws) return fail(PGH_INVALID_ARGUMENT, "null workspace");
     if (n < 2u) return fail(PGH_INVALID_ARGUMENT, crowding_points_message(n));
     if (m < 2u) return fail(PGH_INVALID_ARGUMENT, crowding_objectives_message(m));
@@ -270,10 +265,12 @@ pgh_status pgh_crowding_distance(pgh_fnds_ws *ws, const double *points, uint64_t
     PGH_HIP(hipMemcpyAsync(out, ws->crowd.ptr, n * 8u, hipMemcpyDeviceToDevice, s));
     PGH_HIP(hipStreamSynchronize(s));
     return PGH_OK;
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_crowding_distance_host(const double *points, uint64_t n, uint64_t m, double *out)
-{
+try {
     if (n < 2u) return fail(PGH_INVALID_ARGUMENT, crowding_points_message(n));
     if (m < 2u) return fail(PGH_INVALID_ARGUMENT, crowding_objectives_message(m));
     host_call h;
@@ -284,11 +281,13 @@ pgh_status pgh_crowding_distance_host(const double *points, uint64_t n, uint64_t
     PGH_HIP(hipMemcpyAsync(out, h.ws->crowd.ptr, n * 8u, hipMemcpyDeviceToHost, h.ws->own));
     PGH_HIP(hipStreamSynchronize(h.ws->own));
     return PGH_OK;
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_sort_population_mo(pgh_fnds_ws *ws, const double *objs, uint64_t n, uint64_t m, uint64_t *out,
                                   void *stream)
-{
+try {
     if (!ws) return fail(PGH_INVALID_ARGUMENT, "null workspace");
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (n < 2u) { // corner cases (multi_objective.cpp:427-434)
@@ -324,10 +323,12 @@ pgh_status pgh_sort_population_mo(pgh_fnds_ws *ws, const double *objs, uint64_t 
     PGH_HIP(hipGetLastError());
     PGH_HIP(hipStreamSynchronize(s));
     return PGH_OK;
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_sort_population_mo_host(const double *objs, uint64_t n, uint64_t m, uint64_t *out)
-{
+try {
     if (n < 2u) {
         if (n == 1u) out[0] = 0u;
         return PGH_OK;
@@ -340,11 +341,13 @@ pgh_status pgh_sort_population_mo_host(const double *objs, uint64_t n, uint64_t 
     PGH_HIP(hipMemcpyAsync(out, h.ws->out_staged.ptr, n * 8u, hipMemcpyDeviceToHost, h.ws->own));
     PGH_HIP(hipStreamSynchronize(h.ws->own));
     return PGH_OK;
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_select_best_N_mo(pgh_fnds_ws *ws, const double *objs, uint64_t n, uint64_t m, uint64_t N,
                                 uint64_t *out, uint64_t *out_len, void *stream)
-{
+try {
     if (!ws || !out_len) return fail(PGH_INVALID_ARGUMENT, "null workspace or length");
     hipStream_t s = static_cast<hipStream_t>(stream);
     *out_len = 0;
@@ -395,11 +398,13 @@ pgh_status pgh_select_best_N_mo(pgh_fnds_ws *ws, const double *objs, uint64_t n,
     PGH_HIP(hipStreamSynchronize(s));
     *out_len = N;
     return PGH_OK;
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_select_best_N_mo_host(const double *objs, uint64_t n, uint64_t m, uint64_t N, uint64_t *out,
                                      uint64_t *out_len)
-{
+try {
     if (!out_len) return fail(PGH_INVALID_ARGUMENT, "null length");
     *out_len = 0;
     if (N == 0u || n == 0u) return PGH_OK;
@@ -418,6 +423,8 @@ pgh_status pgh_select_best_N_mo_host(const double *objs, uint64_t n, uint64_t m,
     if (*out_len) PGH_HIP(hipMemcpyAsync(out, h.ws->out_staged.ptr, *out_len * 8u, hipMemcpyDeviceToHost, h.ws->own));
     PGH_HIP(hipStreamSynchronize(h.ws->own));
     return PGH_OK;
+} catch (...) {
+    return api_exception();
 }
 
 } // extern "C"

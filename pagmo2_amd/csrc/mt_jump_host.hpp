@@ -348,7 +348,13 @@ inline void prefetch_jump_polys(const std::vector<uint64_t> &Js, unsigned thread
     };
     std::vector<std::thread> pool;
     const size_t n = std::min<size_t>(threads, todo.size());
-    for (size_t t = 1; t < n; ++t) pool.emplace_back(work);
+    try {
+        pool.reserve(n);
+        for (size_t t = 1; t < n; ++t) pool.emplace_back(work);
+    } catch (...) {
+        // fewer helper threads: the work queue is shared, so this thread and
+        // those started finish it
+    }
     work();
     for (auto &th : pool) th.join();
     std::lock_guard<std::mutex> lk(c.mu);

@@ -123,11 +123,16 @@ struct pinned_buf {
     ~pinned_buf();
 };
 
-// Per-host-thread stream and staging buffers for the _host entry points.
+// Per-host-thread stream, staging buffers and fnds workspace for the _host
+// entry points.  Made on a thread's first call and released when that thread
+// ends (a pthread key destructor); the process's main thread keeps its
+// context until the process ends, so exit() makes no HIP call.
 struct thread_ctx {
     hipStream_t stream = nullptr;
     dev_buf a, b, c, d, e;
     pinned_buf pin;
+    pgh_fnds_ws *fnds = nullptr; // pgh_fnds_thread_workspace, made on first use
+    ~thread_ctx();
 };
 
 // Host-to-device copy of a host buffer on the context's stream.  Large
@@ -135,5 +140,16 @@ struct thread_ctx {
 // DMA while the other fills; small ones are one hipMemcpyAsync.
 pgh_status upload_host(thread_ctx *ctx, void *dst, const void *src, size_t bytes);
 pgh_status get_thread_ctx(thread_ctx **ctx);
+
+// The calling thread's fnds workspace (fnds.hip): owned by its thread_ctx.
+pgh_status thread_fnds_ws(pgh_fnds_ws **ws);
+// Releases a workspace whatever its owner (fnds.hip).
+void fnds_free(pgh_fnds_ws *ws);
+
+// Status and message of an exception caught at the C ABI boundary: every
+// extern "C" entry point is a function-try-block ending in
+//     catch (...) { return pgh::api_exception(); }
+// so that no C++ exception unwinds into a C or ctypes caller.
+pgh_status api_exception() noexcept;
 
 } // namespace pgh

@@ -39,7 +39,8 @@ using namespace dev;
 struct island_desc {
     int32_t kind;
     uint32_t dim, nx, pop;
-    uint32_t gens, radius, lanes, pad;
+    uint32_t gens, radius, lanes;
+    uint32_t memory; // pso_gen's memory flag: velocities kept between evolve calls
     double omega, eta1, eta2, max_vel;
     uint64_t xoff; // X, V, lbX offset (doubles)
     uint64_t foff; // fit, lbfit, offset (doubles)
@@ -121,8 +122,9 @@ __device__ __forceinline__ void store_state(mt_state *__restrict__ dst, const ui
 // The island's std::mt19937 draws, by its segments (one workgroup each).
 // phase 0, the start of an evolve call: every segment takes the island's
 // canonical generator (everything handed out so far); segment 0 hands out
-// the velocity draws (2 words per element, pso_gen.cpp:193-200) when
-// draw_v, and every segment moves to the start of its first generation, s gps
+// the velocity draws (2 words per element, pso_gen.cpp:193-200) unless the
+// island keeps its velocities (memory, once they exist: v_valid), and every
+// segment moves to the start of its first generation, s gps
 // (jumps z^n0 and z^(gps W 2^b) over the bits of s).  phase 1, the batch of
 // generations g0 .. g0 + kGenBatch - 1 (4 words per element and generation,
 // r1 and r2 of pso_gen.cpp:300-301): segment s hands out the words of its
@@ -135,7 +137,7 @@ __global__ __launch_bounds__(kMtJumpBlock) void k_rng(const island_desc *__restr
                                                       const seg_desc *__restrict__ sd, mt_state *__restrict__ st,
                                                       mt_state *__restrict__ segst, jump_tab jt,
                                                       uint32_t *__restrict__ words, uint32_t sg0, uint32_t phase,
-                                                      uint32_t g0, uint32_t draw_v)
+                                                      uint32_t g0, uint32_t v_valid)
 {
     __shared__ uint32_t ring[kMtRing];
     __shared__ uint4 lds_idx[kMtIdxStride / 8u];
@@ -150,6 +152,8 @@ __global__ __launch_bounds__(kMtJumpBlock) void k_rng(const island_desc *__restr
     mt_state *mine = &segst[di.seg0 + sg.s];
     if (phase == 0u) {
         load_state(&st[sg.island], ring, K, C);
+        // velocities are drawn unless the island keeps them (pso_gen.cpp:193-200)
+        const bool draw_v = !(di.memory && v_valid);
         const uint64_t n0 = draw_v ? 2u * elems : 0u;
         if (sg.s == 0u) {
             if (n0) {
@@ -188,7 +192,7 @@ __global__ __launch_bounds__(kBlock) void k_vinit(const island_desc *__restrict_
                                                   uint64_t e0, uint64_t total, const uint32_t *__restrict__ words,
                                                   double *__restrict__ X, double *__restrict__ V,
                                                   double *__restrict__ lbX, double *__restrict__ fit,
-                                                  double *__restrict__ lbfit, uint32_t draw_v, uint32_t from_lb)
+                                                  double *__restrict__ lbfit, uint32_t v_valid, uint32_t from_lb)
 {
     const uint64_t t = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (t >= total) return;
@@ -197,7 +201,7 @@ __global__ __launch_bounds__(kBlock) void k_vinit(const island_desc *__restrict_
     const island_desc di = d[k];
     const uint64_t m = e - di.eoff;
     const uint32_t j = static_cast<uint32_t>(m % di.nx);
-    if (draw_v) {
+    if (!(di.memory && v_valid)) {
         double lb, ub;
         bounds_of(di.kind, j, lb, ub);
         const double vwidth = (ub - lb) * di.max_vel; // pso_gen.cpp:177-181
@@ -588,9 +592,10 @@ struct pgh_archi {
     jump_tab jt{nullptr, nullptr};
     hipStream_t own = nullptr;
     hipStream_t last = nullptr;
+    hipEvent_t order = nullptr; // orders a call after the previous call's stream
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    bool graph_draw_v = false, graph_from_lb = false;
+    bool graph_v_valid = false, graph_from_lb = false;
     migr_ctx *migr = nullptr; // set by pgh_archi_set_migration
     bool one_shot = false;    // pgh_pso_gen_evolve_host's own archipelago
     uint64_t words_half = 0;  // offset of a lone island's second batch of words (0: none)
@@ -601,6 +606,7 @@ struct pgh_archi {
         if (migr) migr_destroy(migr);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+        if (order) (void)hipEventDestroy(order);
         if (own) (void)hipStreamDestroy(own);
     }
 };
@@ -625,8 +631,9 @@ uint32_t blocks_for(uint64_t count)
     return static_cast<uint32_t>((count + kBlock - 1u) / kBlock);
 }
 
-// Launches the evolve call for islands [i0, i1) on stream s.
-pgh_status launch_range(pgh_archi *a, uint32_t i0, uint32_t i1, bool draw_v, bool from_lb, hipStream_t s)
+// Launches the evolve call for islands [i0, i1) on stream s.  v_valid: the
+// islands have velocities from an earlier call (kept by those with memory).
+pgh_status launch_range(pgh_archi *a, uint32_t i0, uint32_t i1, bool v_valid, bool from_lb, hipStream_t s)
 {
     const auto *d = static_cast<const island_desc *>(a->desc.ptr);
     auto *st = static_cast<mt_state *>(a->states.ptr);
@@ -647,9 +654,9 @@ pgh_status launch_range(pgh_archi *a, uint32_t i0, uint32_t i1, bool draw_v, boo
     const auto *sd = static_cast<const seg_desc *>(a->segs.ptr);
     auto *segst = static_cast<mt_state *>(a->seg_states.ptr);
     const jump_tab jt = a->jt;
-    k_rng<<<ng, kMtJumpBlock, 0, s>>>(d, sd, st, segst, jt, w, sg0, 0u, 0u, draw_v ? 1u : 0u);
+    k_rng<<<ng, kMtJumpBlock, 0, s>>>(d, sd, st, segst, jt, w, sg0, 0u, 0u, v_valid ? 1u : 0u);
     PGH_HIP(hipGetLastError());
-    k_vinit<<<blocks_for(total), kBlock, 0, s>>>(d, i0, i1, e0, total, w, X, V, lbX, fit, lbfit, draw_v ? 1u : 0u,
+    k_vinit<<<blocks_for(total), kBlock, 0, s>>>(d, i0, i1, e0, total, w, X, V, lbX, fit, lbfit, v_valid ? 1u : 0u,
                                                  from_lb ? 1u : 0u);
     PGH_HIP(hipGetLastError());
     // a single island of one tile, evolved by the one-shot entry point: a
@@ -758,6 +765,7 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
         di.eta1 = q.eta1;
         di.eta2 = q.eta2;
         di.max_vel = q.max_vel;
+        di.memory = q.memory ? 1u : 0u;
         const uint64_t elems = sp.pop * p.nx;
         di.xoff = xoff;
         di.foff = foff;
@@ -776,7 +784,7 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
         toff += static_cast<uint64_t>(q.gen) * sp.pop;
         eoff += elems;
         a->max_gens = std::max(a->max_gens, q.gen);
-        a->memory.push_back(q.memory);
+        a->memory.push_back(q.memory ? 1u : 0u);
         // evaluation tiles
         uint32_t per = 256u;
         if (p.kind == PGH_LENNARD_JONES) {
@@ -826,6 +834,7 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
     a->total_words = woff;
     a->total_trace = toff;
     PGH_HIP(hipStreamCreateWithFlags(&a->own, hipStreamNonBlocking));
+    PGH_HIP(hipEventCreateWithFlags(&a->order, hipEventDisableTiming));
     a->n_segs = static_cast<uint32_t>(seg_list.size());
     std::vector<uint8_t> tab;
     PGH_TRY(jump_table_bytes(dists, tab));
@@ -897,25 +906,31 @@ extern "C" {
 
 pgh_status pgh_archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_t n_islands,
                             const double *const *X0, const double *const *F0)
-{
+try {
     return archi_create(out, islands, n_islands, X0, F0, nullptr);
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_archi_create_seeded(pgh_archi **out, const pgh_island_spec *islands, uint64_t n_islands,
                                    const uint32_t *pop_seeds)
-{
+try {
     if (!pop_seeds) return fail(PGH_INVALID_ARGUMENT, "null argument");
     return archi_create(out, islands, n_islands, nullptr, nullptr, pop_seeds);
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_archi_destroy(pgh_archi *a)
-{
+try {
     delete a;
     return PGH_OK;
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_archi_set_trace(pgh_archi *a, int on)
-{
+try {
     if (!a) return fail(PGH_INVALID_ARGUMENT, "null archipelago");
     if (on && !a->trace_buf.ptr) PGH_TRY(a->trace_buf.reserve(std::max<uint64_t>(a->total_trace, 1u) * 8u));
     if (a->trace != (on != 0)) {
@@ -927,15 +942,23 @@ pgh_status pgh_archi_set_trace(pgh_archi *a, int on)
     }
     a->trace = on != 0;
     return PGH_OK;
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_archi_evolve(pgh_archi *a, int mode, void *stream)
-{
+try {
     if (!a) return fail(PGH_INVALID_ARGUMENT, "null archipelago");
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : a->own;
-    // velocities are re-drawn unless memory is set and they exist
-    // (pso_gen.cpp:193-200); islands share the memory flag of island 0
-    const bool draw_v = !(a->memory[0] && a->v_valid);
+    // a call on another stream than the previous one starts after it: the
+    // archipelago's buffers are read and written by every call
+    if (a->last && a->last != s) {
+        PGH_HIP(hipEventRecord(a->order, a->last));
+        PGH_HIP(hipStreamWaitEvent(s, a->order, 0));
+    }
+    // velocities are re-drawn unless an island's memory flag is set and they
+    // exist (pso_gen.cpp:193-200); the kernels decide per island
+    const bool v_valid = a->v_valid;
     bool from_lb = a->evolved;
     const uint32_t n = static_cast<uint32_t>(a->h_desc.size());
     auto *X = static_cast<double *>(a->X.ptr);
@@ -954,15 +977,15 @@ pgh_status pgh_archi_evolve(pgh_archi *a, int mode, void *stream)
         if (moved) from_lb = !a->evolved;
     }
     if (mode == PGH_ARCHI_PER_ISLAND) {
-        for (uint32_t k = 0; k < n; ++k) PGH_TRY(launch_range(a, k, k + 1u, draw_v, from_lb, s));
+        for (uint32_t k = 0; k < n; ++k) PGH_TRY(launch_range(a, k, k + 1u, v_valid, from_lb, s));
     } else if (mode == PGH_ARCHI_BATCHED) {
-        if (!a->exec || a->graph_draw_v != draw_v || a->graph_from_lb != from_lb) {
+        if (!a->exec || a->graph_v_valid != v_valid || a->graph_from_lb != from_lb) {
             if (a->exec) (void)hipGraphExecDestroy(a->exec);
             if (a->graph) (void)hipGraphDestroy(a->graph);
             a->exec = nullptr;
             a->graph = nullptr;
             PGH_HIP(hipStreamBeginCapture(a->own, hipStreamCaptureModeThreadLocal));
-            const pgh_status st = launch_range(a, 0u, n, draw_v, from_lb, a->own);
+            const pgh_status st = launch_range(a, 0u, n, v_valid, from_lb, a->own);
             hipGraph_t g = nullptr;
             const hipError_t ce = hipStreamEndCapture(a->own, &g);
             if (st != PGH_OK) {
@@ -972,7 +995,7 @@ pgh_status pgh_archi_evolve(pgh_archi *a, int mode, void *stream)
             if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
             a->graph = g;
             PGH_HIP(hipGraphInstantiate(&a->exec, a->graph, nullptr, nullptr, 0));
-            a->graph_draw_v = draw_v;
+            a->graph_v_valid = v_valid;
             a->graph_from_lb = from_lb;
         }
         PGH_HIP(hipGraphLaunch(a->exec, s));
@@ -985,13 +1008,15 @@ pgh_status pgh_archi_evolve(pgh_archi *a, int mode, void *stream)
     a->evolved = true;
     a->last = s;
     return PGH_OK;
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_archi_set_migration(pgh_archi *a, int type, int handling, const uint64_t *conn_offsets,
                                    const uint64_t *conn_src, const double *conn_weight,
                                    const pgh_migr_rate *select_rate, const pgh_migr_rate *replace_rate,
                                    const uint32_t *migr_seeds)
-{
+try {
     if (!a) return fail(PGH_INVALID_ARGUMENT, "null archipelago");
     if (a->last) PGH_HIP(hipStreamSynchronize(a->last));
     if (!conn_offsets) {
@@ -1011,10 +1036,12 @@ pgh_status pgh_archi_set_migration(pgh_archi *a, int type, int handling, const u
     if (a->migr) migr_destroy(a->migr);
     a->migr = m;
     return PGH_OK;
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_archi_get_migrants(pgh_archi *a, uint64_t k, uint64_t *count, double *X, double *F)
-{
+try {
     if (!a || k >= a->h_desc.size()) return fail(PGH_INVALID_ARGUMENT, "no such island");
     if (!a->migr) {
         if (count) *count = 0u;
@@ -1022,10 +1049,12 @@ pgh_status pgh_archi_get_migrants(pgh_archi *a, uint64_t k, uint64_t *count, dou
     }
     if (a->last) PGH_HIP(hipStreamSynchronize(a->last));
     return migr_get(a->migr, k, count, X, F);
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_archi_get(pgh_archi *a, uint64_t k, double *X, double *F)
-{
+try {
     if (!a || k >= a->h_desc.size()) return fail(PGH_INVALID_ARGUMENT, "no such island");
     if (a->last) PGH_HIP(hipStreamSynchronize(a->last));
     const island_desc &di = a->h_desc[k];
@@ -1039,10 +1068,12 @@ pgh_status pgh_archi_get(pgh_archi *a, uint64_t k, double *X, double *F)
         PGH_HIP(hipMemcpy(F, fs + di.foff, static_cast<uint64_t>(di.pop) * 8u, hipMemcpyDeviceToHost));
     }
     return PGH_OK;
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_archi_get_trace(pgh_archi *a, uint64_t k, double *trace_F)
-{
+try {
     if (!a || k >= a->h_desc.size()) return fail(PGH_INVALID_ARGUMENT, "no such island");
     if (!a->trace) return fail(PGH_INVALID_ARGUMENT, "tracing is off");
     if (a->last) PGH_HIP(hipStreamSynchronize(a->last));
@@ -1050,11 +1081,13 @@ pgh_status pgh_archi_get_trace(pgh_archi *a, uint64_t k, double *trace_F)
     PGH_HIP(hipMemcpy(trace_F, static_cast<double *>(a->trace_buf.ptr) + di.toff,
                       static_cast<uint64_t>(di.gens) * di.pop * 8u, hipMemcpyDeviceToHost));
     return PGH_OK;
+} catch (...) {
+    return api_exception();
 }
 
 pgh_status pgh_pso_gen_evolve_host(const pgh_udp *udp, const pgh_pso_params *prm, double *X, double *F, uint64_t pop,
                                    uint64_t nx)
-{
+try {
     if (!udp || !prm || !X || !F) return fail(PGH_INVALID_ARGUMENT, "null argument");
     prob p;
     PGH_TRY(resolve(udp, p));
@@ -1074,6 +1107,8 @@ pgh_status pgh_pso_gen_evolve_host(const pgh_udp *udp, const pgh_pso_params *prm
     a->one_shot = true;
     PGH_TRY(pgh_archi_evolve(a, PGH_ARCHI_PER_ISLAND, nullptr));
     return pgh_archi_get(a, 0u, X, F);
+} catch (...) {
+    return api_exception();
 }
 
 } // extern "C"

@@ -6,10 +6,13 @@
 //
 // Mirrored semantics (citations into the pagmo sources):
 //   problem: fitness() checks lengths and counts fevals (src/problem.cpp:353-379),
+//            gradient() checks the length and counts gevals (src/problem.cpp:456-467),
+//            not_implemented_error for a UDP without one (problem.hpp:638-643),
 //            extract<T>() (include/pagmo/problem.hpp:1104-1147), thread safety
 //            defaults to basic (problem.hpp:847-851)
 //   bfe:     validate -> call -> validate -> increment_fevals (src/bfe.cpp:91-110)
-//   thread_bfe / member_bfe (src/batch_evaluators/*.cpp)
+//   thread_bfe / member_bfe (src/batch_evaluators/*.cpp); member_bfe does not
+//            count fevals, the bfe wrapper does (member_bfe.cpp:40-45)
 //   population(prob, size, seed) (src/population.cpp:62-78)
 //   pso_gen with set_bfe (src/algorithms/pso_gen.cpp)
 #pragma once
@@ -32,6 +35,11 @@
 
 namespace pagmo
 {
+
+// include/pagmo/exceptions.hpp
+struct not_implemented_error final : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
 
 using vector_double = std::vector<double>;
 typedef std::vector<vector_double>::size_type pop_size_t;
@@ -303,8 +311,14 @@ class problem
         }
         vector_double gradient(const vector_double &x) const override
         {
-            if constexpr (detail::has_grad<T>::value) return m_value.gradient(x);
-            throw std::logic_error("The gradient has not been implemented in the user-defined problem");
+            if constexpr (detail::has_grad<T>::value) {
+                return m_value.gradient(x);
+            } else {
+                (void)x;
+                throw not_implemented_error("The gradient has been requested, but it is not implemented in a UDP "
+                                            "of type '"
+                                            + get_name() + "'");
+            }
         }
         bool has_gradient() const override
         {
@@ -334,7 +348,9 @@ public:
         m_nx = b.first.size();
         m_nf = m_ptr->fitness(b.first).size();
     }
-    problem(const problem &o) : m_ptr(o.m_ptr->clone()), m_nx(o.m_nx), m_nf(o.m_nf), m_fevals(o.m_fevals.load())
+    problem(const problem &o)
+        : m_ptr(o.m_ptr->clone()), m_nx(o.m_nx), m_nf(o.m_nf), m_fevals(o.m_fevals.load()),
+          m_gevals(o.m_gevals.load())
     {
     }
     problem &operator=(const problem &o)
@@ -343,6 +359,7 @@ public:
         m_nx = o.m_nx;
         m_nf = o.m_nf;
         m_fevals = o.m_fevals.load();
+        m_gevals = o.m_gevals.load();
         return *this;
     }
     vector_double fitness(const vector_double &dv) const
@@ -368,7 +385,14 @@ public:
     }
     vector_double gradient(const vector_double &dv) const
     {
-        return m_ptr->gradient(dv);
+        if (dv.size() != m_nx) throw std::invalid_argument("A decision vector is incompatible with a problem");
+        auto g = m_ptr->gradient(dv);
+        ++m_gevals;
+        return g;
+    }
+    unsigned long long get_gevals() const
+    {
+        return m_gevals.load();
     }
     std::pair<vector_double, vector_double> get_bounds() const
     {
@@ -409,7 +433,22 @@ private:
     std::unique_ptr<base> m_ptr;
     vector_double::size_type m_nx = 0, m_nf = 0;
     mutable std::atomic<unsigned long long> m_fevals{0};
+    mutable std::atomic<unsigned long long> m_gevals{0};
+
+    friend vector_double member_batch_fitness(const problem &, const vector_double &);
 };
+
+// the UDP's batch_fitness without counting (prob_invoke_mem_batch_fitness(p,
+// dvs, false), src/problem.cpp:972-984)
+inline vector_double member_batch_fitness(const problem &p, const vector_double &dvs)
+{
+    if (!p.has_batch_fitness()) {
+        throw not_implemented_error("The batch_fitness() method has been invoked, but it is not implemented in a UDP "
+                                    "of type '"
+                                    + p.get_name() + "'");
+    }
+    return p.m_ptr->batch_fitness(dvs);
+}
 
 // ---- batch evaluators --------------------------------------------------------
 class thread_bfe
@@ -449,9 +488,10 @@ public:
 class member_bfe
 {
 public:
+    // src/batch_evaluators/member_bfe.cpp:40-45: fevals are left to pagmo::bfe
     vector_double operator()(const problem &p, const vector_double &dvs) const
     {
-        return p.batch_fitness(dvs);
+        return member_batch_fitness(p, dvs);
     }
 };
 

@@ -16,7 +16,9 @@
 #include <limits>
 #include <random>
 #include <string>
+#include <thread>
 #include <type_traits>
+#include <vector>
 
 #include <pagmo/algorithms/pso_gen.hpp>
 #include <pagmo/population.hpp>
@@ -72,6 +74,43 @@ struct user_udp {
     }
 };
 
+// A UDP with batch_fitness (tests/member_bfe.cpp:64-77): default_bfe's
+// first choice, member_bfe (default_bfe.cpp:56-58).
+struct batch_udp {
+    vector_double fitness(const vector_double &) const
+    {
+        return {0};
+    }
+    std::pair<vector_double, vector_double> get_bounds() const
+    {
+        return {{0}, {1}};
+    }
+    vector_double batch_fitness(const vector_double &dvs) const
+    {
+        return vector_double(dvs.size(), 1.);
+    }
+};
+
+// A user problem with a gradient and no device kernel.
+struct grad_udp {
+    vector_double fitness(const vector_double &x) const
+    {
+        return {x[0] * x[0] + 3. * x[1] * x[2]};
+    }
+    vector_double gradient(const vector_double &x) const
+    {
+        return {2. * x[0], 3. * x[2], 3. * x[1]};
+    }
+    std::pair<vector_double, vector_double> get_bounds() const
+    {
+        return {{-1., -1., -1.}, {1., 1., 1.}};
+    }
+    std::string get_name() const
+    {
+        return "grad_udp";
+    }
+};
+
 // A problem that forbids concurrent use (tests/thread_bfe.cpp:99-116).
 struct unsafe_udp {
     vector_double fitness(const vector_double &) const
@@ -119,6 +158,44 @@ static void cpu_tests()
     // shared; the wrapper adds n_dvs once (bfe.cpp:102-107)
     CHECK(up.get_fevals() == 100u);
 
+    // a UDP with batch_fitness goes to member_bfe, and pagmo::bfe counts its
+    // fevals once (tests/member_bfe.cpp:84-87)
+    {
+        problem bp{batch_udp{}};
+        bfe bb{hip_bfe{}};
+        CHECK(bp.get_fevals() == 0u);
+        CHECK(bb(bp, {1., 2., 3.}) == vector_double(3, 1.));
+        CHECK(bp.get_fevals() == 3u);
+        CHECK(hip_bfe{}(bp, {0.5, 0.25}) == vector_double(2, 1.));
+        CHECK(bp.get_fevals() == 3u); // the UDBFE itself never counts
+    }
+
+    // batch_gradient of a problem with a gradient but no kernel: problem::
+    // gradient row by row (src/problem.cpp:456-467), one gevals per row
+    {
+        problem gp{grad_udp{}};
+        const auto dvs = random_dvs(gp, 37, 4);
+        const auto g = pagmo_hip::batch_gradient(gp, dvs);
+        bool same = g.size() == dvs.size();
+        for (std::size_t i = 0; i < 37u && same; ++i) {
+            const auto gi = grad_udp{}.gradient({dvs[3 * i], dvs[3 * i + 1], dvs[3 * i + 2]});
+            for (std::size_t j = 0; j < 3u; ++j) same = same && gi[j] == g[3 * i + j];
+        }
+        CHECK(same);
+        CHECK(gp.get_gevals() == 37u);
+        CHECK(pagmo_hip::batch_gradient(gp, vector_double{}).empty());
+        const auto mlen = throws<std::invalid_argument>([&] { pagmo_hip::batch_gradient(gp, vector_double(4, 0.)); });
+        CHECK(mlen.find("is not an exact multiple of the dimension of the problem, 3") != std::string::npos);
+        // a UDP without a gradient, with or without a fitness kernel, throws
+        // problem::gradient's not_implemented_error
+        const auto m4 = throws<not_implemented_error>(
+            [&] { pagmo_hip::batch_gradient(problem{user_udp{}}, vector_double(4, 0.)); });
+        CHECK(m4.find("The gradient has been requested, but it is not implemented in a UDP of type") == 0u);
+        const auto m5 = throws<not_implemented_error>(
+            [&] { pagmo_hip::batch_gradient(problem{ackley{3u}}, vector_double(6, 0.)); });
+        CHECK(m5.find("not implemented in a UDP of type 'Ackley Function'") != std::string::npos);
+    }
+
     problem unsafe{unsafe_udp{}};
     const auto msg = throws<std::invalid_argument>([&] { hip_bfe{}(unsafe, vector_double(4, 0.5)); });
     CHECK(msg.find("Cannot execute fitness evaluations in batch mode for a problem of type 'unsafe_prob'")
@@ -146,6 +223,31 @@ static double rel(double a, double b)
     return std::abs(a - b) / std::max(std::abs(b), 1e-300);
 }
 
+// 4 sum over pairs of |r^-12| + |r^-6| of one decision vector of a
+// Lennard-Jones cluster (lennard_jones.cpp:132-151 encoding): the scale of
+// the cancelling sum that the energy is
+static double lj_abs_scale(const double *x, std::size_t nx)
+{
+    const std::size_t atoms = (nx + 6u) / 3u;
+    std::vector<double> p(3u * atoms, 0.);
+    p[5] = x[0];
+    p[7] = x[1];
+    p[8] = x[2];
+    for (std::size_t i = 3; i < nx; ++i) p[6u + (i - 3u)] = x[i];
+    double s = 0.;
+    for (std::size_t i = 0; i < atoms; ++i) {
+        for (std::size_t j = i + 1u; j < atoms; ++j) {
+            double r2 = 0.;
+            for (int c = 0; c < 3; ++c) r2 += (p[3u * i + c] - p[3u * j + c]) * (p[3u * i + c] - p[3u * j + c]);
+            const double s6 = 1. / (r2 * r2 * r2);
+            s += std::abs(s6 * s6) + std::abs(s6);
+        }
+    }
+    return 4. * s;
+}
+
+// exact: bit-identical; otherwise within tol relative, or for Lennard-Jones
+// within tol of the sum of the absolute pair terms
 template <typename UDP>
 static void parity(UDP udp, std::size_t n, bool exact, double tol)
 {
@@ -155,10 +257,14 @@ static void parity(UDP udp, std::size_t n, bool exact, double tol)
     const auto a = hb(p, dvs);
     const auto b = tb(p, dvs);
     CHECK(a.size() == b.size());
+    const bool lj = std::is_same<UDP, lennard_jones>::value;
+    const std::size_t nx = p.get_nx();
     double worst = 0.;
     for (std::size_t i = 0; i < a.size(); ++i) {
         if (exact) {
             CHECK(a[i] == b[i]);
+        } else if (lj) {
+            worst = std::max(worst, a[i] == b[i] ? 0. : std::abs(a[i] - b[i]) / lj_abs_scale(&dvs[i * nx], nx));
         } else {
             worst = std::max(worst, rel(a[i], b[i]));
         }
@@ -169,6 +275,48 @@ static void parity(UDP udp, std::size_t n, bool exact, double tol)
     }
     // the wrapper counted both calls, each once (tests/thread_bfe.cpp:66-96)
     CHECK(p.get_fevals() == 2u * n);
+}
+
+// Islands call their bfe from their own threads (island.cpp:438): 8 host
+// threads, each with its own problems and batches (some large enough for the
+// pinned two-chunk upload), call hip_bfe, batch_gradient and
+// fast_non_dominated_sorting at once; every result equals the serial call's.
+static void concurrent_tests()
+{
+    struct job {
+        problem p;
+        vector_double dvs;
+        vector_double f, g;
+        pagmo::fnds_return_type nd;
+    };
+    std::vector<job> jobs;
+    for (unsigned t = 0; t < 8u; ++t) {
+        problem p = t % 3u == 0u   ? problem{rosenbrock{10u + 60u * t}}
+                    : t % 3u == 1u ? problem{rastrigin{500u + 100u * t}}
+                                   : problem{lennard_jones{20u + t}};
+        const std::size_t n = t == 1u || t == 4u ? 6000u : 700u + 50u * t; // 6000 x 600+ doubles > 16 MiB
+        jobs.push_back(job{p, random_dvs(p, n, 100u + t), {}, {}, {}});
+    }
+    std::vector<job> serial = jobs;
+    auto run = [](job &j, unsigned t) {
+        j.f = hip_bfe{}(j.p, j.dvs);
+        if (t % 3u != 1u) j.g = pagmo_hip::batch_gradient(j.p, j.dvs);
+        std::vector<vector_double> pts;
+        for (std::size_t i = 0; i + 1u < j.f.size(); i += 2u) pts.push_back({j.f[i], j.f[i + 1u], double(i % 5u)});
+        j.nd = pagmo_hip::fast_non_dominated_sorting(pts);
+    };
+    for (unsigned t = 0; t < 8u; ++t) run(serial[t], t);
+    for (int round = 0; round < 3; ++round) {
+        std::vector<job> par = jobs;
+        std::vector<std::thread> pool;
+        for (unsigned t = 0; t < 8u; ++t) pool.emplace_back([&, t] { run(par[t], t); });
+        for (auto &th : pool) th.join();
+        for (unsigned t = 0; t < 8u; ++t) {
+            CHECK(par[t].f == serial[t].f);
+            CHECK(par[t].g == serial[t].g);
+            CHECK(par[t].nd == serial[t].nd);
+        }
+    }
 }
 
 static void gpu_tests()
@@ -184,7 +332,8 @@ static void gpu_tests()
     parity(ackley{1000u}, 512, false, 1e-12);
     parity(schwefel{33u}, 700, false, 1e-12);
     parity(griewank{100u}, 700, false, 1e-12);
-    parity(lennard_jones{5u}, 300, false, 1e-9);
+    parity(lennard_jones{5u}, 300, false, 1e-12);
+    parity(lennard_jones{48u}, 400, false, 1e-12);
 
     // pso_gen with bfe equals pso_gen without (tests/pso_gen.cpp:302-318), and
     // hip_bfe reproduces thread_bfe's trajectory bit for bit on rosenbrock
@@ -462,7 +611,10 @@ int main(int argc, char **argv)
     mo_problem_cpu_tests();
     translate_cpu_tests();
     local_descent_cpu_tests();
-    if (gpu) gpu_tests();
+    if (gpu) {
+        gpu_tests();
+        concurrent_tests();
+    }
     std::printf("%s: %d failure(s)\n", gpu ? "gpu" : "cpu", g_fail);
     return g_fail ? 1 : 0;
 }

@@ -6,13 +6,15 @@ weights below one (the migration engines' draws decide), and islands of
 different problems sharing one dimension.
 
 Bars: decision vectors bit-exact (they depend on fitness only through
-comparisons); fitness values bit-exact for rosenbrock and within the
-kernels' 1e-9 (relative and absolute) otherwise, as in the pso_gen tests."""
+comparisons); fitness values at the kernels' bars (tests/parity_bars.py):
+bit-exact for rosenbrock, 1e-12 relative for the separable problems, 1e-12 of
+the sum of the absolute pair terms for Lennard-Jones."""
 import numpy as np
 import pytest
 
 import oracle as O
 import pagmo2_amd as P
+from parity_bars import assert_fitness
 
 pytestmark = pytest.mark.gpu
 
@@ -25,11 +27,12 @@ def islands(spec, gen, sel, rep):
     return out
 
 
-def same(got, ref):
+def same(kind, got, ref):
     gX, gF = got
     rX, rF = ref
     assert np.array_equal(gX, rX)
-    np.testing.assert_allclose(gF, rF, rtol=1e-9, atol=1e-9)
+    if gX.shape[0]:
+        assert_fitness(kind, gX, gF, rF)
 
 
 def device(isl, conns, mtype, handling, rounds, mode):
@@ -64,9 +67,12 @@ def test_migration_matches_oracle(topo, w, mtype, handling, sel, rep):
     print(f"migration {topo} w={w} type={mtype} handling={handling} sel={sel} rep={rep}", flush=True)
     ref, ref_db = O.archi_migrate(isl, conns, mtype, handling, rounds=4)
     got, got_db = device(isl, conns, mtype, handling, 4, P.ARCHI_BATCHED)
+    # migrants keep the fitness their source island computed, so an island's
+    # rows mix the problems of SPEC10: the separable problems' 1e-12 relative
+    # bar covers them all (rosenbrock rows are bit-exact)
     for k in range(len(isl)):
-        same(got[k], ref[k])
-        same(got_db[k], ref_db[k])
+        same(P.RASTRIGIN, got[k], ref[k])
+        same(P.RASTRIGIN, got_db[k], ref_db[k])
 
 
 def test_per_island_mode_and_lennard_jones():
@@ -78,8 +84,8 @@ def test_per_island_mode_and_lennard_jones():
     for mode in (P.ARCHI_PER_ISLAND, P.ARCHI_BATCHED):
         got, got_db = device(isl, conns, P.MIGRATION_P2P, P.MIGRANTS_PRESERVE, 3, mode)
         for k in range(3):
-            same(got[k], ref[k])
-            same(got_db[k], ref_db[k])
+            same(P.LENNARD_JONES, got[k], ref[k])
+            same(P.LENNARD_JONES, got_db[k], ref_db[k])
 
 
 def test_unconnected_islands_with_migration_set():
@@ -90,8 +96,8 @@ def test_unconnected_islands_with_migration_set():
     ref, ref_db = O.archi_migrate(isl, conns, P.MIGRATION_P2P, P.MIGRANTS_PRESERVE, rounds=2)
     got, got_db = device(isl, conns, P.MIGRATION_P2P, P.MIGRANTS_PRESERVE, 2, P.ARCHI_BATCHED)
     for k in range(3):
-        same(got[k], ref[k])
-        same(got_db[k], ref_db[k])
+        same(P.RASTRIGIN, got[k], ref[k])
+        same(P.RASTRIGIN, got_db[k], ref_db[k])
 
 
 def test_migration_off_again():
@@ -102,7 +108,7 @@ def test_migration_off_again():
     a.set_migration(None)
     a.evolve()
     for k, d in enumerate(isl):
-        same(a.get(k), O.pso_gen_evolve(d["kind"], d["dim"], d["X"], d["F"], d["gen"], d["seed"]))
+        same(d["kind"], a.get(k), O.pso_gen_evolve(d["kind"], d["dim"], d["X"], d["F"], d["gen"], d["seed"]))
     assert a.migrants(0)[1].size == 0
     a.close()
 

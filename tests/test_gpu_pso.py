@@ -10,6 +10,7 @@ import pytest
 
 import oracle as O
 import pagmo2_amd as P
+from parity_bars import assert_fitness
 
 pytestmark = pytest.mark.gpu
 
@@ -47,6 +48,12 @@ def test_host_entry_point():
     (P.ACKLEY, 9, 8, 64),
     (P.LENNARD_JONES, 5, 12, 45),
     (P.LENNARD_JONES, 40, 6, 33),
+    # one tile that does not fit LDS beside the evaluation's staging area
+    # (k_gens_one_tile<false>), with a generator of two segments (k_rng
+    # hands out every batch, no overlap)
+    (P.ROSENBROCK, 40, 256, 70),
+    # a Lennard-Jones tile of 16 individuals of 48 atoms: not staged either
+    (P.LENNARD_JONES, 48, 16, 70),
 ])
 def test_host_entry_point_matches_archipelago(kind, dim, pop, gens):
     """The one-shot entry point runs a single-tile island's generations in one
@@ -63,6 +70,51 @@ def test_host_entry_point_matches_archipelago(kind, dim, pop, gens):
     if kind == P.ROSENBROCK:
         Xo, Fo = O.pso_gen_evolve(kind, dim, X0, F0, gens, 5)
         assert np.array_equal(Xh, Xo) and np.array_equal(Fh, Fo)
+
+
+@pytest.mark.parametrize("mode", ["batched", "per_island"])
+def test_memory_flags_per_island(mode):
+    """Islands with and without pso_gen's memory flag in one archipelago: on
+    the second evolve call only the islands without it re-draw velocities
+    (pso_gen.cpp:193-200), and every island follows its own oracle run."""
+    spec = [(P.ROSENBROCK, 6, 30, False), (P.RASTRIGIN, 5, 40, True), (P.ROSENBROCK, 8, 17, True),
+            (P.LENNARD_JONES, 5, 20, False), (P.RASTRIGIN, 9, 33, False)]
+    isl = []
+    for k, (kind, dim, pop, memory) in enumerate(spec):
+        X0, F0 = O.population_init(kind, dim, pop, 300 + k)
+        isl.append(dict(kind=kind, dim=dim, X0=X0, F0=F0, memory=memory,
+                        algo=P.pso_params(gen=6, seed=310 + k, memory=memory)))
+    a = P.Archipelago(isl)
+    m = P.ARCHI_BATCHED if mode == "batched" else P.ARCHI_PER_ISLAND
+    a.evolve(m)
+    a.evolve(m)
+    for k, d in enumerate(isl):
+        xg, fg = a.get(k)
+        xo, fo = O.pso_gen_evolve(d["kind"], d["dim"], d["X0"], d["F0"], 6, 310 + k, calls=2, memory=d["memory"])
+        assert np.array_equal(xg, xo), k
+        assert_fitness(d["kind"], xg, fg, fo, k)
+    a.close()
+
+
+def test_evolve_calls_on_different_streams():
+    """Two evolve calls on two streams, issued back to back: the second waits
+    for the first (the archipelago's buffers are shared by every call), so
+    the result equals two calls on one stream."""
+    isl = uneven_islands(5, 7, gens=12)
+    ref = P.Archipelago(isl)
+    ref.evolve()
+    ref.evolve()
+    want = [ref.get(k) for k in range(len(isl))]
+    ref.close()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    for mode in (P.ARCHI_BATCHED, P.ARCHI_PER_ISLAND):
+        a = P.Archipelago(isl)
+        a.evolve(mode, stream=s1)
+        a.evolve(mode, stream=s2)
+        for k in range(len(isl)):
+            xg, fg = a.get(k)
+            assert np.array_equal(xg, want[k][0]) and np.array_equal(fg, want[k][1]), (mode, k)
+        a.close()
 
 
 def test_two_evolve_calls_continue_the_stream():
@@ -120,12 +172,7 @@ def test_islands_against_oracle():
         xg, fg = a.get(k)
         xo, fo = O.pso_gen_evolve(d["kind"], d["dim"], d["X0"], d["F0"], 5, 42 + k)
         assert np.array_equal(xg, xo), k
-        if d["kind"] == P.ROSENBROCK:
-            assert np.array_equal(fg, fo), k
-        elif d["kind"] == P.RASTRIGIN:
-            np.testing.assert_allclose(fg, fo, rtol=1e-12, atol=0)
-        else:
-            np.testing.assert_allclose(fg, fo, rtol=1e-9, atol=1e-9)
+        assert_fitness(d["kind"], xg, fg, fo, k)
     a.close()
 
 
@@ -197,7 +244,4 @@ def test_split_generators_match_the_oracle(gens):
         assert np.array_equal(xb, xp) and np.array_equal(fb, fp), k
         xo, fo = O.pso_gen_evolve(d["kind"], d["dim"], d["X0"], d["F0"], gens, 80 + k, calls=2)
         assert np.array_equal(xb, xo), k
-        if d["kind"] == P.ROSENBROCK:
-            assert np.array_equal(fb, fo), k
-        else:
-            np.testing.assert_allclose(fb, fo, rtol=1e-9, atol=1e-9)
+        assert_fitness(d["kind"], xb, fb, fo, k)

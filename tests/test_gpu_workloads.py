@@ -1,0 +1,135 @@
+"""Every benchmark workload at its full size, against the oracle.
+
+* separable bulk evaluation: rastrigin and ackley, n = 1000, 65536
+  individuals drawn from std::mt19937(42) in batch_random_decision_vector
+  order within the problems' bounds ([-5.12, 5.12] and [-15, 30]); the
+  device's draws equal the oracle's, and every fitness is within 1e-12
+  relative of the oracle's thread_bfe on all host cores;
+* Lennard-Jones, 48 atoms (nx 138), 16384 individuals uniform in [-3, 3]
+  from std::mt19937(42): fitness within 1e-12 of the sum of the absolute
+  pair terms, gradient within 1e-12 norm-wise of the oracle's analytic
+  gradient for every row; on a sample of rows also within 1e-7 of
+  estimate_gradient_h (dx = 1e-6, gradients_and_hessians.hpp:199-244) and of
+  central differences with h = 1e-6 (rows whose closest pair is at least 0.01
+  apart: the truncation error grows as (h / r_min)^2);
+* the uneven archipelago: 64 islands (pagmo2_amd.workloads), pso_gen 200
+  generations, batched launches equal to per-island launches bit for bit on
+  every island, and the first 5 generations of every island equal to the
+  oracle's pso_gen (positions bit-exact, fitness at the kernel's bar).
+The multi-objective ranking workload runs at full size in test_gpu_fnds.py
+and test_gpu_mo_sort.py; the plain method (rosenbrock n = 10, 64
+individuals, 100 generations) in test_gpu_pso.py.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pagmo2_amd as P
+from pagmo2_amd.workloads import archipelago_islands
+from parity_bars import assert_fitness, lj_positions
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+DEV = "cuda:0"
+# host threads of the oracle's thread_bfe: the box's CPU share
+THREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("kind", [P.RASTRIGIN, P.ACKLEY])
+def test_separable_bulk_full_size(kind):
+    n, dim = 65536, 1000
+    say(f"separable bulk kind={kind} {n}x{dim}: device draws and bfe")
+    x = torch.empty((n, dim), dtype=torch.float64, device=DEV)
+    P.batch_random_dv(kind, dim, n, 42, x)
+    f = torch.empty((n, 1), dtype=torch.float64, device=DEV)
+    P.bfe_eval(kind, dim, x, f)
+    torch.cuda.synchronize()
+    xh = x.cpu().numpy()
+    fh = f.cpu().numpy()[:, 0]
+    del x, f
+    say("oracle draws and thread_bfe")
+    assert np.array_equal(xh, O.batch_random_dv(kind, dim, n, 42))
+    ref = O.bfe(kind, dim, xh, threads=THREADS)[:, 0]
+    assert_fitness(kind, xh, fh, ref)
+
+
+def test_lennard_jones_full_size():
+    atoms, n = 48, 16384
+    nx = 3 * atoms - 6
+    x = O.batch_uniform(np.full(nx, -3.0), np.full(nx, 3.0), n, 42)
+    say(f"lennard_jones {atoms} atoms x {n}: fitness and gradient")
+    xd = torch.from_numpy(x).to(DEV)
+    f = torch.empty((n, 1), dtype=torch.float64, device=DEV)
+    g = torch.empty((n, nx), dtype=torch.float64, device=DEV)
+    fg = torch.empty((n, 1), dtype=torch.float64, device=DEV)
+    P.bfe_eval(P.LENNARD_JONES, atoms, xd, f)
+    P.batch_gradient(P.LENNARD_JONES, atoms, xd, g, fg)
+    torch.cuda.synchronize()
+    f, g, fg = f.cpu().numpy()[:, 0], g.cpu().numpy(), fg.cpu().numpy()[:, 0]
+    say("oracle fitness and gradient")
+    rf = O.bfe(P.LENNARD_JONES, atoms, x, threads=THREADS)[:, 0]
+    rg = O.gradient(P.LENNARD_JONES, atoms, x)
+    assert_fitness(P.LENNARD_JONES, x, f, rf)
+    assert_fitness(P.LENNARD_JONES, x, fg, rf)
+    err = np.linalg.norm(g - rg, axis=1)
+    scale = np.linalg.norm(rg, axis=1)
+    assert np.all(err <= 1e-12 * scale), np.flatnonzero(err > 1e-12 * scale)[:5]
+    # estimate_gradient_h and central differences on a sample that includes
+    # the rows with the closest pairs
+    rmin = np.empty(n)
+    for i in range(n):
+        p = lj_positions(x[i])
+        d = np.sum((p[:, None, :] - p[None, :, :]) ** 2, axis=-1)
+        d[np.diag_indices(atoms)] = np.inf
+        rmin[i] = np.sqrt(d.min())
+    order = np.argsort(rmin)
+    sample = np.unique(np.concatenate([order[:8], np.arange(0, n, n // 24)]))
+    say(f"estimate_gradient_h and central differences on {sample.size} rows")
+    for i in sample:
+        gh = O.estimate_gradient(P.LENNARD_JONES, atoms, x[i], 1e-6, True)
+        assert np.linalg.norm(gh - g[i]) <= 1e-7 * np.linalg.norm(g[i]), (i, rmin[i])
+        if rmin[i] >= 0.01:
+            fd = np.empty(nx)
+            for k in range(nx):
+                xp, xm = x[i].copy(), x[i].copy()
+                xp[k] += 1e-6
+                xm[k] -= 1e-6
+                fd[k] = (O.fitness_one(P.LENNARD_JONES, atoms, xp)[0]
+                         - O.fitness_one(P.LENNARD_JONES, atoms, xm)[0]) / 2e-6
+            assert np.linalg.norm(fd - g[i]) <= 1e-7 * np.linalg.norm(g[i]), (i, rmin[i])
+
+
+def test_uneven_archipelago_full_size():
+    isl = archipelago_islands(gens=200)
+    finals = {}
+    for mode, name in ((P.ARCHI_BATCHED, "batched"), (P.ARCHI_PER_ISLAND, "per-island")):
+        say(f"archipelago 64 islands x 200 generations, {name}")
+        a = P.Archipelago.seeded(isl)
+        a.evolve(mode)
+        finals[name] = [a.get(k) for k in range(len(isl))]
+        a.close()
+    for k in range(len(isl)):
+        xb, fb = finals["batched"][k]
+        xp, fp = finals["per-island"][k]
+        assert np.array_equal(xb, xp) and np.array_equal(fb, fp), k
+    del finals
+    say("archipelago 64 islands x 5 generations against the oracle")
+    isl5 = archipelago_islands(gens=5)
+    a = P.Archipelago.seeded(isl5)
+    a.evolve()
+    got = [a.get(k) for k in range(len(isl5))]
+    a.close()
+    for k, d in enumerate(isl5):
+        X0 = O.batch_random_dv(d["kind"], d["dim"], d["pop"], d["seed"])
+        F0 = O.bfe(d["kind"], d["dim"], X0, threads=THREADS)
+        xo, fo = O.pso_gen_evolve(d["kind"], d["dim"], X0, F0, 5, d["seed"], threads=THREADS)
+        xg, fg = got[k]
+        assert np.array_equal(xg, xo), k
+        assert_fitness(d["kind"], xg, fg, fo, k)

@@ -354,6 +354,219 @@ def test_pso_gen_single_generation_independent():
     assert np.array_equal(Fo, Fp)
 
 
+# Pure-Python fitness in the reference's operation order (rosenbrock.cpp:59-66,
+# rastrigin.cpp:62-72); math.cos is the same glibc routine the reference calls.
+def _py_rosenbrock(x):
+    f = 0.0
+    for i in range(len(x) - 1):
+        t = x[i] * x[i] - x[i + 1]
+        f = f + ((100.0 * t) * t + (x[i] - 1.0) * (x[i] - 1.0))
+    return f
+
+
+def _py_rastrigin(x):
+    omega = 2.0 * math.pi
+    f = 0.0
+    for v in x:
+        f += v * v - 10.0 * math.cos(omega * v)
+    return f + 10.0 * len(x)
+
+
+def _le_f(a, b):
+    # less_than_f(a, b) || equal_to_f(a, b), custom_comparisons.hpp:54-100
+    if math.isnan(a):
+        return math.isnan(b)
+    if math.isnan(b):
+        return True
+    return a <= b
+
+
+class _PsoPython:
+    """Independent pso_gen (variant 5, lbest) over several generations and
+    evolve calls, written from pso_gen.cpp:120-530 and :593-699.
+
+    The random words come from numpy's MT19937 (RandomState with a 32-bit
+    seed seeds like std::mt19937; the standard's 10000th-output known answer
+    pins it below), mapped by the libstdc++ canonical rule `canonical`.  The
+    `mistake` argument plants one plausible restatement error so that the
+    tests can show each of them changes the trajectory.
+    """
+
+    def __init__(self, seed, lb, ub, fit, radius=2, omega=0.7298, eta1=2.05, eta2=2.05, max_vel=0.5, memory=False,
+                 mistake=None):
+        self.words = np.random.RandomState(seed)
+        self.lb, self.ub, self.fit = list(lb), list(ub), fit
+        self.radius, self.omega, self.eta1, self.eta2, self.max_vel = radius, omega, eta1, eta2, max_vel
+        self.memory, self.mistake = memory, mistake
+        self.V = None
+        self.cognitive_nonzero = 0  # visits where lbX[p][d] - X[p][d] != 0
+
+    def draw(self, a, b):
+        u0, u1 = (int(w) for w in self.words.randint(0, 2 ** 32, size=2, dtype=np.uint64))
+        return canonical(u0, u1) * (b - a) + a
+
+    def evolve(self, X, F, gens):
+        pop, nx = len(X), len(X[0])
+        lb, ub = self.lb, self.ub
+        maxv = [(ub[j] - lb[j]) * self.max_vel for j in range(nx)]
+        minv = [-1.0 * m for m in maxv]
+        X = [list(map(float, r)) for r in X]
+        lbX = [list(r) for r in X]
+        lbfit = [float(f) for f in F]
+        if self.V is None or len(self.V) != pop or not self.memory or self.mistake == "velocity_reset":
+            self.V = [[self.draw(minv[j], maxv[j]) for j in range(nx)] for _ in range(pop)]
+        V = self.V
+        R = self.radius
+        neigh = [[(p - j) % pop for j in range(R, 0, -1)] + [(p + j) % pop for j in range(1, R + 1)]
+                 for p in range(pop)]
+        for _ in range(gens):
+            for p in range(pop):
+                b = neigh[p][0]
+                for q in neigh[p][1:]:
+                    if _le_f(lbfit[q], lbfit[b]) if self.mistake != "first_wins" else lbfit[q] < lbfit[b]:
+                        b = q
+                best = X[b] if self.mistake == "best_from_x" else lbX[b]
+                mine = X[p] if self.mistake == "cognitive_x" else lbX[p]
+                for d in range(nx):
+                    r1 = self.draw(0.0, 1.0)
+                    r2 = self.draw(0.0, 1.0)
+                    self.cognitive_nonzero += lbX[p][d] != X[p][d]
+                    V[p][d] = self.omega * (V[p][d] + self.eta1 * r1 * (mine[d] - X[p][d])
+                                            + self.eta2 * r2 * (best[d] - X[p][d]))
+            for p in range(pop):
+                for d in range(nx):
+                    if V[p][d] > maxv[d]:
+                        V[p][d] = maxv[d]
+                    elif V[p][d] < minv[d]:
+                        V[p][d] = minv[d]
+                    nxv = X[p][d] + V[p][d]
+                    if nxv < lb[d]:
+                        nxv = lb[d]
+                        V[p][d] = 0.0
+                    elif nxv > ub[d]:
+                        nxv = ub[d]
+                        V[p][d] = 0.0
+                    X[p][d] = nxv
+            fit = [self.fit(r) for r in X]
+            for p in range(pop):
+                if _le_f(fit[p], lbfit[p]):
+                    lbfit[p] = fit[p]
+                    lbX[p] = list(X[p])
+        # pop.set_xf(i, lbX[i], lbfit[i]), pso_gen.cpp:526-528
+        return np.array(lbX), np.array(lbfit)
+
+
+def test_numpy_mt19937_known_answer():
+    # the standard's known answer for std::mt19937 (10000th output, default
+    # seed 5489): pins the independent word source of _PsoPython
+    w = np.random.RandomState(5489).randint(0, 2 ** 32, size=10000, dtype=np.uint64)
+    assert int(w[-1]) == 4123659995
+
+
+def _rastrigin_pop_with_mirrors(pop, dim, seed):
+    # rastrigin is even in every coordinate: row 2k+1 = -row 2k has a
+    # bit-identical fitness at a different position, so the later-neighbour
+    # tie rule (pso_gen.cpp:615-617) decides which position is followed
+    X, _ = O.population_init(O.RASTRIGIN, dim, pop, seed)
+    X[1::2] = -X[0::2][: pop // 2]
+    F = np.array([_py_rastrigin(r) for r in X])
+    assert np.all(F[1::2] == F[0::2][: pop // 2])
+    return X, F
+
+
+@pytest.mark.parametrize("kind,dim,pop,gens,calls", [
+    (O.ROSENBROCK, 5, 12, 4, 1),
+    (O.ROSENBROCK, 3, 9, 3, 2),
+    (O.RASTRIGIN, 4, 10, 5, 1),
+])
+def test_pso_gen_generations_independent(kind, dim, pop, gens, calls):
+    """Several generations (and evolve calls) of the oracle's pso_gen against
+    the independent Python loop: from generation 2 on lbX != X for every
+    particle that did not improve, so the cognitive term eta1 r1 (lbX - X),
+    the best neighbour's memory lbX[best], the velocity carried from one
+    generation to the next and the `<=` memory rule all enter."""
+    fit = _py_rosenbrock if kind == O.ROSENBROCK else _py_rastrigin
+    if kind == O.RASTRIGIN:
+        X, F = _rastrigin_pop_with_mirrors(pop, dim, 5)
+    else:
+        X, F = O.population_init(kind, dim, pop, 9)
+        F = F[:, 0]
+        assert np.array_equal(F, [fit(r) for r in X])
+    _, _, lb, ub = O.problem_info(kind, dim)
+    seed = 77
+    Xo, Fo = O.pso_gen_evolve(kind, dim, X, F, gens, seed, calls=calls, memory=calls > 1)
+    py = _PsoPython(seed, lb, ub, fit, memory=calls > 1)
+    Xp, Fp = X, F
+    for _ in range(calls):
+        Xp, Fp = py.evolve(Xp, Fp, gens)
+    assert np.array_equal(Xo, Xp)
+    assert np.array_equal(Fo, Fp)
+    # the memory differs from the position in many visits after generation
+    # 1, so the cognitive term is not identically zero in this run
+    assert py.cognitive_nonzero > pop * dim
+
+
+@pytest.mark.parametrize("mistake", ["cognitive_x", "best_from_x", "velocity_reset", "first_wins"])
+def test_pso_gen_independent_loop_detects_mistakes(mistake):
+    """Each planted restatement error changes the result of the run above, so
+    the independent comparison would catch it (the mistakes are planted in the
+    Python loop, never in the product)."""
+    if mistake == "first_wins":
+        kind, dim, pop, gens, calls = O.RASTRIGIN, 4, 10, 5, 1
+        X, F = _rastrigin_pop_with_mirrors(pop, dim, 5)
+        fit = _py_rastrigin
+    else:
+        kind, dim, pop, gens, calls = O.ROSENBROCK, 3, 9, 3, 2
+        X, F = O.population_init(kind, dim, pop, 9)
+        F = F[:, 0]
+        fit = _py_rosenbrock
+    _, _, lb, ub = O.problem_info(kind, dim)
+    Xo, Fo = O.pso_gen_evolve(kind, dim, X, F, gens, 77, calls=calls, memory=calls > 1)
+    py = _PsoPython(77, lb, ub, fit, memory=calls > 1, mistake=mistake)
+    Xp, Fp = X, F
+    for _ in range(calls):
+        Xp, Fp = py.evolve(Xp, Fp, gens)
+    assert not (np.array_equal(Xo, Xp) and np.array_equal(Fo, Fp))
+
+
+# ---- the two-objective sphere of the multi-objective ranking workload --------
+# f_k(x) = ||x - c_k||^2 has no reference counterpart; pinned by its closed
+# forms and by a sequential sum written here.
+
+def test_mo_sphere_closed_forms():
+    rng = np.random.default_rng(3)
+    n, nobj = 30, 2
+    C = rng.uniform(0, 1, (nobj, n))
+    for k in range(nobj):
+        f = O.fitness_one(O.MO_SPHERE, n, C[k], centers=C, nobj=nobj)
+        assert f[k] == 0.0
+        other = 1 - k
+        assert f[other] == pytest.approx(float(np.sum((C[k] - C[other]) ** 2)), rel=1e-15)
+        for i in (0, 7, n - 1):
+            e = C[k].copy()
+            e[i] += 1.0
+            # (c + 1 - c)^2 = 1 exactly when c + 1 rounds back to c + 1 - c = 1
+            d = e[i] - C[k][i]
+            assert O.fitness_one(O.MO_SPHERE, n, e, centers=C, nobj=nobj)[k] == d * d
+
+
+def test_mo_sphere_sequential_sum():
+    rng = np.random.default_rng(4)
+    n, nobj, pop = 30, 3, 64
+    C = rng.uniform(0, 1, (nobj, n))
+    X = rng.uniform(-1, 2, (pop, n))
+    got = O.bfe(O.MO_SPHERE, n, X, centers=C, nobj=nobj)
+    want = np.empty((pop, nobj))
+    for p in range(pop):
+        for k in range(nobj):
+            acc = 0.0
+            for i in range(n):
+                d = float(X[p, i]) - float(C[k, i])
+                acc += d * d
+            want[p, k] = acc
+    assert np.array_equal(got, want)
+
+
 # ---- crowding distance and multi-objective sorting (multi_objective.cpp:280-465)
 
 def _f(v):
