@@ -146,6 +146,21 @@ pgh_status pgh_local_descent(const pgh_udp *udp, double *dvs, uint64_t n_dvs, ui
  * lb/ub host arrays of nx (or both NULL).  Synchronous. */
 pgh_status pgh_local_descent_host(const pgh_udp *udp, double *dvs, uint64_t dvs_len, double *fvs, uint64_t fvs_len,
                                   const double *lb, const double *ub, uint32_t iters, double step0);
+/* No counterpart in the reference.  pgh_local_descent for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same arguments and results without the
+ * work buffer.  The work lives in a library-owned device array of
+ * PGH_LOCAL_DESCENT_NOALLOC_MAX_WORK doubles (64 MiB) per GPU, made when the
+ * library's code is loaded on that GPU and held until the process ends; a
+ * batch needing more (pgh_local_descent_work_len(n_dvs, nx) beyond it) gets
+ * INVALID_ARGUMENT.  Allocates nothing.  `stream` must belong to the current
+ * device.  Every call on a GPU uses that GPU's one array: calls whose work
+ * overlaps in time on one GPU (several streams or host threads) race and give
+ * wrong results; serialise them. */
+#define PGH_LOCAL_DESCENT_NOALLOC_MAX_WORK (UINT64_C(1) << 23)
+pgh_status pgh_local_descent_noalloc(const pgh_udp *udp, double *dvs, uint64_t n_dvs, uint64_t nx, double *fvs,
+                                     const double *lb, const double *ub, uint32_t iters, double step0,
+                                     void *stream);
 
 /* ---- fast_non_dominated_sorting (src/utils/multi_objective.cpp:200-257,
  *      return type include/pagmo/utils/multi_objective.hpp:68-72) over a

@@ -30,7 +30,7 @@ EXPORTS = [
     "pgh_fnds_copy", "pgh_fnds_set_peel_blocks", "pgh_fnds_tune", "pgh_lj_set_groups", "pgh_lj_tune",
     "pgh_crowding_distance", "pgh_crowding_distance_host", "pgh_sort_population_mo", "pgh_sort_population_mo_host",
     "pgh_select_best_N_mo", "pgh_select_best_N_mo_host",
-    "pgh_local_descent_work_len", "pgh_local_descent", "pgh_local_descent_host",
+    "pgh_local_descent_work_len", "pgh_local_descent", "pgh_local_descent_host", "pgh_local_descent_noalloc",
     "pgh_mt19937_generate", "pgh_batch_random_dv",
     "pgh_archi_create", "pgh_archi_create_seeded", "pgh_archi_destroy", "pgh_archi_evolve", "pgh_archi_get",
     "pgh_archi_set_trace",
@@ -140,6 +140,7 @@ def _declare(L):
         "pgh_batch_gradient_host": [udp_p, vp, u64, vp, u64, vp],
         "pgh_local_descent": [udp_p, vp, u64, u64, vp, vp, vp, u32, C.c_double, vp, u64, vp],
         "pgh_local_descent_host": [udp_p, vp, u64, vp, u64, vp, vp, u32, C.c_double],
+        "pgh_local_descent_noalloc": [udp_p, vp, u64, u64, vp, vp, vp, u32, C.c_double, vp],
         "pgh_fnds_create": [C.POINTER(vp)],
         "pgh_fnds_thread_workspace": [C.POINTER(vp)],
         "pgh_fnds_destroy": [vp],
@@ -273,6 +274,19 @@ def local_descent(kind, dim, dvs, fvs, iters, step0, lb=None, ub=None, work=None
         work = torch.empty(max(need, 1), dtype=torch.float64, device=dvs.device)
     check(lib().pgh_local_descent(C.byref(u), dvs.data_ptr(), n, nx, fvs.data_ptr(), _dptr(lb), _dptr(ub), iters,
                                   step0, work.data_ptr(), work.numel(), _stream(stream)))
+
+
+# doubles of pgh_local_descent_noalloc's device array (PGH_LOCAL_DESCENT_NOALLOC_MAX_WORK)
+LOCAL_DESCENT_NOALLOC_MAX_WORK = 1 << 23
+
+
+def local_descent_noalloc(kind, dim, dvs, fvs, iters, step0, lb=None, ub=None, stream=None):
+    """local_descent with the work buffer in the library's own device array
+    (nothing allocated; calls overlapping on one GPU race)."""
+    u = make_udp(kind, dim)
+    n, nx = dvs.shape
+    check(lib().pgh_local_descent_noalloc(C.byref(u), dvs.data_ptr(), n, nx, fvs.data_ptr(), _dptr(lb), _dptr(ub),
+                                          iters, step0, _stream(stream)))
 
 
 def mt19937_generate(seed, n, out, skip=0, stream=None):

@@ -325,9 +325,6 @@ pgh_status resolve(const pgh_udp *udp, prob &out)
     return PGH_OK;
 }
 
-namespace
-{
-
 pgh_status check_shape(const prob &p, uint64_t nx, uint64_t nf)
 {
     if (nx != p.nx) {
@@ -344,6 +341,9 @@ pgh_status check_shape(const prob &p, uint64_t nx, uint64_t nf)
     }
     return PGH_OK;
 }
+
+namespace
+{
 
 // problem::gradient throws not_implemented_error for UDPs without a gradient
 // (problem.hpp:633-645); lennard_jones gets the new adjoint kernel.
@@ -384,6 +384,8 @@ pgh_status apply_shift(const prob &p, double *x, uint64_t n_elems, thread_ctx *c
     return launch_shift(x, n_elems, p.nx, static_cast<const double *>(ctx->e.ptr), ctx->stream);
 }
 
+} // namespace
+
 // pgh_local_descent's arguments beyond the problem and the buffers
 pgh_status check_local_descent(const prob &p, const double *lb, const double *ub, double step0)
 {
@@ -398,6 +400,9 @@ pgh_status check_local_descent(const prob &p, const double *lb, const double *ub
     }
     return PGH_OK;
 }
+
+namespace
+{
 
 // Copies host MO_SPHERE centres to the device buffer `buf` and repoints p.
 pgh_status stage_centers(prob &p, const pgh_udp *udp, dev_buf &buf, hipStream_t s)

@@ -5,6 +5,8 @@
 // handled by one wavefront and its arithmetic does not depend on the launch,
 // so results are the same for any batch composition.
 #include <cfloat>
+#include <mutex>
+#include <string>
 
 #include "pgh_internal.hpp"
 
@@ -118,4 +120,66 @@ pgh_status launch_local_descent(const prob &p, double *x, uint64_t n, double *f,
     return PGH_OK;
 }
 
+namespace
+{
+
+// The work buffer of pgh_local_descent_noalloc: one copy per device, made
+// when the code object is loaded and kept until the process ends.  In host
+// code `scratch` only names the variable; its device address on the current
+// device comes from hipGetSymbolAddress (scratch_on_current_device).
+static __device__ __align__(256) double scratch[PGH_LOCAL_DESCENT_NOALLOC_MAX_WORK];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up on the
+// first call made with that device current.
+pgh_status scratch_on_current_device(double **out)
+{
+    int dev = 0;
+    PGH_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(PGH_DEVICE_ERROR, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        if (!ptr) return fail(PGH_DEVICE_ERROR, "hipGetSymbolAddress(scratch) gave a null address");
+        g_scratch_ptr[dev] = static_cast<double *>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return PGH_OK;
+}
+
+} // namespace
 } // namespace pgh
+
+using namespace pgh;
+
+extern "C" {
+
+pgh_status pgh_local_descent_noalloc(const pgh_udp *udp, double *dvs, uint64_t n_dvs, uint64_t nx, double *fvs,
+                                     const double *lb, const double *ub, uint32_t iters, double step0, void *stream)
+try {
+    prob p;
+    PGH_TRY(resolve(udp, p));
+    PGH_TRY(check_shape(p, nx, p.nf));
+    PGH_TRY(check_local_descent(p, lb, ub, step0));
+    if (n_dvs == 0u) return PGH_OK;
+    if (!dvs || !fvs) return fail(PGH_INVALID_ARGUMENT, "null buffer");
+    // local_descent_work_len without overflow: 2 n (nx + 1) doubles
+    const uint64_t cap = PGH_LOCAL_DESCENT_NOALLOC_MAX_WORK;
+    if (n_dvs > cap / 2u / (nx + 1u)) {
+        return fail(PGH_INVALID_ARGUMENT, "pgh_local_descent_noalloc: " + std::to_string(n_dvs) + " individuals of "
+                                              + std::to_string(nx) + " dimensions need a work buffer beyond its "
+                                              + std::to_string(cap) + " doubles; use pgh_local_descent");
+    }
+    double *work = nullptr;
+    PGH_TRY(scratch_on_current_device(&work));
+    return launch_local_descent(p, dvs, n_dvs, fvs, lb, ub, iters, step0, work, static_cast<hipStream_t>(stream));
+} catch (...) {
+    return api_exception();
+}
+
+} // extern "C"

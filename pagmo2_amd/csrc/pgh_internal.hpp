@@ -65,6 +65,12 @@ uint32_t lj_groups_for(uint32_t atoms, bool grad);
 // Refuses grids beyond what one launch addresses (fitness.hip).
 pgh_status check_launch_size(uint64_t n, uint32_t nx);
 
+// ---- argument checks shared by entry points (capi.hip) ------------------------
+// The reference's messages for a decision / fitness vector of the wrong size.
+pgh_status check_shape(const prob &p, uint64_t nx, uint64_t nf);
+// pgh_local_descent's arguments beyond the problem and the buffers.
+pgh_status check_local_descent(const prob &p, const double *lb, const double *ub, double step0);
+
 // ---- memetic local step (local.hip) -------------------------------------------
 uint64_t local_descent_work_len(uint64_t n, uint64_t nx);
 pgh_status launch_local_descent(const prob &p, double *x, uint64_t n, double *f, const double *lb, const double *ub,

@@ -135,6 +135,23 @@ def test_local_descent_host_checks():
     assert x0.shape == (0, 3) and f0.shape == (0,)
 
 
+def test_local_descent_noalloc_limit():
+    # a batch whose work exceeds the library's device array is refused before
+    # any HIP call; the largest batch that fits passes that check
+    nx = 138
+    most = P.LOCAL_DESCENT_NOALLOC_MAX_WORK // (2 * (nx + 1))
+    assert P.local_descent_work_len(most, nx) <= P.LOCAL_DESCENT_NOALLOC_MAX_WORK
+    assert P.local_descent_work_len(most + 1, nx) > P.LOCAL_DESCENT_NOALLOC_MAX_WORK
+    u = P.make_udp(P.LENNARD_JONES, 48)
+    rc = P.lib().pgh_local_descent_noalloc(C.byref(u), 8, most + 1, nx, 8, None, None, 1, 0.1, None)
+    assert rc == P.INVALID_ARGUMENT and "work buffer beyond" in P.lib().pgh_last_error().decode()
+    # the same argument checks as pgh_local_descent
+    ua = P.make_udp(P.ACKLEY, 3)
+    assert P.lib().pgh_local_descent_noalloc(C.byref(ua), 8, 1, 3, 8, None, None, 1, 0.1, None) == P.UNSUPPORTED
+    assert P.lib().pgh_local_descent_noalloc(C.byref(u), 8, 1, 137, 8, None, None, 1, 0.1, None) == P.INVALID_ARGUMENT
+    assert P.lib().pgh_local_descent_noalloc(C.byref(u), None, 0, nx, None, None, None, 1, 0.1, None) == P.OK
+
+
 
 def test_zdt_dtlz_descriptions(kats):
     # shapes (zdt.cpp:108-157, dtlz.cpp:116-119) and the constructors' messages
