@@ -282,7 +282,7 @@ typedef struct pgh_island_spec {
 } pgh_island_spec;
 
 enum pgh_archi_mode {
-    PGH_ARCHI_BATCHED = 0,   /* every phase of a generation is one launch over all islands, graph-captured */
+    PGH_ARCHI_BATCHED = 0,   /* a generation of all islands is one launch (or one per problem class), graph-captured */
     PGH_ARCHI_PER_ISLAND = 1 /* each island launches its own kernels, one island after another */
 };
 
@@ -306,6 +306,39 @@ pgh_status pgh_archi_get(pgh_archi *a, uint64_t k, double *X, double *F);
  * when the archipelago was created with tracing on; see pgh_archi_set_trace. */
 pgh_status pgh_archi_set_trace(pgh_archi *a, int on);
 pgh_status pgh_archi_get_trace(pgh_archi *a, uint64_t k, double *trace_F);
+
+/* Launch mapping of a generation (no counterpart in the reference; no
+ * result depends on it): a generation moves, evaluates and updates the memory
+ * of every island's individuals in items of rows, handed out by a queue to a
+ * persistent grid.
+ *   form:          PGH_ARCHI_STEP_ONE, one launch per generation over every
+ *                  item; PGH_ARCHI_STEP_SPLIT, one launch per problem class
+ *                  (rosenbrock, separable, lennard_jones), each with its own
+ *                  LDS and register footprint (batched mode only)
+ *   lj_lds:        LDS bytes for a Lennard-Jones item's staged individuals
+ *                  (as many individuals per item as fit, at least one; 0 =
+ *                  64 KiB, at most 64 KiB)
+ *   blocks_per_cu: persistent workgroups per CU and launch (0 = as many as
+ *                  can be resident)
+ * pgh_archi_tune times candidate mappings (form x lj_lds in {64, 32, 16}
+ * KiB x blocks_per_cu in {0, 2}) with one batched evolve call each, `reps`
+ * times, from the archipelago's current state, which it restores afterwards;
+ * every candidate must leave a population bit-identical to the first's
+ * (else DEVICE_ERROR).  It keeps the fastest and returns it in *chosen.
+ * Synchronous; refused while migration is on. */
+enum pgh_archi_step_form {
+    PGH_ARCHI_STEP_ONE = 0,
+    PGH_ARCHI_STEP_SPLIT = 1
+};
+typedef struct pgh_archi_mapping {
+    uint32_t form;
+    uint32_t lj_lds;
+    uint32_t blocks_per_cu;
+    uint32_t pad;
+} pgh_archi_mapping;
+pgh_status pgh_archi_set_mapping(pgh_archi *a, const pgh_archi_mapping *m);
+pgh_status pgh_archi_get_mapping(pgh_archi *a, pgh_archi_mapping *m);
+pgh_status pgh_archi_tune(pgh_archi *a, uint32_t reps, pgh_archi_mapping *chosen);
 
 /* ---- migration between the islands (island::evolve, src/island.cpp:452-642;
  *      the migrants database, src/archipelago.cpp:658-713).  Once set, every

@@ -21,6 +21,7 @@ LIB_PATH = os.environ.get("PGH_LIB") or os.path.join(HERE, "libpagmo_hip.so")
 OK, INVALID_ARGUMENT, OVERFLOW, UNSUPPORTED, DEVICE_ERROR, NO_DEVICE, OUT_OF_MEMORY = range(7)
 ROSENBROCK, RASTRIGIN, ACKLEY, SCHWEFEL, GRIEWANK, LENNARD_JONES, MO_SPHERE, ZDT, DTLZ = range(1, 10)
 ARCHI_BATCHED, ARCHI_PER_ISLAND = 0, 1
+ARCHI_STEP_ONE, ARCHI_STEP_SPLIT = 0, 1
 
 # every symbol include/pagmo_hip/pgh.h declares
 EXPORTS = [
@@ -35,6 +36,7 @@ EXPORTS = [
     "pgh_archi_create", "pgh_archi_create_seeded", "pgh_archi_destroy", "pgh_archi_evolve", "pgh_archi_get",
     "pgh_archi_set_trace",
     "pgh_archi_get_trace", "pgh_archi_set_migration", "pgh_archi_get_migrants", "pgh_pso_gen_evolve_host",
+    "pgh_archi_set_mapping", "pgh_archi_get_mapping", "pgh_archi_tune",
 ]
 
 
@@ -57,6 +59,10 @@ class pgh_pso_params(C.Structure):
 
 class pgh_island_spec(C.Structure):
     _fields_ = [("udp", pgh_udp), ("pop", C.c_uint64), ("algo", pgh_pso_params)]
+
+
+class pgh_archi_mapping(C.Structure):
+    _fields_ = [("form", C.c_uint32), ("lj_lds", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class pgh_migr_rate(C.Structure):
@@ -167,6 +173,9 @@ def _declare(L):
         "pgh_archi_get": [vp, u64, vp, vp],
         "pgh_archi_set_trace": [vp, ci],
         "pgh_archi_get_trace": [vp, u64, vp],
+        "pgh_archi_set_mapping": [vp, C.POINTER(pgh_archi_mapping)],
+        "pgh_archi_get_mapping": [vp, C.POINTER(pgh_archi_mapping)],
+        "pgh_archi_tune": [vp, u32, C.POINTER(pgh_archi_mapping)],
         "pgh_archi_set_migration": [vp, ci, ci, vp, vp, vp, C.POINTER(pgh_migr_rate), C.POINTER(pgh_migr_rate),
                                     C.POINTER(u32)],
         "pgh_archi_get_migrants": [vp, u64, C.POINTER(u64), vp, vp],
@@ -484,6 +493,22 @@ class Archipelago:
 
     def evolve(self, mode=ARCHI_BATCHED, stream=None):
         check(lib().pgh_archi_evolve(self.h, mode, _stream(stream) if stream is not None else None))
+
+    def set_mapping(self, form=ARCHI_STEP_SPLIT, lj_lds=0, blocks_per_cu=0):
+        m = pgh_archi_mapping(form, lj_lds, blocks_per_cu, 0)
+        check(lib().pgh_archi_set_mapping(self.h, C.byref(m)))
+
+    def mapping(self):
+        m = pgh_archi_mapping()
+        check(lib().pgh_archi_get_mapping(self.h, C.byref(m)))
+        return dict(form=m.form, lj_lds=m.lj_lds, blocks_per_cu=m.blocks_per_cu)
+
+    def tune(self, reps=1):
+        """Times the candidate launch mappings from the current state (which it
+        restores) and keeps the fastest; returns it."""
+        m = pgh_archi_mapping()
+        check(lib().pgh_archi_tune(self.h, reps, C.byref(m)))
+        return dict(form=m.form, lj_lds=m.lj_lds, blocks_per_cu=m.blocks_per_cu)
 
     def get(self, k):
         pop, nx = self.shapes[k]

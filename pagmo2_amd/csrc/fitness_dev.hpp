@@ -398,6 +398,46 @@ __device__ __forceinline__ double rosenbrock_eval(const double *__restrict__ x, 
     return retval;
 }
 
+// Rosenbrock fitness of `rows` (1..8) consecutive rows of nx elements (at
+// `first`, row stride nx) by one wavefront: the lanes take column chunks of
+// 64, each lane computing one element's term for all rows (contiguous loads),
+// the terms go to the wavefront's LDS tile (8 x kRosenWaveStride doubles) and
+// lane r (r < rows) adds row r's terms in the reference's order
+// (rosenbrock.cpp:62-64), so the sum is rosenbrock_eval's bit for bit.  The
+// value is valid in lanes r < rows.  Clamped indices keep every load in the
+// rows given; a clamped term is never summed.
+constexpr uint32_t kRosenWaveRows = 8, kRosenWaveStride = 65; // odd stride: the summing lanes hit distinct banks
+
+__device__ __forceinline__ double rosenbrock_wave_rows(const double *__restrict__ first, uint32_t rows, uint32_t nx,
+                                                       double *__restrict__ tile, uint32_t lane)
+{
+    constexpr uint32_t W = 64;
+    double acc = 0.;
+    for (uint32_t c0 = 0; c0 < nx; c0 += W) {
+        const uint32_t ii = min(c0 + lane, nx - 1u), in = min(ii + 1u, nx - 1u);
+        double xi[kRosenWaveRows], xn[kRosenWaveRows];
+#pragma unroll
+        for (uint32_t r = 0; r < kRosenWaveRows; ++r) {
+            const double *x = first + static_cast<uint64_t>(min(r, rows - 1u)) * nx;
+            xi[r] = x[ii];
+            xn[r] = x[in];
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kRosenWaveRows; ++r) {
+            const double d = xi[r] * xi[r] - xn[r];
+            tile[r * kRosenWaveStride + lane] = 100. * d * d + (xi[r] - 1) * (xi[r] - 1);
+        }
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t cnt = c0 + 1u < nx ? min(W, nx - 1u - c0) : 0u;
+        if (lane < rows) {
+            const double *t = tile + lane * kRosenWaveStride;
+            for (uint32_t j = 0; j < cnt; ++j) acc += t[j];
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    return acc;
+}
+
 // rosenbrock.cpp:83-92, element i of m_dim from its neighbours xm = x[i-1]
 // (unused for i = 0), x0 = x[i] and xp = x[i+1] (unused for i = m_dim - 1).
 // Both parts are always evaluated (no branch, so that callers' loads of xm

@@ -47,13 +47,17 @@ struct island_desc {
     uint64_t woff; // word buffer offset (u32)
     uint64_t toff; // trace offset (doubles), gens x pop
     uint64_t eoff; // cumulative pop*nx of the islands before this one
-    // the generator split in `segs` segments (k_rng): of every batch of
-    // kGenBatch generations, segment s hands out the words of generations
-    // s gps .. (s + 1) gps - 1; seg0 is the first segment's index; j_* index
-    // the archipelago's jump table (z^n0, z^(gps W 2^b) for b = 0..,
-    // z^((segs - 1) gps W))
+    // the generator split in `segs` segments: a batch of gps generations
+    // (kGenBatch, or all of a call's when fewer) is bwords = gps W words, and
+    // segment s hands out its words s swords .. (s + 1) swords - 1 (the last
+    // segment the rest); seg0 is the first segment's index; j_* index the
+    // archipelago's jump table: z^n0 (the velocity draws), z^(swords 2^b) for
+    // b = 0.. (to segment s), z^(bwords - swords) and z^((segs - 1) swords)
+    // (from the end of a full segment, and of the last one, to the same
+    // segment of the next batch)
+    uint64_t bwords, swords;
     uint32_t segs, seg0, gps;
-    uint32_t j_n0, j_tree, j_main;
+    uint32_t j_n0, j_tree, j_main, j_last;
 };
 
 struct seg_desc {
@@ -68,11 +72,14 @@ namespace
 {
 
 constexpr uint32_t kBlock = 256;
-// k_rng hands out the words of kGenBatch generations at a time; an island's
-// generator is split in segments (k_rng) so that each hands out at most
-// kSegWords words of a batch, or a single generation's
+// The generators hand out the words of kGenBatch generations at a time; an
+// island's generator is split in segments of a fixed number of words (a power
+// of two chosen per archipelago between kSegMin and kSegMax words) so that
+// the batch's words are made by many workgroups at once
 constexpr uint32_t kGenBatch = 32;
-constexpr uint64_t kSegWords = uint64_t(1) << 20;
+constexpr uint64_t kSegMin = uint64_t(1) << 16, kSegMax = uint64_t(1) << 22, kSoloWords = uint64_t(1) << 20;
+// k_rng_gen's workgroup: the generator steps 227 words at a time (mt_run)
+constexpr uint32_t kMtGenBlock = 256;
 
 // "a <= b" in the sense of less_than_f(a, b) || equal_to_f(a, b)
 // (custom_comparisons.hpp:54-100): NaN is larger than everything but itself.
@@ -120,41 +127,57 @@ __device__ __forceinline__ void store_state(mt_state *__restrict__ dst, const ui
 }
 
 // The island's std::mt19937 draws, by its segments (one workgroup each).
-// phase 0, the start of an evolve call: every segment takes the island's
-// canonical generator (everything handed out so far); segment 0 hands out
-// the velocity draws (2 words per element, pso_gen.cpp:193-200) unless the
-// island keeps its velocities (memory, once they exist: v_valid), and every
-// segment moves to the start of its first generation, s gps
-// (jumps z^n0 and z^(gps W 2^b) over the bits of s).  phase 1, the batch of
-// generations g0 .. g0 + kGenBatch - 1 (4 words per element and generation,
-// r1 and r2 of pso_gen.cpp:300-301): segment s hands out the words of its
-// gps generations (those before the island's last) into their slots of the
-// word buffer, then jumps over the other segments' generations to its share
-// of the next batch.  The segment that hands out the island's last
-// generation stops there and becomes the canonical generator.  The words are
-// those of one sequential generator, in order.
+// Every word of an evolve call has a position in the island's one sequential
+// stream: first the velocity draws (n0 = 2 words per element,
+// pso_gen.cpp:193-200, unless the island keeps its velocities: memory, once
+// they exist), then generation g's 4 words per element (r1 and r2 of
+// pso_gen.cpp:300-301) at n0 + g W.  Batch b holds generations
+// b gps .. (b + 1) gps - 1, and segment s of the batch hands out its words
+// s swords .. (s + 1) swords - 1 (the last segment the rest) into their place
+// in the word buffer.
+//
+// k_rng phase 0, the start of an evolve call: every segment takes the
+// island's canonical generator (everything handed out so far); segment 0
+// hands out the velocity draws and every segment moves to its share of the
+// first batch, n0 + s swords (jumps z^n0 and z^(swords 2^b) over the bits of
+// s).  k_rng_gen, batch b: each segment hands out its share; the segment
+// that hands out the island's last word keeps the generator there as the
+// canonical one.  k_rng phase 2, after batch b when another follows: each
+// segment with more to hand out jumps from the end of its share to its share
+// of batch b + 1.  The words are those of one sequential generator, in order.
+__device__ __forceinline__ void seg_span(const island_desc &di, uint32_t s, uint32_t b, uint64_t n0, uint64_t &start,
+                                         uint64_t &count, uint64_t &end)
+{
+    const uint64_t W = 4ull * di.pop * di.nx;
+    end = n0 + static_cast<uint64_t>(di.gens) * W; // one past the island's last word of the call
+    start = n0 + static_cast<uint64_t>(b) * di.bwords + static_cast<uint64_t>(s) * di.swords;
+    const uint64_t share = s + 1u < di.segs ? di.swords : di.bwords - static_cast<uint64_t>(s) * di.swords;
+    count = start < end ? min(share, end - start) : 0u;
+}
+
+__device__ __forceinline__ uint64_t velocity_words(const island_desc &di, uint32_t v_valid)
+{
+    return (di.memory && v_valid) ? 0u : 2ull * di.pop * di.nx;
+}
+
 __global__ __launch_bounds__(kMtJumpBlock) void k_rng(const island_desc *__restrict__ d,
                                                       const seg_desc *__restrict__ sd, mt_state *__restrict__ st,
                                                       mt_state *__restrict__ segst, jump_tab jt,
                                                       uint32_t *__restrict__ words, uint32_t sg0, uint32_t phase,
-                                                      uint32_t g0, uint32_t v_valid)
+                                                      uint32_t b, uint32_t v_valid)
 {
     __shared__ uint32_t ring[kMtRing];
     __shared__ uint4 lds_idx[kMtIdxStride / 8u];
     const seg_desc sg = sd[sg0 + blockIdx.x];
     const island_desc di = d[sg.island];
-    const uint64_t elems = static_cast<uint64_t>(di.pop) * di.nx;
-    const uint64_t W = 4u * elems;
     const auto jump = [&](uint64_t &K, uint64_t &C, uint64_t J, uint32_t j) {
         mt_jump(ring, K, C, J, jt.lists + static_cast<uint64_t>(j) * kMtIdxStride, jt.lens[j], lds_idx);
     };
+    const uint64_t n0 = velocity_words(di, v_valid);
     uint64_t K, C;
     mt_state *mine = &segst[di.seg0 + sg.s];
     if (phase == 0u) {
         load_state(&st[sg.island], ring, K, C);
-        // velocities are drawn unless the island keeps them (pso_gen.cpp:193-200)
-        const bool draw_v = !(di.memory && v_valid);
-        const uint64_t n0 = draw_v ? 2u * elems : 0u;
         if (sg.s == 0u) {
             if (n0) {
                 mt_run(ring, K, C, n0, words + di.woff);
@@ -163,26 +186,53 @@ __global__ __launch_bounds__(kMtJumpBlock) void k_rng(const island_desc *__restr
             if (di.gens == 0u) store_state(&st[sg.island], ring, K, C);
         } else {
             if (n0) jump(K, C, n0, di.j_n0);
-            for (uint32_t b = 0; (sg.s >> b) != 0u; ++b) {
-                if ((sg.s >> b) & 1u) jump(K, C, (di.gps * W) << b, di.j_tree + b);
+            for (uint32_t bit = 0; (sg.s >> bit) != 0u; ++bit) {
+                if ((sg.s >> bit) & 1u) jump(K, C, di.swords << bit, di.j_tree + bit);
             }
         }
         store_state(mine, ring, K, C);
         return;
     }
-    const uint32_t first = g0 + sg.s * di.gps;
-    if (first >= di.gens) return; // no generation of this batch, nor of a later one
-    const uint32_t last = min(first + di.gps, di.gens); // one past
+    // phase 2: after batch b, to batch b + 1 (a one-segment generator ends
+    // its share where the next begins)
+    if (di.segs == 1u) return;
+    uint64_t start, count, end;
+    seg_span(di, sg.s, b, n0, start, count, end);
+    if (count == 0u || start + count == end || start + di.bwords >= end) return;
     load_state(mine, ring, K, C);
-    const uint64_t n = (last - first) * W;
-    mt_run(ring, K, C, n, words + di.woff + static_cast<uint64_t>(first - g0) * W);
-    C += n;
-    if (last == di.gens) {
-        store_state(&st[sg.island], ring, K, C);
-    } else if (first + kGenBatch < di.gens) {
-        if (di.segs > 1u) jump(K, C, static_cast<uint64_t>(di.segs - 1u) * di.gps * W, di.j_main);
-        store_state(mine, ring, K, C);
+    if (sg.s + 1u < di.segs) {
+        jump(K, C, di.bwords - di.swords, di.j_main);
+    } else {
+        jump(K, C, static_cast<uint64_t>(di.segs - 1u) * di.swords, di.j_last);
     }
+    store_state(mine, ring, K, C);
+}
+
+// Batch b's words of one segment (see k_rng).
+__device__ __forceinline__ void rng_gen_segment(const island_desc &di, uint32_t s, uint32_t b, uint64_t n0,
+                                                mt_state *__restrict__ st, mt_state *__restrict__ mine,
+                                                uint32_t *__restrict__ batch_words, uint32_t *ring)
+{
+    uint64_t start, count, end;
+    seg_span(di, s, b, n0, start, count, end);
+    if (count == 0u) return;
+    uint64_t K, C;
+    load_state(mine, ring, K, C);
+    mt_run(ring, K, C, count, batch_words + static_cast<uint64_t>(s) * di.swords);
+    C += count;
+    store_state(start + count == end ? st : mine, ring, K, C);
+}
+
+__global__ __launch_bounds__(kMtGenBlock) void k_rng_gen(const island_desc *__restrict__ d,
+                                                         const seg_desc *__restrict__ sd, mt_state *__restrict__ st,
+                                                         mt_state *__restrict__ segst, uint32_t *__restrict__ words,
+                                                         uint32_t sg0, uint32_t b, uint32_t v_valid)
+{
+    __shared__ uint32_t ring[kMtRing];
+    const seg_desc sg = sd[sg0 + blockIdx.x];
+    const island_desc di = d[sg.island];
+    rng_gen_segment(di, sg.s, b, velocity_words(di, v_valid), &st[sg.island], &segst[di.seg0 + sg.s],
+                    words + di.woff, ring);
 }
 
 // Start of an evolve call (pso_gen.cpp:183-200): V = U(minv, maxv) per
@@ -192,7 +242,8 @@ __global__ __launch_bounds__(kBlock) void k_vinit(const island_desc *__restrict_
                                                   uint64_t e0, uint64_t total, const uint32_t *__restrict__ words,
                                                   double *__restrict__ X, double *__restrict__ V,
                                                   double *__restrict__ lbX, double *__restrict__ fit,
-                                                  double *__restrict__ lbfit, uint32_t v_valid, uint32_t from_lb)
+                                                  double *__restrict__ lbfit, uint32_t *__restrict__ selA,
+                                                  uint32_t v_valid, uint32_t from_lb)
 {
     const uint64_t t = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (t >= total) return;
@@ -210,6 +261,7 @@ __global__ __launch_bounds__(kBlock) void k_vinit(const island_desc *__restrict_
         V[di.xoff + m] = mt_uniform(w[0], w[1], minv, maxv);
     }
     const uint64_t p = m / di.nx;
+    if (j == 0u) selA[di.foff + p] = 0u; // every row's memory starts in lbX0 (k_step)
     if (from_lb) {
         X[di.xoff + m] = lbX[di.xoff + m];
         if (j == 0u) fit[di.foff + p] = lbfit[di.foff + p];
@@ -219,18 +271,11 @@ __global__ __launch_bounds__(kBlock) void k_vinit(const island_desc *__restrict_
     }
 }
 
-// Velocity and position of element m of island di in generation gen.
-__device__ __forceinline__ void update_elem(const island_desc &di, uint64_t m, const uint32_t *__restrict__ words,
-                                            double *__restrict__ X, double *__restrict__ V,
-                                            const double *__restrict__ lbX, const double *__restrict__ lbfit,
-                                            uint32_t gen)
+// Best neighbour of particle p on the lbest ring (pso_gen.cpp:679-699):
+// radius..1 to the left, then 1..radius to the right; the later neighbour
+// wins ties (:615-617).  lbf: the island's memory fitness.
+__device__ __forceinline__ uint32_t best_neighbour(uint32_t p, uint32_t S, uint32_t R, const double *__restrict__ lbf)
 {
-    const uint32_t p = static_cast<uint32_t>(m / di.nx);
-    const uint32_t dd = static_cast<uint32_t>(m - static_cast<uint64_t>(p) * di.nx);
-    const uint32_t S = di.pop, R = di.radius;
-    const double *lbf = lbfit + di.foff;
-    // lbest ring (pso_gen.cpp:679-699): radius..1 to the left, then 1..radius
-    // to the right; the later neighbour wins ties (:615-617)
     uint32_t bn = p < R ? p - R + S : p - R;
     for (uint32_t j = R - 1u; j >= 1u; --j) {
         const uint32_t q = p < j ? p - j + S : p - j;
@@ -240,13 +285,16 @@ __device__ __forceinline__ void update_elem(const island_desc &di, uint64_t m, c
         const uint32_t q = p + j >= S ? p + j - S : p + j;
         if (le_f(lbf[q], lbf[bn])) bn = q;
     }
-    const uint32_t *w = words + di.woff + static_cast<uint64_t>(gen % kGenBatch) * (4ull * di.pop * di.nx) + 4u * m;
-    const double r1 = mt_canonical(w[0], w[1]);
-    const double r2 = mt_canonical(w[2], w[3]);
-    const uint64_t xi = di.xoff + m;
-    const double x = X[xi];
-    const double best = lbX[di.xoff + static_cast<uint64_t>(bn) * di.nx + dd];
-    double v = di.omega * (V[xi] + di.eta1 * r1 * (lbX[xi] - x) + di.eta2 * r2 * (best - x));
+    return bn;
+}
+
+// Variant 5 velocity (pso_gen.cpp:302-305), its clamp (:336-342) and the
+// bounded move (:345-361) of dimension dd: returns the new position, v the new
+// velocity.
+__device__ __forceinline__ double pso_move(const island_desc &di, uint32_t dd, double x, double v_old, double lbx,
+                                           double best, double r1, double r2, double &v)
+{
+    v = di.omega * (v_old + di.eta1 * r1 * (lbx - x) + di.eta2 * r2 * (best - x));
     double lb, ub;
     bounds_of(di.kind, dd, lb, ub);
     const double vwidth = (ub - lb) * di.max_vel;
@@ -264,23 +312,34 @@ __device__ __forceinline__ void update_elem(const island_desc &di, uint64_t m, c
         nxv = ub;
         v = 0.;
     }
-    X[xi] = nxv;
-    V[xi] = v;
+    return nxv;
 }
 
-__global__ __launch_bounds__(kBlock) void k_update(const island_desc *__restrict__ d, uint32_t i0, uint32_t i1,
-                                                   uint64_t e0, uint64_t total, const uint32_t *__restrict__ words,
-                                                   double *__restrict__ X, double *__restrict__ V,
-                                                   const double *__restrict__ lbX, const double *__restrict__ lbfit,
-                                                   uint32_t gen)
+// The r1, r2 words of element m in generation gen (4 words per element and
+// generation, in draw order).
+__device__ __forceinline__ const uint32_t *gen_words(const island_desc &di, const uint32_t *__restrict__ words,
+                                                     uint32_t gen, uint64_t m)
 {
-    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    if (t >= total) return;
-    const uint64_t e = e0 + t;
-    const uint32_t k = find_island(d, i0, i1, e);
-    const island_desc di = d[k];
-    if (gen >= di.gens) return;
-    update_elem(di, e - di.eoff, words, X, V, lbX, lbfit, gen);
+    return words + di.woff + static_cast<uint64_t>(gen % kGenBatch) * (4ull * di.pop * di.nx) + 4u * m;
+}
+
+// Velocity and position of element m of island di in generation gen.
+__device__ __forceinline__ void update_elem(const island_desc &di, uint64_t m, const uint32_t *__restrict__ words,
+                                            double *__restrict__ X, double *__restrict__ V,
+                                            const double *__restrict__ lbX, const double *__restrict__ lbfit,
+                                            uint32_t gen)
+{
+    const uint32_t p = static_cast<uint32_t>(m / di.nx);
+    const uint32_t dd = static_cast<uint32_t>(m - static_cast<uint64_t>(p) * di.nx);
+    const uint32_t bn = best_neighbour(p, di.pop, di.radius, lbfit + di.foff);
+    const uint32_t *w = gen_words(di, words, gen, m);
+    const double r1 = mt_canonical(w[0], w[1]);
+    const double r2 = mt_canonical(w[2], w[3]);
+    const uint64_t xi = di.xoff + m;
+    const double best = lbX[di.xoff + static_cast<uint64_t>(bn) * di.nx + dd];
+    double v;
+    X[xi] = pso_move(di, dd, X[xi], V[xi], lbX[xi], best, r1, r2, v);
+    V[xi] = v;
 }
 
 struct eval_ctx {
@@ -447,14 +506,356 @@ __device__ __forceinline__ void eval_tile(const island_desc &di, const tile &tl,
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_eval(const island_desc *__restrict__ d, const tile *__restrict__ tiles,
-                                                 uint32_t t0, eval_ctx c)
+// ---- one generation of every island in one launch (k_step) ------------------
+// A generation's work is split in items: rows [first, first + count) of one
+// island.  A workgroup takes an item, moves its rows (best neighbour, velocity,
+// position), evaluates them and updates their memory; a generation is one
+// launch of workgroups that take items from a queue until it is empty.
+//
+// Items of different islands, and of the same island, run concurrently.  A
+// particle's move reads the memory (lbX, lbfit) of its neighbours, which
+// other items of the same generation update; the reference moves every
+// particle before it evaluates any (pso_gen.cpp:232-449).  So the memory is
+// kept twice: lbfit in two buffers, read from one and written to the other
+// in alternate generations, and each row's lbX in one of two buffers, named
+// by a per-row selector (itself kept in two buffers like lbfit).  A row that
+// improves is written to the buffer its selector does not name, and the next
+// selector names that buffer; a row that does not improve keeps its buffer.
+// So no generation writes anything that the same generation reads from
+// another item, and the arithmetic per element is that of the launch-per-phase
+// evolve (update_elem, eval_tile), bit for bit.
+
+enum : uint32_t { kStepRosen = 0u, kStepSep = 1u, kStepLj = 2u, kStepAll = 3u };
+
+__host__ __device__ constexpr uint32_t step_class(int kind)
+{
+    return kind == ROSENBROCK ? kStepRosen : (kind == LENNARD_JONES ? kStepLj : kStepSep);
+}
+
+// LDS of a rosenbrock item: each wavefront's term tile (rosenbrock_wave_rows)
+constexpr uint32_t kStepRosenRows = (kBlock / 64u) * kRosenWaveRows;
+constexpr size_t kStepRosenLds = (kBlock / 64u) * kRosenWaveRows * kRosenWaveStride * sizeof(double);
+
+struct step_args {
+    const island_desc *d;
+    const tile *items;
+    const uint32_t *order; // item indices in the order they are handed out; null: base + i
+    uint32_t base, n_items;
+    uint32_t *queue; // this launch's counter, zero before the launch
+    const uint32_t *words;
+    double *X, *V, *lbX0, *lbX1, *fit, *trace;
+    const double *lbf_cur;
+    double *lbf_nxt;
+    const uint32_t *sel_cur;
+    uint32_t *sel_nxt;
+    uint32_t gen;
+};
+
+// Per row of an item (at most kBlock rows): its best neighbour and the
+// memory rows of the particle and of that neighbour, found once per row
+// rather than once per element.
+struct step_rows {
+    uint32_t bn[kBlock];
+    uint32_t sel[kBlock]; // bit 0: the particle's memory is in lbX1, bit 1: the neighbour's
+};
+
+// Moves the rows of item tl (update_elem's arithmetic, the memory read through
+// the selectors).  All threads of the workgroup; a __syncthreads() must
+// follow.
+__device__ __forceinline__ void step_move(const island_desc &di, const tile &tl, const step_args &a, step_rows &rw)
+{
+    const double *lbf = a.lbf_cur + di.foff;
+    const uint32_t *sel = a.sel_cur + di.foff;
+    for (uint32_t t = threadIdx.x; t < tl.count; t += kBlock) {
+        const uint32_t p = tl.first + t;
+        const uint32_t bn = best_neighbour(p, di.pop, di.radius, lbf);
+        rw.bn[t] = bn;
+        rw.sel[t] = (sel[p] ? 1u : 0u) | (sel[bn] ? 2u : 0u);
+    }
+    __syncthreads();
+    const uint64_t e0 = static_cast<uint64_t>(tl.first) * di.nx;
+    const uint64_t e1 = e0 + static_cast<uint64_t>(tl.count) * di.nx;
+    for (uint64_t m = e0 + threadIdx.x; m < e1; m += kBlock) {
+        const uint32_t p = static_cast<uint32_t>(m / di.nx);
+        const uint32_t dd = static_cast<uint32_t>(m - static_cast<uint64_t>(p) * di.nx);
+        const uint32_t t = p - tl.first;
+        const uint32_t bn = rw.bn[t], sb = rw.sel[t];
+        const uint4 w = *reinterpret_cast<const uint4 *>(gen_words(di, a.words, a.gen, m));
+        const double r1 = mt_canonical(w.x, w.y);
+        const double r2 = mt_canonical(w.z, w.w);
+        const uint64_t xi = di.xoff + m;
+        const double best = ((sb & 2u) ? a.lbX1 : a.lbX0)[di.xoff + static_cast<uint64_t>(bn) * di.nx + dd];
+        const double lbx = ((sb & 1u) ? a.lbX1 : a.lbX0)[xi];
+        double v;
+        a.X[xi] = pso_move(di, dd, a.X[xi], a.V[xi], lbx, best, r1, r2, v);
+        a.V[xi] = v;
+    }
+}
+
+// Memory update of particle p with fitness f (pso_gen.cpp:443-449) by the L
+// lanes of its group (lane r copies elements r, r + L, ...), and its fitness
+// and trace.
+__device__ __forceinline__ void step_remember(const island_desc &di, const step_args &a, uint32_t p, double f,
+                                              uint32_t r, uint32_t L)
+{
+    const uint64_t fi = di.foff + p;
+    const double lbf = a.lbf_cur[fi];
+    const uint32_t s = a.sel_cur[fi];
+    if (le_f(f, lbf)) {
+        const uint64_t row = di.xoff + static_cast<uint64_t>(p) * di.nx;
+        double *dst = (s ? a.lbX0 : a.lbX1) + row;
+        for (uint32_t i = r; i < di.nx; i += L) dst[i] = a.X[row + i];
+        if (r == 0u) {
+            a.lbf_nxt[fi] = f;
+            a.sel_nxt[fi] = s ^ 1u;
+        }
+    } else if (r == 0u) {
+        a.lbf_nxt[fi] = lbf;
+        a.sel_nxt[fi] = s;
+    }
+    if (r == 0u) {
+        a.fit[fi] = f;
+        if (a.trace) a.trace[di.toff + static_cast<uint64_t>(a.gen) * di.pop + p] = f;
+    }
+}
+
+// Rosenbrock: 8 rows per wavefront (rosenbrock_wave_rows), then the memory
+// update by the wavefront's 64 lanes, row by row.
+__device__ __forceinline__ void step_eval_rosen(const island_desc &di, const tile &tl, const step_args &a,
+                                                double *lds)
+{
+    const uint32_t wave = threadIdx.x / 64u, lane = threadIdx.x % 64u;
+    const uint32_t r0 = wave * kRosenWaveRows;
+    if (r0 >= tl.count) return; // wavefront-uniform
+    const uint32_t rows = min(kRosenWaveRows, tl.count - r0);
+    const uint32_t p0 = tl.first + r0;
+    const double f = rosenbrock_wave_rows(a.X + di.xoff + static_cast<uint64_t>(p0) * di.nx, rows, di.nx,
+                                          lds + wave * kRosenWaveRows * kRosenWaveStride, lane);
+    for (uint32_t r = 0; r < rows; ++r) step_remember(di, a, p0 + r, __shfl(f, static_cast<int>(r), 64), lane, 64u);
+}
+
+// An individual with an element beyond the streaming cosine's range, again
+// with the reference's expression (rare; out of line so that its registers do
+// not weigh on the streaming code).
+template <int KIND>
+__device__ __noinline__ double step_sep_cold(const double *__restrict__ x, uint32_t nx)
+{
+    return (nx % 2u == 0u) ? separable_eval_exact<KIND, 1, true>(x, nx, 0)
+                           : separable_eval_exact<KIND, 1, false>(x, nx, 0);
+}
+
+template <int KIND, int L>
+__device__ __forceinline__ void step_eval_sep_l(const island_desc &di, const tile &tl, const step_args &a)
+{
+    const uint32_t g = threadIdx.x / L, r = threadIdx.x % L;
+    const uint32_t gg = g < tl.count ? g : tl.count - 1u;
+    const uint32_t p = tl.first + gg;
+    const double *x = a.X + di.xoff + static_cast<uint64_t>(p) * di.nx;
+    double f = (di.nx % 2u == 0u) ? separable_eval<KIND, L, true>(x, di.nx, static_cast<int>(r))
+                                  : separable_eval<KIND, L, false>(x, di.nx, static_cast<int>(r));
+    if (is_recompute_mark(f)) {
+        double e = 0.;
+        if (r == 0u) e = step_sep_cold<KIND>(x, di.nx);
+        f = __shfl(e, 0, L);
+    }
+    if (g < tl.count) step_remember(di, a, p, f, r, L);
+}
+
+template <int KIND>
+__device__ __forceinline__ void step_eval_sep(const island_desc &di, const tile &tl, const step_args &a)
+{
+    switch (di.lanes) {
+        case 64:
+            step_eval_sep_l<KIND, 64>(di, tl, a);
+            return;
+        case 32:
+            step_eval_sep_l<KIND, 32>(di, tl, a);
+            return;
+        case 16:
+            step_eval_sep_l<KIND, 16>(di, tl, a);
+            return;
+        default:
+            step_eval_sep_l<KIND, 4>(di, tl, a);
+    }
+}
+
+template <int L, uint32_t APL>
+__device__ __forceinline__ void step_eval_lj_a(const island_desc &di, const tile &tl, const step_args &a, double *lds)
+{
+    const uint32_t atoms = di.dim;
+    lj_stage(a.X + di.xoff + static_cast<uint64_t>(tl.first) * di.nx, tl.count, atoms, false, lds);
+    __syncthreads();
+    const uint32_t g = threadIdx.x / L, r = threadIdx.x % L;
+    if (g * L >= kBlock || g >= tl.count) return; // the item's groups
+    const double f = lj_energy<L, APL>(lds + g * lj_stride(atoms, false), atoms, static_cast<int>(r));
+    step_remember(di, a, tl.first + g, f, r, L);
+}
+
+template <int L>
+__device__ __forceinline__ void step_eval_lj_l(const island_desc &di, const tile &tl, const step_args &a, double *lds)
+{
+    switch (lj_apl(di.dim, L)) {
+        case 1:
+            step_eval_lj_a<L, 1>(di, tl, a, lds);
+            return;
+        case 2:
+            step_eval_lj_a<L, 2>(di, tl, a, lds);
+            return;
+        case 3:
+            step_eval_lj_a<L, 3>(di, tl, a, lds);
+            return;
+        case 4:
+            step_eval_lj_a<L, 4>(di, tl, a, lds);
+            return;
+        default:
+            break;
+    }
+    if constexpr (L >= 32) {
+        switch (lj_apl(di.dim, L)) {
+            case 5:
+                step_eval_lj_a<L, 5>(di, tl, a, lds);
+                return;
+            case 6:
+                step_eval_lj_a<L, 6>(di, tl, a, lds);
+                return;
+            case 7:
+                step_eval_lj_a<L, 7>(di, tl, a, lds);
+                return;
+            case 8:
+                step_eval_lj_a<L, 8>(di, tl, a, lds);
+                return;
+            default:
+                break;
+        }
+    }
+    step_eval_lj_a<L, 0>(di, tl, a, lds);
+}
+
+__device__ __forceinline__ void step_eval_lj(const island_desc &di, const tile &tl, const step_args &a, double *lds)
+{
+    switch (di.lanes) {
+        case 32:
+            step_eval_lj_l<32>(di, tl, a, lds);
+            return;
+        case 16:
+            step_eval_lj_l<16>(di, tl, a, lds);
+            return;
+        case 8:
+            step_eval_lj_l<8>(di, tl, a, lds);
+            return;
+        case 4:
+            step_eval_lj_l<4>(di, tl, a, lds);
+            return;
+        case 2:
+            step_eval_lj_l<2>(di, tl, a, lds);
+            return;
+        default:
+            step_eval_lj_l<1>(di, tl, a, lds);
+            return;
+    }
+}
+
+template <uint32_t CLASS>
+__device__ __forceinline__ void step_item(const island_desc &di, const tile &tl, const step_args &a, double *lds,
+                                          step_rows &rw)
+{
+    step_move(di, tl, a, rw);
+    __syncthreads(); // the item's new rows are visible to the whole workgroup
+    const uint32_t cls = CLASS == kStepAll ? step_class(di.kind) : CLASS;
+    if (cls == kStepRosen) {
+        if constexpr (CLASS == kStepRosen || CLASS == kStepAll) step_eval_rosen(di, tl, a, lds);
+    } else if (cls == kStepLj) {
+        if constexpr (CLASS == kStepLj || CLASS == kStepAll) step_eval_lj(di, tl, a, lds);
+    } else {
+        if constexpr (CLASS == kStepSep || CLASS == kStepAll) {
+            switch (di.kind) {
+                case RASTRIGIN:
+                    step_eval_sep<RASTRIGIN>(di, tl, a);
+                    break;
+                case ACKLEY:
+                    step_eval_sep<ACKLEY>(di, tl, a);
+                    break;
+                case SCHWEFEL:
+                    step_eval_sep<SCHWEFEL>(di, tl, a);
+                    break;
+                default:
+                    step_eval_sep<GRIEWANK>(di, tl, a);
+                    break;
+            }
+        }
+    }
+}
+
+// One generation of the items order[0 .. n_items) (or base + i): a persistent
+// grid whose workgroups take items from the queue until it is empty (every
+// wavefront reads the same index, so all leave the loop together).
+// Wavefronts per SIMD each class's kernel is built for (its register budget):
+// the move phase streams from HBM and wants occupancy; the Lennard-Jones
+// evaluation wants registers for its atom slots.
+#ifndef PGH_STEP_BOUNDED
+#define PGH_STEP_BOUNDED 1
+#endif
+constexpr int step_min_waves(uint32_t cls)
+{
+    return !PGH_STEP_BOUNDED ? 1 : (cls == kStepRosen || cls == kStepSep ? 8 : (cls == kStepLj ? 4 : 3));
+}
+
+template <uint32_t CLASS>
+__global__ __launch_bounds__(kBlock, step_min_waves(CLASS)) void k_step(step_args a)
 {
     extern __shared__ double lds[];
-    const tile tl = tiles[t0 + blockIdx.x];
-    const island_desc di = d[tl.island];
-    if (c.gen >= di.gens) return;
-    eval_tile(di, tl, c, lds);
+    __shared__ uint32_t s_next;
+    __shared__ step_rows rw;
+    for (;;) {
+        if (threadIdx.x == 0) s_next = atomicAdd(a.queue, 1u);
+        __syncthreads();
+        const uint32_t i = s_next;
+        if (i >= a.n_items) return;
+        const uint32_t it = a.order ? a.order[i] : a.base + i;
+        const tile tl = a.items[it];
+        const island_desc di = a.d[tl.island];
+        if (a.gen < di.gens) step_item<CLASS>(di, tl, a, lds, rw);
+        __syncthreads(); // LDS and s_next are reused by the next item
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_zero_u32(uint32_t *__restrict__ p, uint64_t n)
+{
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// *differs = 1 when the n words of x and y are not all equal (the tuner's
+// check that two mappings leave identical states).
+__global__ __launch_bounds__(kBlock) void k_differs(const uint32_t *__restrict__ x, const uint32_t *__restrict__ y,
+                                                    uint64_t n, uint32_t *__restrict__ differs)
+{
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n && x[i] != y[i]) *differs = 1u;
+}
+
+// End of an evolve call: island k's memory back into lbX0 and lbfit (A), the
+// buffers the rest of the library reads.  After the island's gens
+// generations its memory fitness is in A (gens even) or B, and row p's memory
+// in the buffer named by its final selector (in selA or selB likewise).  The
+// selectors are only read here (k_vinit clears selA at the next call's start).
+__global__ __launch_bounds__(kBlock) void k_settle(const island_desc *__restrict__ d, uint32_t i0, uint32_t i1,
+                                                   uint64_t e0, uint64_t total, double *__restrict__ lbX0,
+                                                   const double *__restrict__ lbX1, double *__restrict__ lbfA,
+                                                   const double *__restrict__ lbfB, const uint32_t *__restrict__ selA,
+                                                   const uint32_t *__restrict__ selB)
+{
+    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= total) return;
+    const uint64_t e = e0 + t;
+    const uint32_t k = find_island(d, i0, i1, e);
+    const island_desc di = d[k];
+    const uint64_t m = e - di.eoff;
+    const uint32_t p = static_cast<uint32_t>(m / di.nx);
+    const bool odd = (di.gens & 1u) != 0u;
+    const uint64_t fi = di.foff + p;
+    if ((odd ? selB[fi] : selA[fi]) != 0u) lbX0[di.xoff + m] = lbX1[di.xoff + m];
+    if (odd && m == static_cast<uint64_t>(p) * di.nx) lbfA[fi] = lbfB[fi];
 }
 
 // Generations g0 .. g1 - 1 (within one batch of random words) of one island
@@ -483,22 +884,10 @@ __global__ __launch_bounds__(kBlock) void k_gens_one_tile(const island_desc *__r
     const island_desc di = d[island];
     const uint64_t elems = static_cast<uint64_t>(di.pop) * di.nx;
     if (blockIdx.x == 1u) {
+        // batch g1 / gps of a one-segment generator (k_rng_gen's work; the
+        // velocity draws shift the whole call's words alike)
         __shared__ uint32_t ring[kMtRing];
-        const uint32_t first = g1;
-        if (first >= di.gens) return;
-        const uint32_t last = min(first + di.gps, di.gens);
-        const uint64_t W = 4u * elems;
-        mt_state *mine = &segst[di.seg0];
-        uint64_t K, C;
-        load_state(mine, ring, K, C);
-        const uint64_t n = (last - first) * W;
-        mt_run(ring, K, C, n, words_next + di.woff);
-        C += n;
-        if (last == di.gens) {
-            store_state(&st[island], ring, K, C);
-        } else if (first + kGenBatch < di.gens) {
-            store_state(mine, ring, K, C);
-        }
+        rng_gen_segment(di, 0u, g1 / di.gps, 0u, &st[island], &segst[di.seg0], words_next + di.woff, ring);
         return;
     }
     const uint32_t end = min(g1, di.gens);
@@ -588,6 +977,21 @@ struct pgh_archi {
     // and go up in one copy
     dev_buf arena, trace_buf;
     arena_view desc, tiles, segs, jump_table, states, X, V, lbX, fit, lbfit, words, seg_states;
+    // k_step: the second memory buffers, the selectors and one queue counter
+    // per launch (in the arena); its items (per island, contiguous) and their
+    // dispatch orders in a table of their own, rebuilt with the mapping
+    arena_view lbX1, lbfitB, selA, selB, queues;
+    uint64_t queue_stride = 0; // counters per generation: one per island, then one per step class
+    dev_buf step_tab;
+    const tile *d_items = nullptr;
+    const uint32_t *d_order_all = nullptr, *d_order_cls = nullptr;
+    std::vector<tile> h_items;
+    std::vector<uint32_t> item_begin;       // per island, into h_items (+ end sentinel)
+    uint32_t class_begin[4] = {0, 0, 0, 0}; // per step class, into the class order (+ end)
+    size_t lj_lds = 0;                      // LDS of the Lennard-Jones items
+    uint32_t n_cus = 0;
+    pgh_archi_mapping map{PGH_ARCHI_STEP_SPLIT, 0u, 0u, 0u};
+    uint32_t step_per_cu[4] = {1u, 1u, 1u, 1u}; // k_step workgroups per CU, per class
     uint32_t n_segs = 0;
     jump_tab jt{nullptr, nullptr};
     hipStream_t own = nullptr;
@@ -631,6 +1035,139 @@ uint32_t blocks_for(uint64_t count)
     return static_cast<uint32_t>((count + kBlock - 1u) / kBlock);
 }
 
+// The items of every island for the mapping a->map: 32 rows for rosenbrock
+// (8 per wavefront), one L-lane group per row for the separable problems
+// (256 / L rows), and for lennard_jones as many individuals as fit the
+// mapping's LDS (at most one group of L lanes each per workgroup, at least
+// one individual).  Batched launches hand them out by decreasing estimated
+// cost, over all items (one launch) or within each problem class (a launch
+// per class).  The cost only orders the items; no result depends on it.
+void step_occupancy(pgh_archi *a);
+
+pgh_status build_items(pgh_archi *a)
+{
+    const uint64_t budget = a->map.lj_lds ? a->map.lj_lds : 65536u;
+    a->h_items.clear();
+    a->item_begin.assign(1u, 0u);
+    a->lj_lds = 0u;
+    std::vector<double> cost;
+    std::vector<uint32_t> cls_of;
+    for (size_t k = 0; k < a->h_desc.size(); ++k) {
+        const island_desc &di = a->h_desc[k];
+        const uint32_t cls = step_class(di.kind);
+        uint32_t per = 0u;
+        double unit = 0.; // per row, in units of one element's move
+        if (cls == kStepRosen) {
+            per = kStepRosenRows;
+            unit = 1.2 * di.nx;
+        } else if (cls == kStepLj) {
+            const uint64_t bytes = static_cast<uint64_t>(lj_stride(di.dim, false)) * sizeof(double);
+            per = static_cast<uint32_t>(std::min<uint64_t>(kBlock / di.lanes, budget / bytes));
+            per = std::max(per, 1u);
+            a->lj_lds = std::max<size_t>(a->lj_lds, per * bytes);
+            unit = di.nx + 0.25 * di.dim * (di.dim - 1u);
+        } else {
+            per = kBlock / di.lanes;
+            unit = 1.3 * di.nx;
+        }
+        for (uint32_t f = 0; f < di.pop; f += per) {
+            const uint32_t c = std::min(per, di.pop - f);
+            a->h_items.push_back(tile{static_cast<uint32_t>(k), f, c, 0u});
+            cost.push_back(unit * c);
+            cls_of.push_back(cls);
+        }
+        a->item_begin.push_back(static_cast<uint32_t>(a->h_items.size()));
+    }
+    const size_t n = a->h_items.size();
+    std::vector<uint32_t> all(n), by_cls;
+    for (size_t i = 0; i < n; ++i) all[i] = static_cast<uint32_t>(i);
+    std::stable_sort(all.begin(), all.end(), [&](uint32_t x, uint32_t y) { return cost[x] > cost[y]; });
+    for (uint32_t c = 0; c < 3u; ++c) {
+        a->class_begin[c] = static_cast<uint32_t>(by_cls.size());
+        for (uint32_t i : all) {
+            if (cls_of[i] == c) by_cls.push_back(i);
+        }
+    }
+    a->class_begin[3] = static_cast<uint32_t>(by_cls.size());
+    // the table: items, then the two orders
+    const size_t ib = n * sizeof(tile), ob = n * sizeof(uint32_t);
+    std::vector<uint8_t> tab(ib + 2u * ob + 16u, 0u);
+    std::memcpy(tab.data(), a->h_items.data(), ib);
+    std::memcpy(tab.data() + ib, all.data(), ob);
+    std::memcpy(tab.data() + ib + ob, by_cls.data(), ob);
+    PGH_TRY(a->step_tab.reserve(tab.size()));
+    PGH_HIP(hipMemcpy(a->step_tab.ptr, tab.data(), tab.size(), hipMemcpyHostToDevice));
+    auto *base = static_cast<uint8_t *>(a->step_tab.ptr);
+    a->d_items = reinterpret_cast<const tile *>(base);
+    a->d_order_all = reinterpret_cast<const uint32_t *>(base + ib);
+    a->d_order_cls = reinterpret_cast<const uint32_t *>(base + ib + ob);
+    step_occupancy(a);
+    return PGH_OK;
+}
+
+size_t step_lds(const pgh_archi *a, uint32_t cls);
+
+// k_step workgroups per CU for each class at the current items' LDS: as many
+// as can be resident, or the mapping's count.
+void step_occupancy(pgh_archi *a)
+{
+    const void *fn[4] = {reinterpret_cast<const void *>(&k_step<kStepRosen>),
+                         reinterpret_cast<const void *>(&k_step<kStepSep>),
+                         reinterpret_cast<const void *>(&k_step<kStepLj>),
+                         reinterpret_cast<const void *>(&k_step<kStepAll>)};
+    for (uint32_t c = 0; c < 4u; ++c) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn[c], static_cast<int>(kBlock), step_lds(a, c)) !=
+                hipSuccess ||
+            per_cu < 1) {
+            (void)hipGetLastError();
+            per_cu = 1;
+        }
+        if (a->map.blocks_per_cu) per_cu = std::min<int>(per_cu, static_cast<int>(a->map.blocks_per_cu));
+        a->step_per_cu[c] = static_cast<uint32_t>(per_cu);
+    }
+}
+
+size_t step_lds(const pgh_archi *a, uint32_t cls)
+{
+    switch (cls) {
+        case kStepRosen:
+            return kStepRosenLds;
+        case kStepSep:
+            return 0u;
+        case kStepLj:
+            return a->lj_lds;
+        default:
+            return std::max(kStepRosenLds, a->lj_lds);
+    }
+}
+
+// Launches k_step<cls> over n_items items as a persistent grid: as many
+// workgroups as can be resident (or the mapping's count per CU), at most one
+// per item.
+pgh_status launch_step(const pgh_archi *a, uint32_t cls, const step_args &args, hipStream_t s)
+{
+    const size_t lds = step_lds(a, cls);
+    const uint32_t grid = std::min<uint32_t>(args.n_items, a->step_per_cu[cls] * a->n_cus);
+    if (grid == 0u) return PGH_OK;
+    switch (cls) {
+        case kStepRosen:
+            k_step<kStepRosen><<<grid, kBlock, lds, s>>>(args);
+            break;
+        case kStepSep:
+            k_step<kStepSep><<<grid, kBlock, lds, s>>>(args);
+            break;
+        case kStepLj:
+            k_step<kStepLj><<<grid, kBlock, lds, s>>>(args);
+            break;
+        default:
+            k_step<kStepAll><<<grid, kBlock, lds, s>>>(args);
+            break;
+    }
+    PGH_HIP(hipGetLastError());
+    return PGH_OK;
+}
+
 // Launches the evolve call for islands [i0, i1) on stream s.  v_valid: the
 // islands have velocities from an earlier call (kept by those with memory).
 pgh_status launch_range(pgh_archi *a, uint32_t i0, uint32_t i1, bool v_valid, bool from_lb, hipStream_t s)
@@ -654,10 +1191,12 @@ pgh_status launch_range(pgh_archi *a, uint32_t i0, uint32_t i1, bool v_valid, bo
     const auto *sd = static_cast<const seg_desc *>(a->segs.ptr);
     auto *segst = static_cast<mt_state *>(a->seg_states.ptr);
     const jump_tab jt = a->jt;
-    k_rng<<<ng, kMtJumpBlock, 0, s>>>(d, sd, st, segst, jt, w, sg0, 0u, 0u, v_valid ? 1u : 0u);
+    const uint32_t vv = v_valid ? 1u : 0u;
+    k_rng<<<ng, kMtJumpBlock, 0, s>>>(d, sd, st, segst, jt, w, sg0, 0u, 0u, vv);
     PGH_HIP(hipGetLastError());
-    k_vinit<<<blocks_for(total), kBlock, 0, s>>>(d, i0, i1, e0, total, w, X, V, lbX, fit, lbfit, v_valid ? 1u : 0u,
-                                                 from_lb ? 1u : 0u);
+    auto *selA = static_cast<uint32_t *>(a->selA.ptr);
+    k_vinit<<<blocks_for(total), kBlock, 0, s>>>(d, i0, i1, e0, total, w, X, V, lbX, fit, lbfit, selA,
+                                                 v_valid ? 1u : 0u, from_lb ? 1u : 0u);
     PGH_HIP(hipGetLastError());
     // a single island of one tile, evolved by the one-shot entry point: a
     // batch of generations per launch (k_gens_one_tile)
@@ -672,13 +1211,22 @@ pgh_status launch_range(pgh_archi *a, uint32_t i0, uint32_t i1, bool v_valid, bo
         if (stage_off + need <= 65536u) stage_bytes = need;
     }
     const auto *tl = static_cast<const tile *>(a->tiles.ptr);
+    auto *lbX1 = static_cast<double *>(a->lbX1.ptr);
+    auto *lbfitB = static_cast<double *>(a->lbfitB.ptr);
+    auto *selB = static_cast<uint32_t *>(a->selB.ptr);
+    auto *queues = static_cast<uint32_t *>(a->queues.ptr);
+    const uint32_t n_isl = static_cast<uint32_t>(a->h_desc.size());
     // ... and the next batch's words handed out beside it, into the other
     // half of the word buffer, when the generator is one segment
     const bool overlap = fused && a->words_half != 0u && a->h_desc[i0].segs == 1u;
     for (uint32_t g = 0; g < gens; ++g) {
         if (g % kGenBatch == 0u) {
             const uint32_t b = g / kGenBatch;
-            if (!overlap || g == 0u) k_rng<<<ng, kMtJumpBlock, 0, s>>>(d, sd, st, segst, jt, w, sg0, 1u, g, 0u);
+            if (!overlap || g == 0u) {
+                if (b > 0u) k_rng<<<ng, kMtJumpBlock, 0, s>>>(d, sd, st, segst, jt, w, sg0, 2u, b - 1u, vv);
+                k_rng_gen<<<ng, kMtGenBlock, 0, s>>>(d, sd, st, segst, w, sg0, b, vv);
+                PGH_HIP(hipGetLastError());
+            }
             if (fused) {
                 const uint32_t *wc = w + (overlap ? (b % 2u) * a->words_half : 0u);
                 uint32_t *wn = w + (overlap ? ((b + 1u) % 2u) * a->words_half : 0u);
@@ -694,8 +1242,51 @@ pgh_status launch_range(pgh_archi *a, uint32_t i0, uint32_t i1, bool v_valid, bo
             }
         }
         if (fused) continue;
-        k_update<<<blocks_for(total), kBlock, 0, s>>>(d, i0, i1, e0, total, w, X, V, lbX, lbfit, g);
-        k_eval<<<nt, kBlock, a->lds_bytes, s>>>(d, static_cast<const tile *>(a->tiles.ptr), t0, make_ctx(a, g));
+        if (g == 0u) {
+            // this call's queue counters
+            const uint64_t nq = static_cast<uint64_t>(a->max_gens) * a->queue_stride;
+            k_zero_u32<<<blocks_for(nq), kBlock, 0, s>>>(queues, nq);
+            PGH_HIP(hipGetLastError());
+        }
+        step_args args{};
+        args.d = d;
+        args.items = a->d_items;
+        args.words = w;
+        args.X = X;
+        args.V = V;
+        args.lbX0 = lbX;
+        args.lbX1 = lbX1;
+        args.fit = fit;
+        args.trace = a->trace ? static_cast<double *>(a->trace_buf.ptr) : nullptr;
+        // the memory fitness and the selectors alternate between their buffers
+        args.lbf_cur = (g % 2u) ? lbfitB : lbfit;
+        args.lbf_nxt = (g % 2u) ? lbfit : lbfitB;
+        args.sel_cur = (g % 2u) ? selB : selA;
+        args.sel_nxt = (g % 2u) ? selA : selB;
+        args.gen = g;
+        uint32_t *qg = queues + static_cast<uint64_t>(g) * a->queue_stride;
+        if (i1 == i0 + 1u) {
+            // one island: its own items, in order
+            args.base = a->item_begin[i0];
+            args.n_items = a->item_begin[i1] - a->item_begin[i0];
+            args.queue = qg + i0;
+            PGH_TRY(launch_step(a, step_class(a->h_desc[i0].kind), args, s));
+        } else if (a->map.form == PGH_ARCHI_STEP_ONE) {
+            args.order = a->d_order_all + a->item_begin[i0];
+            args.n_items = a->item_begin[i1] - a->item_begin[i0];
+            args.queue = qg + n_isl + kStepAll;
+            PGH_TRY(launch_step(a, kStepAll, args, s));
+        } else {
+            for (uint32_t c = 0; c < 3u; ++c) {
+                args.order = a->d_order_cls + a->class_begin[c];
+                args.n_items = a->class_begin[c + 1u] - a->class_begin[c];
+                args.queue = qg + n_isl + c;
+                PGH_TRY(launch_step(a, c, args, s));
+            }
+        }
+    }
+    if (!fused) {
+        k_settle<<<blocks_for(total), kBlock, 0, s>>>(d, i0, i1, e0, total, lbX, lbX1, lbfit, lbfitB, selA, selB);
         PGH_HIP(hipGetLastError());
     }
     return PGH_OK;
@@ -713,6 +1304,48 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
     uint64_t xoff = 0, foff = 0, woff = 0, toff = 0, eoff = 0;
     std::vector<seg_desc> seg_list;
     std::vector<uint64_t> dists;
+    {
+        int dev = 0, cus = 0;
+        PGH_HIP(hipGetDevice(&dev));
+        PGH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        a->n_cus = static_cast<uint32_t>(std::max(cus, 1));
+    }
+    // The generators' share per segment: about two shares per CU over all
+    // islands' batches (a power of two in [kSegMin, kSegMax]), so that the
+    // batch's words are made by workgroups spread over the device while the
+    // jumps that position the segments (each costs about as much as making
+    // a few hundred thousand words) stay few.  The jumps
+    // z^(seg_words 2^b) that move segment s to its first share are the same
+    // for every island and come first in the jump table; other distances are
+    // entered once each.
+    uint64_t seg_words = kSegMin, max_segs = 1u;
+    {
+        uint64_t total = 0;
+        std::vector<uint64_t> bw(n_islands, 0u);
+        for (uint64_t k = 0; k < n_islands; ++k) {
+            prob p;
+            PGH_TRY(resolve(&islands[k].udp, p));
+            const uint64_t gps = std::min<uint64_t>(kGenBatch, std::max<uint64_t>(islands[k].algo.gen, 1u));
+            bw[k] = gps * 4u * islands[k].pop * p.nx;
+            total += bw[k];
+        }
+        const uint64_t want = total / (2u * a->n_cus);
+        while (seg_words * 2u <= want && seg_words < kSegMax) seg_words *= 2u;
+        // a lone island of a small batch stays one generator (the one-shot
+        // evolve hands out its next batch beside the current one)
+        if (n_islands == 1u && bw[0] <= kSoloWords) seg_words = std::max(seg_words, bw[0]);
+        for (uint64_t k = 0; k < n_islands; ++k) {
+            if (bw[k] > seg_words) max_segs = std::max<uint64_t>(max_segs, (bw[k] + seg_words - 1u) / seg_words);
+        }
+    }
+    for (uint64_t b = 0; (max_segs - 1u) >> b; ++b) dists.push_back(seg_words << b);
+    const auto dist_index = [&dists](uint64_t J) {
+        for (size_t i = 0; i < dists.size(); ++i) {
+            if (dists[i] == J) return static_cast<uint32_t>(i);
+        }
+        dists.push_back(J);
+        return static_cast<uint32_t>(dists.size() - 1u);
+    };
     a->tile_begin.push_back(0);
     for (uint64_t k = 0; k < n_islands; ++k) {
         const pgh_island_spec &sp = islands[k];
@@ -720,6 +1353,11 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
         PGH_TRY(resolve(&sp.udp, p));
         if (p.nf != 1u) return fail(PGH_UNSUPPORTED, "pso_gen needs a single-objective problem");
         if (p.shift) return fail(PGH_UNSUPPORTED, "the archipelago does not take translated problems");
+        if (p.kind != PGH_ROSENBROCK && p.kind != PGH_RASTRIGIN && p.kind != PGH_ACKLEY && p.kind != PGH_SCHWEFEL &&
+            p.kind != PGH_GRIEWANK && p.kind != PGH_LENNARD_JONES) {
+            return fail(PGH_UNSUPPORTED, "the device archipelago evolves rosenbrock, rastrigin, ackley, schwefel, "
+                                         "griewank and lennard_jones islands");
+        }
         probs.push_back(p);
         const pgh_pso_params &q = sp.algo;
         // pso_gen.cpp:83-112 parameter checks
@@ -799,27 +1437,21 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
         } else {
             di.lanes = 1u;
         }
-        // the generator's segments (k_rng): as few as keep each segment's
-        // share of a batch within kSegWords words (a power of two, at most
-        // one segment per generation of the batch)
+        // the generator's segments (k_rng): batches of gps generations split in
+        // shares of seg_words words (one share when the batch is smaller)
         {
             const uint64_t W = 4u * elems;
-            const uint32_t batch = std::min<uint32_t>(kGenBatch, std::max<uint32_t>(q.gen, 1u));
+            di.gps = std::min<uint32_t>(kGenBatch, std::max<uint32_t>(q.gen, 1u));
+            di.bwords = static_cast<uint64_t>(di.gps) * W;
             di.seg0 = static_cast<uint32_t>(seg_list.size());
-            di.segs = 1u;
-            while (di.segs < batch && ((batch + di.segs - 1u) / di.segs) * W > kSegWords) di.segs *= 2u;
-            // gps * segs = kGenBatch when a call has a batch or more of
-            // generations; otherwise the single batch's generations
-            di.gps = (batch + di.segs - 1u) / di.segs;
-            di.j_n0 = di.j_tree = di.j_main = 0u;
+            di.swords = std::min(di.bwords, seg_words);
+            di.segs = static_cast<uint32_t>((di.bwords + di.swords - 1u) / di.swords);
+            di.j_n0 = di.j_tree = di.j_main = di.j_last = 0u;
             if (di.segs > 1u) {
-                const uint64_t share = static_cast<uint64_t>(di.gps) * W;
-                di.j_n0 = static_cast<uint32_t>(dists.size());
-                dists.push_back(2u * elems);
-                di.j_tree = static_cast<uint32_t>(dists.size());
-                for (uint32_t b = 0; (di.segs - 1u) >> b; ++b) dists.push_back(share << b);
-                di.j_main = static_cast<uint32_t>(dists.size());
-                dists.push_back((di.segs - 1u) * share);
+                di.j_n0 = dist_index(2u * elems);
+                di.j_tree = 0u; // the shared z^(seg_words 2^b) block
+                di.j_main = dist_index(di.bwords - di.swords);
+                di.j_last = dist_index(static_cast<uint64_t>(di.segs - 1u) * di.swords);
             }
             for (uint32_t sgi = 0; sgi < di.segs; ++sgi) seg_list.push_back(seg_desc{static_cast<uint32_t>(k), sgi});
         }
@@ -860,12 +1492,20 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
     place(a->lbfit, foff * 8u);
     place(a->words, std::max<uint64_t>(woff, 1u) * 4u);
     place(a->seg_states, seg_list.size() * sizeof(mt_state));
+    place(a->lbX1, xoff * 8u);
+    place(a->lbfitB, foff * 8u);
+    place(a->selA, foff * 4u);
+    place(a->selB, foff * 4u);
+    a->queue_stride = n_islands + 4u;
+    place(a->queues, std::max<uint64_t>(a->max_gens, 1u) * a->queue_stride * 4u);
     PGH_TRY(a->arena.reserve(arena_bytes));
     auto *base = static_cast<uint8_t *>(a->arena.ptr);
     for (arena_view *v : {&a->desc, &a->tiles, &a->segs, &a->jump_table, &a->states, &a->X, &a->V, &a->lbX, &a->fit,
-                          &a->lbfit, &a->words, &a->seg_states}) {
+                          &a->lbfit, &a->words, &a->seg_states, &a->lbX1, &a->lbfitB, &a->selA, &a->selB,
+                          &a->queues}) {
         v->ptr = base + v->off;
     }
+    PGH_TRY(build_items(a.get()));
     {
         std::vector<uint8_t> head(head_bytes, 0u);
         auto put = [&](const arena_view &v, const void *src, uint64_t bytes) {
@@ -1007,6 +1647,138 @@ try {
     a->v_valid = true;
     a->evolved = true;
     a->last = s;
+    return PGH_OK;
+} catch (...) {
+    return api_exception();
+}
+
+pgh_status pgh_archi_set_mapping(pgh_archi *a, const pgh_archi_mapping *m)
+try {
+    if (!a || !m) return fail(PGH_INVALID_ARGUMENT, "null argument");
+    if (m->form != PGH_ARCHI_STEP_ONE && m->form != PGH_ARCHI_STEP_SPLIT) {
+        return fail(PGH_INVALID_ARGUMENT, "unknown generation launch form " + std::to_string(m->form));
+    }
+    if (m->lj_lds > 65536u) return fail(PGH_INVALID_ARGUMENT, "lj_lds is at most 65536 bytes");
+    if (m->blocks_per_cu > 64u) return fail(PGH_INVALID_ARGUMENT, "blocks_per_cu is at most 64");
+    if (a->last) PGH_HIP(hipStreamSynchronize(a->last));
+    a->map = *m;
+    a->map.pad = 0u;
+    // the captured graph bakes the items and launch shapes in
+    if (a->exec) (void)hipGraphExecDestroy(a->exec);
+    if (a->graph) (void)hipGraphDestroy(a->graph);
+    a->exec = nullptr;
+    a->graph = nullptr;
+    return build_items(a);
+} catch (...) {
+    return api_exception();
+}
+
+pgh_status pgh_archi_get_mapping(pgh_archi *a, pgh_archi_mapping *m)
+try {
+    if (!a || !m) return fail(PGH_INVALID_ARGUMENT, "null argument");
+    *m = a->map;
+    return PGH_OK;
+} catch (...) {
+    return api_exception();
+}
+
+pgh_status pgh_archi_tune(pgh_archi *a, uint32_t reps, pgh_archi_mapping *chosen)
+try {
+    if (!a) return fail(PGH_INVALID_ARGUMENT, "null archipelago");
+    if (a->migr) return fail(PGH_UNSUPPORTED, "pgh_archi_tune: turn migration off first");
+    reps = std::max(reps, 1u);
+    if (a->last) PGH_HIP(hipStreamSynchronize(a->last));
+    // the state an evolve call reads and writes: the generators, X, V, lbX,
+    // fit and lbfit (contiguous in the arena), and the host flags
+    auto *arena = static_cast<uint8_t *>(a->arena.ptr);
+    const uint64_t off = a->states.off, bytes = a->words.off - a->states.off;
+    const bool v_valid = a->v_valid, evolved = a->evolved, trace = a->trace;
+    const pgh_archi_mapping before = a->map;
+    dev_buf snap, ref, flag;
+    PGH_TRY(snap.reserve(bytes));
+    PGH_TRY(ref.reserve(bytes));
+    PGH_TRY(flag.reserve(4u));
+    PGH_HIP(hipMemcpyAsync(snap.ptr, arena + off, bytes, hipMemcpyDeviceToDevice, a->own));
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    PGH_HIP(hipEventCreate(&t0));
+    PGH_HIP(hipEventCreate(&t1));
+    const auto release = [&]() {
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+    };
+    const auto restore = [&]() -> pgh_status {
+        PGH_HIP(hipMemcpyAsync(arena + off, snap.ptr, bytes, hipMemcpyDeviceToDevice, a->own));
+        a->v_valid = v_valid;
+        a->evolved = evolved;
+        return PGH_OK;
+    };
+    a->trace = false;
+    std::vector<pgh_archi_mapping> cand;
+    for (uint32_t form : {static_cast<uint32_t>(PGH_ARCHI_STEP_SPLIT), static_cast<uint32_t>(PGH_ARCHI_STEP_ONE)}) {
+        for (uint32_t lds : {65536u, 32768u, 16384u}) {
+            for (uint32_t bpc : {0u, 2u}) cand.push_back(pgh_archi_mapping{form, lds, bpc, 0u});
+        }
+    }
+    pgh_status st = PGH_OK;
+    float best_ms = 0.f;
+    pgh_archi_mapping best = cand[0];
+    for (size_t c = 0; c < cand.size() && st == PGH_OK; ++c) {
+        st = pgh_archi_set_mapping(a, &cand[c]);
+        float ms = 0.f;
+        for (uint32_t r = 0; r <= reps && st == PGH_OK; ++r) {
+            // the first run captures the graph; the others are timed
+            st = restore();
+            if (st != PGH_OK) break;
+            if (r > 0u && (st = hip_status(hipEventRecord(t0, a->own), "hipEventRecord")) != PGH_OK) break;
+            st = pgh_archi_evolve(a, PGH_ARCHI_BATCHED, nullptr);
+            if (st != PGH_OK) break;
+            if (r > 0u) {
+                if ((st = hip_status(hipEventRecord(t1, a->own), "hipEventRecord")) != PGH_OK) break;
+                if ((st = hip_status(hipEventSynchronize(t1), "hipEventSynchronize")) != PGH_OK) break;
+                float e = 0.f;
+                if ((st = hip_status(hipEventElapsedTime(&e, t0, t1), "hipEventElapsedTime")) != PGH_OK) break;
+                ms = r == 1u ? e : std::min(ms, e);
+            }
+        }
+        if (st != PGH_OK) break;
+        if (c == 0u) {
+            st = hip_status(hipMemcpyAsync(ref.ptr, arena + off, bytes, hipMemcpyDeviceToDevice, a->own), "copy");
+        } else {
+            uint32_t differs = 0u;
+            k_zero_u32<<<1, kBlock, 0, a->own>>>(static_cast<uint32_t *>(flag.ptr), 1u);
+            k_differs<<<blocks_for(bytes / 4u), kBlock, 0, a->own>>>(static_cast<const uint32_t *>(ref.ptr),
+                                                                      reinterpret_cast<const uint32_t *>(arena + off),
+                                                                      bytes / 4u, static_cast<uint32_t *>(flag.ptr));
+            if ((st = hip_status(hipMemcpyAsync(&differs, flag.ptr, 4u, hipMemcpyDeviceToHost, a->own), "copy")) !=
+                    PGH_OK ||
+                (st = hip_status(hipStreamSynchronize(a->own), "hipStreamSynchronize")) != PGH_OK) {
+                break;
+            }
+            if (differs) {
+                st = fail(PGH_DEVICE_ERROR, "pgh_archi_tune: the mapping {form " + std::to_string(cand[c].form) +
+                                                ", lj_lds " + std::to_string(cand[c].lj_lds) + ", blocks_per_cu " +
+                                                std::to_string(cand[c].blocks_per_cu) +
+                                                "} left a state different from the first mapping's");
+                break;
+            }
+        }
+        if (c == 0u || ms < best_ms) {
+            best_ms = ms;
+            best = cand[c];
+        }
+    }
+    // back to the state before tuning, with the fastest mapping (or the old
+    // one after a failure)
+    const pgh_status rs = restore();
+    a->trace = trace;
+    const pgh_status mst = pgh_archi_set_mapping(a, st == PGH_OK ? &best : &before);
+    const hipError_t ss = hipStreamSynchronize(a->own);
+    release();
+    if (st != PGH_OK) return st;
+    PGH_TRY(rs);
+    PGH_TRY(mst);
+    PGH_HIP(ss);
+    if (chosen) *chosen = best;
     return PGH_OK;
 } catch (...) {
     return api_exception();

@@ -161,6 +161,73 @@ def test_batched_equals_per_island():
     b.close()
 
 
+@pytest.mark.parametrize("form,lj_lds,bpc", [
+    (P.ARCHI_STEP_SPLIT, 0, 0),
+    (P.ARCHI_STEP_ONE, 0, 0),
+    (P.ARCHI_STEP_ONE, 16384, 1),
+    (P.ARCHI_STEP_SPLIT, 4096, 2),   # one Lennard-Jones individual per item
+    (P.ARCHI_STEP_SPLIT, 32768, 1),  # a single workgroup per CU takes many items
+])
+def test_batched_mappings_equal_per_island(form, lj_lds, bpc):
+    """Every launch mapping of the batched generation (one launch or one per
+    problem class, Lennard-Jones items of any size, any persistent grid)
+    gives the per-island launches' populations and traces bit for bit, over
+    two evolve calls of an odd number of generations, with and without
+    pso_gen's memory flag."""
+    isl = uneven_islands(9, 3, gens=7)
+    for k in (1, 4):
+        isl[k]["algo"] = P.pso_params(gen=7, seed=42 + k, memory=True)
+    a = P.Archipelago(isl, trace=True)
+    b = P.Archipelago(isl, trace=True)
+    a.set_mapping(form, lj_lds, bpc)
+    assert a.mapping() == dict(form=form, lj_lds=lj_lds, blocks_per_cu=bpc)
+    for _ in range(2):
+        a.evolve(P.ARCHI_BATCHED)
+        b.evolve(P.ARCHI_PER_ISLAND)
+    for k in range(len(isl)):
+        xa, fa = a.get(k)
+        xb, fb = b.get(k)
+        assert np.array_equal(xa, xb) and np.array_equal(fa, fb), k
+        assert np.array_equal(a.trace(k), b.trace(k)), k
+    a.close()
+    b.close()
+
+
+def test_mapping_checks():
+    a = P.Archipelago(uneven_islands(3, 4, gens=2))
+    for bad in [dict(form=7), dict(lj_lds=65537), dict(blocks_per_cu=65)]:
+        with pytest.raises(P.PghError) as e:
+            a.set_mapping(**bad)
+        assert e.value.status == P.INVALID_ARGUMENT
+    a.close()
+
+
+def test_tune_keeps_the_state():
+    """The tuner evolves candidate mappings from the current state and puts
+    it back: the populations after tuning are those before, and the next
+    evolve call equals that of an archipelago that was never tuned."""
+    isl = uneven_islands(7, 6, gens=9)
+    a = P.Archipelago(isl)
+    b = P.Archipelago(isl)
+    a.evolve()
+    b.evolve()
+    before = [a.get(k) for k in range(len(isl))]
+    chosen = a.tune(reps=1)
+    assert chosen == a.mapping()
+    assert chosen["form"] in (P.ARCHI_STEP_ONE, P.ARCHI_STEP_SPLIT)
+    for k in range(len(isl)):
+        x, f = a.get(k)
+        assert np.array_equal(x, before[k][0]) and np.array_equal(f, before[k][1]), k
+    a.evolve()
+    b.evolve()
+    for k in range(len(isl)):
+        xa, fa = a.get(k)
+        xb, fb = b.get(k)
+        assert np.array_equal(xa, xb) and np.array_equal(fa, fb), k
+    a.close()
+    b.close()
+
+
 def test_islands_against_oracle():
     """Positions depend on fitness only through comparisons, so a few
     generations reproduce the oracle's positions exactly; fitness values agree

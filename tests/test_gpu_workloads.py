@@ -109,16 +109,20 @@ def test_lennard_jones_full_size():
 def test_uneven_archipelago_full_size():
     isl = archipelago_islands(gens=200)
     finals = {}
-    for mode, name in ((P.ARCHI_BATCHED, "batched"), (P.ARCHI_PER_ISLAND, "per-island")):
+    for mode, form, name in ((P.ARCHI_BATCHED, P.ARCHI_STEP_SPLIT, "batched, a launch per class"),
+                             (P.ARCHI_BATCHED, P.ARCHI_STEP_ONE, "batched, one launch"),
+                             (P.ARCHI_PER_ISLAND, P.ARCHI_STEP_SPLIT, "per-island")):
         say(f"archipelago 64 islands x 200 generations, {name}")
         a = P.Archipelago.seeded(isl)
+        a.set_mapping(form)
         a.evolve(mode)
         finals[name] = [a.get(k) for k in range(len(isl))]
         a.close()
     for k in range(len(isl)):
-        xb, fb = finals["batched"][k]
         xp, fp = finals["per-island"][k]
-        assert np.array_equal(xb, xp) and np.array_equal(fb, fp), k
+        for name in ("batched, a launch per class", "batched, one launch"):
+            xb, fb = finals[name][k]
+            assert np.array_equal(xb, xp) and np.array_equal(fb, fp), (name, k)
     del finals
     say("archipelago 64 islands x 5 generations against the oracle")
     isl5 = archipelago_islands(gens=5)
