@@ -47,11 +47,30 @@ def ensure_built():
 
 
 def cores():
+    """Host threads for the CPU baselines: the cores this process may run on,
+    at most 16 (a one-GPU box's share of its host)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
     return max(1, min(n, 16))
+
+
+def host_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = None
+    return dict(nproc=os.cpu_count(), affinity=affinity, model=model)
 
 
 class EventTimer:
@@ -111,9 +130,16 @@ def cpu_baseline(O, sep):
     xa = sep["xa"][:rows].cpu().numpy()
     _, tr = O.bfe_timed(O.RASTRIGIN, NX, xr, n, 5)
     _, ta = O.bfe_timed(O.ACKLEY, NX, xa, n, 5)
+    # the same on one thread, on a quarter of the rows
+    r1 = rows // 4
+    _, tr1 = O.bfe_timed(O.RASTRIGIN, NX, xr[:r1], 1, 3)
+    _, ta1 = O.bfe_timed(O.ACKLEY, NX, xa[:r1], 1, 3)
     return dict(value=2 * rows / (tr + ta), unit="evals/s", cores=n, kind="port",
+                value_1thread=2 * r1 / (tr1 + ta1), host=host_info(),
                 sample=f"oracle thread_bfe restatement, rastrigin + ackley, {rows} x {NX} rows each of the same "
-                       f"seed-42 matrices, median of 5 calls, {n} threads")
+                       f"seed-42 matrices, median of 5 calls, {n} threads (the host's cores available to this "
+                       f"process, at most 16: one GPU's share of its box); value_1thread: {r1} rows each, "
+                       f"one thread, median of 3")
 
 
 def uniform_on_device(P, torch, count, seed, a, b):
@@ -380,6 +406,12 @@ def bench_archipelago(P, torch, gens, O=None):
         t0 = time.perf_counter()
         a = P.Archipelago.seeded(isl)  # synchronous
         res[name + "_create_s"] = time.perf_counter() - t0
+        if mode == P.ARCHI_BATCHED:
+            # the launch mapping from the tuner (one timed evolve call per
+            # candidate, from the initial state it then restores)
+            t0 = time.perf_counter()
+            res["mapping"] = a.tune(reps=1)
+            res["tune_s"] = time.perf_counter() - t0
         torch.cuda.synchronize()
         # three evolve calls of `gens` generations each; the first two capture
         # the batched mode's graphs (the first call starts from X, later ones
@@ -411,7 +443,9 @@ def bench_archipelago(P, torch, gens, O=None):
     res.update(workload=f"64 uneven islands (pop logU[32,4096], dim logU[10,500], rosenbrock/rastrigin/"
                         f"lennard_jones round robin), pso_gen {gens} generations per evolve call (the third call "
                         f"timed); initial populations drawn and evaluated on the device",
-               speedup=res["per_island_s"] / res["batched_s"], identical=bool(same))
+               speedup=res["per_island_s"] / res["batched_s"], identical=bool(same),
+               batched_us_per_generation=res["batched_s"] / gens * 1e6,
+               per_island_us_per_generation=res["per_island_s"] / gens * 1e6)
     return res
 
 

@@ -311,10 +311,11 @@ pgh_status pgh_archi_get_trace(pgh_archi *a, uint64_t k, double *trace_F);
  * result depends on it): a generation moves, evaluates and updates the memory
  * of every island's individuals in items of rows, handed out by a queue to a
  * persistent grid.
- *   form:          PGH_ARCHI_STEP_ONE, one launch per generation over every
- *                  item; PGH_ARCHI_STEP_SPLIT, one launch per problem class
- *                  (rosenbrock, separable, lennard_jones), each with its own
- *                  LDS and register footprint (batched mode only)
+ *   form:          PGH_ARCHI_STEP_ONE (the default), one launch per
+ *                  generation over every item; PGH_ARCHI_STEP_SPLIT, one
+ *                  launch per problem class (rosenbrock, separable,
+ *                  lennard_jones), each with its own LDS and register
+ *                  footprint (batched mode only)
  *   lj_lds:        LDS bytes for a Lennard-Jones item's staged individuals
  *                  (as many individuals per item as fit, at least one; 0 =
  *                  64 KiB, at most 64 KiB)

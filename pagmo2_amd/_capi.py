@@ -494,7 +494,7 @@ class Archipelago:
     def evolve(self, mode=ARCHI_BATCHED, stream=None):
         check(lib().pgh_archi_evolve(self.h, mode, _stream(stream) if stream is not None else None))
 
-    def set_mapping(self, form=ARCHI_STEP_SPLIT, lj_lds=0, blocks_per_cu=0):
+    def set_mapping(self, form=ARCHI_STEP_ONE, lj_lds=0, blocks_per_cu=0):
         m = pgh_archi_mapping(form, lj_lds, blocks_per_cu, 0)
         check(lib().pgh_archi_set_mapping(self.h, C.byref(m)))
 

@@ -990,7 +990,7 @@ struct pgh_archi {
     uint32_t class_begin[4] = {0, 0, 0, 0}; // per step class, into the class order (+ end)
     size_t lj_lds = 0;                      // LDS of the Lennard-Jones items
     uint32_t n_cus = 0;
-    pgh_archi_mapping map{PGH_ARCHI_STEP_SPLIT, 0u, 0u, 0u};
+    pgh_archi_mapping map{PGH_ARCHI_STEP_ONE, 0u, 0u, 0u}; // the 64-island workload's fastest form
     uint32_t step_per_cu[4] = {1u, 1u, 1u, 1u}; // k_step workgroups per CU, per class
     uint32_t n_segs = 0;
     jump_tab jt{nullptr, nullptr};
