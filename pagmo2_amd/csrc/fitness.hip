@@ -106,9 +106,10 @@ __global__ __launch_bounds__(kBlock, sep_min_waves(RASTRIGIN)) void k_rastrigin_
 // A wavefront takes 8 rows through column chunks of 64: for each row the
 // lanes read 64 consecutive elements (contiguous loads), and compute in
 // parallel each element's term of the sum and, with GRAD, its gradient
-// (stored contiguously).  The terms go to the wavefront's LDS tile, where
-// lane r (r < 8) then adds row r's terms in the reference's order
-// (rosenbrock.cpp:62-64), so the sum is the reference's bit for bit.  Each
+// (stored contiguously, non-temporal: nothing reads it back).  The terms go
+// to the wavefront's LDS tile, where lane r (r < 8) then adds row r's terms in
+// the reference's order (rosenbrock.cpp:62-64), so the sum is the reference's
+// bit for bit; the next chunk's loads are issued before that sum.  Each
 // wavefront uses its own tile: no workgroup barrier.
 constexpr uint32_t kRWRows = 8, kRWStride = 65; // odd stride: the 8 summing lanes hit distinct banks
 
@@ -123,32 +124,38 @@ __global__ __launch_bounds__(kBlock) void k_rosen_wave(const double *__restrict_
     if (row0 >= n) return; // wavefront-uniform
     const uint32_t rows = static_cast<uint32_t>(min(static_cast<uint64_t>(kRWRows), n - row0));
     double *tile = terms + wave * kRWRows * kRWStride;
-    double acc = 0.;
-    for (uint32_t c0 = 0; c0 < nx; c0 += W) {
-        const uint32_t i = c0 + lane;
-        // Clamped indices: lanes past the row's end take element nx - 1, and
-        // rows past the batch the last row, so that every load and store
-        // stays in the batch with no branch; such a lane stores the very
-        // value that the element's own lane stores.  Its term is not summed.
-        const uint32_t ii = min(i, nx - 1u), in = min(ii + 1u, nx - 1u), im = ii == 0u ? 0u : ii - 1u;
-        double xm[kRWRows], xi[kRWRows], xn[kRWRows];
+    const double *__restrict__ base = dvs + row0 * nx;
+    double xm[kRWRows], xi[kRWRows], xn[kRWRows];
+    // Clamped indices: lanes past the row's end take element nx - 1, and rows
+    // past the batch the last row, so that every load stays in the batch with
+    // no branch; such a lane stores nothing and its term is not summed.
+    const auto load = [&](uint32_t c0) {
+        const uint32_t ii = min(c0 + lane, nx - 1u), in = min(ii + 1u, nx - 1u), im = ii == 0u ? 0u : ii - 1u;
 #pragma unroll
         for (uint32_t r = 0; r < kRWRows; ++r) {
-            const double *x = dvs + (row0 + min(r, rows - 1u)) * nx;
+            const double *x = base + static_cast<uint64_t>(min(r, rows - 1u)) * nx;
             xi[r] = x[ii];
             xn[r] = x[in];
             if constexpr (GRAD) xm[r] = x[im];
         }
+    };
+    load(0u);
+    double acc = 0.;
+    for (uint32_t c0 = 0; c0 < nx; c0 += W) {
+        const uint32_t ii = min(c0 + lane, nx - 1u);
 #pragma unroll
         for (uint32_t r = 0; r < kRWRows; ++r) {
             const double d = xi[r] * xi[r] - xn[r];
-            const double term = 100. * d * d + (xi[r] - 1) * (xi[r] - 1);
-            tile[r * kRWStride + lane] = term;
+            tile[r * kRWStride + lane] = 100. * d * d + (xi[r] - 1) * (xi[r] - 1);
             if constexpr (GRAD) {
-                grad[(row0 + min(r, rows - 1u)) * nx + ii] = rosenbrock_grad3(xm[r], xi[r], xn[r], nx, ii);
+                const double gv = rosenbrock_grad3(xm[r], xi[r], xn[r], nx, ii);
+                if (r < rows && c0 + lane < nx) {
+                    __builtin_nontemporal_store(gv, grad + (row0 + r) * nx + ii);
+                }
             }
         }
         __builtin_amdgcn_wave_barrier();
+        if (c0 + W < nx) load(c0 + W);
         // terms i = c0 .. c0 + cnt - 1 of row `lane`, in order
         const uint32_t cnt = c0 + 1u < nx ? min(W, nx - 1u - c0) : 0u;
         if (lane < rows) {

@@ -245,6 +245,15 @@ __device__ __forceinline__ void separable_pair(sep_acc<KIND, FAST> &acc, const d
     }
 }
 
+// Two gradient elements to a 16-byte aligned slot, non-temporal (written
+// once, never read back by the kernel).
+__device__ __forceinline__ void store_pair_nt(double2 *p, double a, double b)
+{
+    typedef double v2 __attribute__((ext_vector_type(2)));
+    const v2 v = {a, b};
+    __builtin_nontemporal_store(v, reinterpret_cast<v2 *>(p));
+}
+
 // rastrigin.cpp:103, g = 2 x + 10 omega sin(omega x) with sin_2pi_small:
 // valid for |x| < 2^10.  A row holding a larger element (or inf, NaN) is
 // flagged by the fitness pass, and its gradient is computed again with the
@@ -277,7 +286,7 @@ __device__ __forceinline__ double separable_eval_impl(const double *__restrict__
         const auto add_round = [&](const double2 &v, uint32_t j) {
             acc.add(v.x, 2u * (c + j * L));
             acc.add(v.y, 2u * (c + j * L) + 1u);
-            if constexpr (GRAD) grow[j * L] = make_double2(rastrigin_grad_fast(v.x), rastrigin_grad_fast(v.y));
+            if constexpr (GRAD) store_pair_nt(grow + j * L, rastrigin_grad_fast(v.x), rastrigin_grad_fast(v.y));
         };
         if (full > 0u) {
             // two rounds per iteration with the loads alternating between a
@@ -308,7 +317,7 @@ __device__ __forceinline__ double separable_eval_impl(const double *__restrict__
             const double2 v = row[full * L];
             acc.add(v.x, 2u * (c + full * L));
             acc.add(v.y, 2u * (c + full * L) + 1u);
-            if constexpr (GRAD) grow[full * L] = make_double2(rastrigin_grad_fast(v.x), rastrigin_grad_fast(v.y));
+            if constexpr (GRAD) store_pair_nt(grow + full * L, rastrigin_grad_fast(v.x), rastrigin_grad_fast(v.y));
         }
     } else {
         for (; c < npairs; c += L) separable_pair<KIND, L, ALIGNED, FAST>(acc, x, nx, c);
