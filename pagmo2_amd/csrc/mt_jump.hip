@@ -62,6 +62,13 @@ pgh_status jump_table_bytes(const std::vector<uint64_t> &dists, std::vector<uint
     return PGH_OK;
 }
 
+pgh_status jump_polys_multiples(uint64_t S, uint64_t count)
+{
+    if (!mtj::the_field().ok) return fail(PGH_DEVICE_ERROR, "mt19937 characteristic polynomial not found");
+    mtj::prefetch_multiples(S, count);
+    return PGH_OK;
+}
+
 pgh_status jump_table_device(const std::vector<uint64_t> &dists, const void **out)
 {
     static std::mutex mu;

@@ -361,5 +361,23 @@ inline void prefetch_jump_polys(const std::vector<uint64_t> &Js, unsigned thread
     for (size_t i = 0; i < todo.size(); ++i) c.map.emplace(todo[i], std::move(res[i]));
 }
 
+// z^(k S) mod phi for k = 1 .. count, memoised: one multiplication by z^S
+// each rather than one exponentiation each.
+inline void prefetch_multiples(uint64_t S, uint64_t count)
+{
+    if (count == 0u || S == 0u) return;
+    const field &f = the_field();
+    const poly step = jump_poly(S);
+    poly cur = step;
+    std::vector<std::pair<uint64_t, poly>> res;
+    for (uint64_t k = 2; k <= count; ++k) {
+        cur = f.mul(cur, step);
+        res.emplace_back(k * S, cur);
+    }
+    poly_cache &c = the_cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    for (auto &kv : res) c.map.emplace(kv.first, std::move(kv.second));
+}
+
 } // namespace mtj
 } // namespace pgh

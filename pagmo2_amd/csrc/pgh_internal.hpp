@@ -87,6 +87,9 @@ pgh_status launch_batch_random_dv(const prob &p, uint64_t n, uint32_t seed, doub
 pgh_status jump_table_device(const std::vector<uint64_t> &dists, const void **out);
 // The same table as bytes, for a caller that keeps its own device copy.
 pgh_status jump_table_bytes(const std::vector<uint64_t> &dists, std::vector<uint8_t> &out);
+// Memoises z^(k S) mod phi for k = 1 .. count (host), so that a table of those
+// distances costs one multiplication per entry.
+pgh_status jump_polys_multiples(uint64_t S, uint64_t count);
 // Bytes of the table's header (one 32-bit length per distance, 16-byte padded).
 constexpr size_t jump_table_head(size_t n_dists)
 {

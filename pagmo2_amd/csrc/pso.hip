@@ -45,19 +45,20 @@ struct island_desc {
     uint64_t xoff; // X, V, lbX offset (doubles)
     uint64_t foff; // fit, lbfit, offset (doubles)
     uint64_t woff; // word buffer offset (u32)
+    uint64_t whalf; // offset of the second half of a double-buffered word buffer (batches alternate), 0: one
     uint64_t toff; // trace offset (doubles), gens x pop
     uint64_t eoff; // cumulative pop*nx of the islands before this one
     // the generator split in `segs` segments: a batch of gps generations
     // (kGenBatch, or all of a call's when fewer) is bwords = gps W words, and
     // segment s hands out its words s swords .. (s + 1) swords - 1 (the last
     // segment the rest); seg0 is the first segment's index; j_* index the
-    // archipelago's jump table: z^n0 (the velocity draws), z^(swords 2^b) for
-    // b = 0.. (to segment s), z^(bwords - swords) and z^((segs - 1) swords)
-    // (from the end of a full segment, and of the last one, to the same
-    // segment of the next batch)
+    // archipelago's jump table: z^n0 (the velocity draws), z^(s swords) at
+    // j_pos + s - 1 (to segment s), z^(bwords - swords) and
+    // z^((segs - 1) swords) (from the end of a full segment, and of the last
+    // one, to the same segment of the next batch)
     uint64_t bwords, swords;
     uint32_t segs, seg0, gps;
-    uint32_t j_n0, j_tree, j_main, j_last;
+    uint32_t j_n0, j_pos, j_main, j_last;
 };
 
 struct seg_desc {
@@ -139,8 +140,7 @@ __device__ __forceinline__ void store_state(mt_state *__restrict__ dst, const ui
 // k_rng phase 0, the start of an evolve call: every segment takes the
 // island's canonical generator (everything handed out so far); segment 0
 // hands out the velocity draws and every segment moves to its share of the
-// first batch, n0 + s swords (jumps z^n0 and z^(swords 2^b) over the bits of
-// s).  k_rng_gen, batch b: each segment hands out its share; the segment
+// first batch, n0 + s swords (jumps z^n0 and z^(s swords)).  k_rng_gen, batch b: each segment hands out its share; the segment
 // that hands out the island's last word keeps the generator there as the
 // canonical one.  k_rng phase 2, after batch b when another follows: each
 // segment with more to hand out jumps from the end of its share to its share
@@ -186,9 +186,7 @@ __global__ __launch_bounds__(kMtJumpBlock) void k_rng(const island_desc *__restr
             if (di.gens == 0u) store_state(&st[sg.island], ring, K, C);
         } else {
             if (n0) jump(K, C, n0, di.j_n0);
-            for (uint32_t bit = 0; (sg.s >> bit) != 0u; ++bit) {
-                if ((sg.s >> bit) & 1u) jump(K, C, di.swords << bit, di.j_tree + bit);
-            }
+            jump(K, C, static_cast<uint64_t>(sg.s) * di.swords, di.j_pos + sg.s - 1u);
         }
         store_state(mine, ring, K, C);
         return;
@@ -232,7 +230,7 @@ __global__ __launch_bounds__(kMtGenBlock) void k_rng_gen(const island_desc *__re
     const seg_desc sg = sd[sg0 + blockIdx.x];
     const island_desc di = d[sg.island];
     rng_gen_segment(di, sg.s, b, velocity_words(di, v_valid), &st[sg.island], &segst[di.seg0 + sg.s],
-                    words + di.woff, ring);
+                    words + di.woff + ((b & 1u) ? di.whalf : 0u), ring);
 }
 
 // Start of an evolve call (pso_gen.cpp:183-200): V = U(minv, maxv) per
@@ -320,7 +318,8 @@ __device__ __forceinline__ double pso_move(const island_desc &di, uint32_t dd, d
 __device__ __forceinline__ const uint32_t *gen_words(const island_desc &di, const uint32_t *__restrict__ words,
                                                      uint32_t gen, uint64_t m)
 {
-    return words + di.woff + static_cast<uint64_t>(gen % kGenBatch) * (4ull * di.pop * di.nx) + 4u * m;
+    const uint64_t half = ((gen / kGenBatch) & 1u) ? di.whalf : 0u;
+    return words + di.woff + half + static_cast<uint64_t>(gen % kGenBatch) * (4ull * di.pop * di.nx) + 4u * m;
 }
 
 // Velocity and position of element m of island di in generation gen.
@@ -995,6 +994,8 @@ struct pgh_archi {
     uint32_t n_segs = 0;
     jump_tab jt{nullptr, nullptr};
     hipStream_t own = nullptr;
+    hipStream_t side = nullptr; // a batched evolve's random words for the next batch
+    hipEvent_t ev_main = nullptr, ev_side = nullptr;
     hipStream_t last = nullptr;
     hipEvent_t order = nullptr; // orders a call after the previous call's stream
     hipGraph_t graph = nullptr;
@@ -1012,6 +1013,9 @@ struct pgh_archi {
         if (graph) (void)hipGraphDestroy(graph);
         if (order) (void)hipEventDestroy(order);
         if (own) (void)hipStreamDestroy(own);
+        if (side) (void)hipStreamDestroy(side);
+        if (ev_main) (void)hipEventDestroy(ev_main);
+        if (ev_side) (void)hipEventDestroy(ev_side);
     }
 };
 
@@ -1219,10 +1223,31 @@ pgh_status launch_range(pgh_archi *a, uint32_t i0, uint32_t i1, bool v_valid, bo
     // ... and the next batch's words handed out beside it, into the other
     // half of the word buffer, when the generator is one segment
     const bool overlap = fused && a->words_half != 0u && a->h_desc[i0].segs == 1u;
+    // several islands at once: the next batch's words are made on the side
+    // stream while the current batch's generations run (each island with more
+    // than one batch alternates between the two halves of its word buffer)
+    const bool side_rng = !fused && i1 > i0 + 1u;
     for (uint32_t g = 0; g < gens; ++g) {
         if (g % kGenBatch == 0u) {
             const uint32_t b = g / kGenBatch;
-            if (!overlap || g == 0u) {
+            if (side_rng) {
+                if (b == 0u) {
+                    k_rng_gen<<<ng, kMtGenBlock, 0, s>>>(d, sd, st, segst, w, sg0, 0u, vv);
+                    PGH_HIP(hipGetLastError());
+                } else {
+                    PGH_HIP(hipStreamWaitEvent(s, a->ev_side, 0)); // batch b's words
+                }
+                if (g + kGenBatch < gens) {
+                    // batch b - 1's generations are done: its half may take
+                    // batch b + 1's words
+                    PGH_HIP(hipEventRecord(a->ev_main, s));
+                    PGH_HIP(hipStreamWaitEvent(a->side, a->ev_main, 0));
+                    k_rng<<<ng, kMtJumpBlock, 0, a->side>>>(d, sd, st, segst, jt, w, sg0, 2u, b, vv);
+                    k_rng_gen<<<ng, kMtGenBlock, 0, a->side>>>(d, sd, st, segst, w, sg0, b + 1u, vv);
+                    PGH_HIP(hipGetLastError());
+                    PGH_HIP(hipEventRecord(a->ev_side, a->side));
+                }
+            } else if (!overlap || g == 0u) {
                 if (b > 0u) k_rng<<<ng, kMtJumpBlock, 0, s>>>(d, sd, st, segst, jt, w, sg0, 2u, b - 1u, vv);
                 k_rng_gen<<<ng, kMtGenBlock, 0, s>>>(d, sd, st, segst, w, sg0, b, vv);
                 PGH_HIP(hipGetLastError());
@@ -1314,10 +1339,10 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
     // islands' batches (a power of two in [kSegMin, kSegMax]), so that the
     // batch's words are made by workgroups spread over the device while the
     // jumps that position the segments (each costs about as much as making
-    // a few hundred thousand words) stay few.  The jumps
-    // z^(seg_words 2^b) that move segment s to its first share are the same
-    // for every island and come first in the jump table; other distances are
-    // entered once each.
+    // a few hundred thousand words) stay few.  The jumps z^(s seg_words) that
+    // move segment s to its first share are the same for every island and
+    // come first in the jump table (one multiplication each on the host);
+    // other distances are entered once each.
     uint64_t seg_words = kSegMin, max_segs = 1u;
     {
         uint64_t total = 0;
@@ -1338,7 +1363,8 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
             if (bw[k] > seg_words) max_segs = std::max<uint64_t>(max_segs, (bw[k] + seg_words - 1u) / seg_words);
         }
     }
-    for (uint64_t b = 0; (max_segs - 1u) >> b; ++b) dists.push_back(seg_words << b);
+    for (uint64_t k = 1; k < max_segs; ++k) dists.push_back(k * seg_words);
+    PGH_TRY(jump_polys_multiples(seg_words, max_segs - 1u));
     const auto dist_index = [&dists](uint64_t J) {
         for (size_t i = 0; i < dists.size(); ++i) {
             if (dists[i] == J) return static_cast<uint32_t>(i);
@@ -1412,12 +1438,17 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
         di.eoff = eoff;
         xoff += align_up(elems, 32u); // 256-byte aligned rows blocks
         foff += align_up(sp.pop, 32u);
-        // the words of a batch of generations; a lone island of up to 2^22
-        // words per batch gets room for a second batch, which the one-shot
-        // evolve fills while it evolves the first (k_gens_one_tile)
+        // the words of a batch of generations.  Islands of more than one batch
+        // in an archipelago of several get room for two, used in turn: a
+        // batched evolve makes the next batch's words while the current batch's
+        // generations run.  A lone island of up to 2^22 words per batch gets
+        // room for a second batch too, which the one-shot evolve fills while it
+        // evolves the first (k_gens_one_tile, its own alternation).
         const uint64_t wbatch = align_up(elems * 4u * kGenBatch, 64u);
         const bool second = n_islands == 1u && wbatch <= (uint64_t(1) << 22);
-        woff += second ? 2u * wbatch : wbatch;
+        const bool alternate = n_islands > 1u && q.gen > kGenBatch;
+        di.whalf = alternate ? wbatch : 0u;
+        woff += (second || alternate) ? 2u * wbatch : wbatch;
         if (second) a->words_half = wbatch;
         toff += static_cast<uint64_t>(q.gen) * sp.pop;
         eoff += elems;
@@ -1446,10 +1477,10 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
             di.seg0 = static_cast<uint32_t>(seg_list.size());
             di.swords = std::min(di.bwords, seg_words);
             di.segs = static_cast<uint32_t>((di.bwords + di.swords - 1u) / di.swords);
-            di.j_n0 = di.j_tree = di.j_main = di.j_last = 0u;
+            di.j_n0 = di.j_pos = di.j_main = di.j_last = 0u;
             if (di.segs > 1u) {
                 di.j_n0 = dist_index(2u * elems);
-                di.j_tree = 0u; // the shared z^(seg_words 2^b) block
+                di.j_pos = 0u; // the shared z^(s seg_words) block
                 di.j_main = dist_index(di.bwords - di.swords);
                 di.j_last = dist_index(static_cast<uint64_t>(di.segs - 1u) * di.swords);
             }
@@ -1466,6 +1497,9 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
     a->total_words = woff;
     a->total_trace = toff;
     PGH_HIP(hipStreamCreateWithFlags(&a->own, hipStreamNonBlocking));
+    PGH_HIP(hipStreamCreateWithFlags(&a->side, hipStreamNonBlocking));
+    PGH_HIP(hipEventCreateWithFlags(&a->ev_main, hipEventDisableTiming));
+    PGH_HIP(hipEventCreateWithFlags(&a->ev_side, hipEventDisableTiming));
     PGH_HIP(hipEventCreateWithFlags(&a->order, hipEventDisableTiming));
     a->n_segs = static_cast<uint32_t>(seg_list.size());
     std::vector<uint8_t> tab;
