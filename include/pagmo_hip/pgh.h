@@ -201,10 +201,11 @@ pgh_status pgh_fnds_copy(pgh_fnds_ws *ws, uint64_t *dom_count, uint64_t *rank, u
  *      Synchronises `stream`.
  *
  *      Peel workgroups of a workspace: 0 = the default rule (one per 256
- *      points, 8 to the CU count); at most 4096.  pgh_fnds_tune tries one per
- *      64, 128, 256, 512 and 1024 points, within 8 and the CU count, and sets
- *      the fastest on the workspace (*chosen = 0 when n > 32768: nothing to
- *      tune). */
+ *      points); at most 4096 may be set, and a run clamps the count to the
+ *      peel's range: at least one per 1024 points, at most one per 64 points,
+ *      the CU count and 256.  pgh_fnds_tune tries one per 64, 128, 256, 512
+ *      and 1024 points within that range, and sets the fastest on the
+ *      workspace (*chosen = 0 when n > 32768: nothing to tune). */
 pgh_status pgh_fnds_set_peel_blocks(pgh_fnds_ws *ws, uint32_t blocks);
 pgh_status pgh_fnds_tune(pgh_fnds_ws *ws, const double *objs, uint64_t n, uint64_t m, uint32_t reps, void *stream,
                          uint32_t *chosen);

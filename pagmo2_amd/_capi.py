@@ -323,10 +323,10 @@ class Fnds:
             lib().pgh_fnds_destroy(self.h)
             self.h = None
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
         # at interpreter exit the process end reclaims device memory; no HIP
         # call is made from the finalizer then
-        if not sys.is_finalizing():
+        if not _finalizing():
             self.close()
 
     def run(self, objs, stream=None):
@@ -486,9 +486,9 @@ class Archipelago:
             lib().pgh_archi_destroy(self.h)
             self.h = None
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
         # as for Fnds: nothing is released while the interpreter finalizes
-        if not sys.is_finalizing():
+        if not _finalizing():
             self.close()
 
     def evolve(self, mode=ARCHI_BATCHED, stream=None):

@@ -16,12 +16,13 @@
 //               reference's push_back order (multi_objective.cpp:215-227 appends
 //               j < i while scanning row i, then larger indices in scan order)
 //   k_front0 +  the front peel (multi_objective.cpp:228-254), n <= 32768:
-//   k_peel_grid F0 ascending; then per level, one pass over the whole grid
-//               decrements the counters of the rows dominated by the current
-//               front and records for each the position (in the current front)
-//               of its last dominator; the elements whose counter reached zero
-//               are ordered by (that position, index) -- the order in which the
-//               reference's nested loop appends them -- in one workgroup.
+//   k_peel_cols F0 ascending; then per level, each workgroup decrements the
+//               counters of its own columns (in LDS) for the rows of the
+//               current front and records for each element the position of
+//               its last dominator; the elements whose counter reached zero are
+//               exchanged between workgroups and ordered by (that position,
+//               index) -- the order in which the reference's nested loop
+//               appends them -- in every workgroup.
 //   k_peel      the same peel in one workgroup for larger n.
 #include <hip/hip_runtime.h>
 
@@ -313,45 +314,59 @@ __global__ __launch_bounds__(kPeelBlock) void k_peel(const uint64_t *__restrict_
 }
 
 
-// ---- grid-wide peel (n <= 32768) --------------------------------------------
-// Level L turns front L into front L+1.  Its (member, word) items are claimed
-// in chunks by any workgroup through claim[L]; each set bit decrements the
-// counter of the dominated element and raises its key to ((L+1) << 16 |
-// position of the member), and the decrement that reaches zero appends the
-// element to a completion list.  done[L] counts processed items: the
-// workgroup whose addition completes the level orders the list by (position,
-// index) -- the order of the reference's nested loop -- writes front L+1 and
-// publishes its end offset.  No workgroup ever waits for another one to
-// arrive, only for a level to be published by a workgroup that is running, so
-// the kernel needs no co-residency: workgroups that start late join the
-// current level or find the work done.  Every wait is still bounded (a spin
-// limit sets a flag the host reports) so that no wave can loop forever.
-// Control words: ctl[0] completion count of the current level, ctl[1] number
-// of published fronts (front_offsets[0..ctl[1]] are final), ctl[2] set once
-// the last front is known.  lvl[2L] = claim[L], lvl[2L+1] = done[L].
+// ---- column-partitioned peel (n <= 32768) -----------------------------------
+// Workgroup b owns the columns (dominated elements) of D's words
+// [b wpw, (b + 1) wpw): their counters and keys live in its LDS and only it
+// changes them.  Every workgroup holds the current front in LDS (slot = the
+// member's position in the front, low 16 bits = its index).  Level L, in
+// every workgroup:
+//   1. for each (member at position i, own word w), every set bit c of
+//      D[member][w] raises key[c] to ((L+1) << 16 | i) and decrements cnt[c];
+//      the decrement that reaches zero appends c to the workgroup's list;
+//   2. wavefront 0 writes the list as sort keys (position of the element's
+//      last dominator in front L << 16 | index) into the level's half of comp
+//      at the workgroup's first column, writes the elements' ranks, and
+//      publishes its flag ((L+1) << 16 | list length);
+//   3. thread g < G waits for workgroup g's flag of level L; the lists are
+//      gathered into LDS and sorted by key -- the order in which the
+//      reference's nested loop appends them (multi_objective.cpp:236-251) --
+//      which is front L+1.  Workgroup 0 writes it out.
+// So a level costs one pass of D loads and one exchange of flags and lists.
+// Every workgroup waits for every other one, so all must be resident at once:
+// the kernel is launched cooperatively.  Each half of comp and flag is written
+// again two levels later, only after every workgroup has published the level
+// in between, which each does after gathering this one.  Every wait is
+// bounded: a spin limit sets a flag that the host reports, and workgroups
+// that see it stop waiting.
 constexpr uint32_t kGridBlock = 1024;
 constexpr uint32_t kGridMaxN = 32768;
-// Upper limit of a set peel grid.  The peel is correct for any count:
-// workgroups that are not resident start after the resident ones finish and
-// find the work done.
+// Columns per workgroup: at most 64 kColWords (counters, keys and list in LDS
+// beside the front).  Workgroups: at most kColMaxG (one flag poller each).
+constexpr uint32_t kColWords = 16;
+constexpr uint32_t kColMaxG = 256;
+// The largest count pgh_fnds_set_peel_blocks accepts; the kernel clamps any
+// count to the range it supports (the fronts do not depend on it).
 constexpr uint32_t kMaxPeelBlocks = 4096;
-constexpr uint32_t kChunk = 1024;         // items per claim: one per thread
-constexpr uint32_t kSpinLimit = 1u << 26; // ~2 s of s_sleep(1) polls
+constexpr uint32_t kSpinLimit = 1u << 22; // polls of about 1 us each: a few seconds
+// Fronts of up to kColSortMax elements are ordered by counting, larger ones by
+// a bitonic sort.
+constexpr uint32_t kColSortMax = 1024;
+constexpr size_t kColMaxLds = (static_cast<size_t>(kGridMaxN) + 3u * 64u * kColWords) * 4u;
 
 __device__ __forceinline__ uint64_t ld_relaxed64(const uint64_t *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_release(uint32_t *p, uint32_t v)
+__device__ __forceinline__ void st_relaxed(uint32_t *p, uint32_t v)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// F0 in ascending order (multi_objective.cpp:228-233) and the control words.
+// F0 in ascending order (multi_objective.cpp:228-233).
 __global__ __launch_bounds__(kPeelBlock) void k_front0(const uint32_t *__restrict__ dom_count, uint32_t n,
                                                        uint64_t *__restrict__ front_items,
                                                        uint64_t *__restrict__ front_offsets,
-                                                       uint64_t *__restrict__ rank, uint32_t *__restrict__ ctl)
+                                                       uint64_t *__restrict__ rank)
 {
     __shared__ uint64_t sh[kPeelBlock / 64u];
     const uint32_t t = threadIdx.x;
@@ -370,9 +385,6 @@ __global__ __launch_bounds__(kPeelBlock) void k_front0(const uint32_t *__restric
     if (t == 0) {
         front_offsets[0] = 0;
         front_offsets[1] = total;
-        ctl[0] = 0;
-        ctl[1] = 1;
-        ctl[2] = 0;
     }
 }
 
@@ -397,149 +409,178 @@ __device__ __forceinline__ void block_bitonic_sort(uint32_t *sh, uint32_t P)
     }
 }
 
-// Front L+1 from the m completed elements of level L (the workgroup that
-// completed the level, after acquiring every other workgroup's writes).
-__device__ void publish_front(uint32_t L, uint32_t e, uint32_t n, uint32_t *sort_lds,
-                              const uint32_t *comp, const uint32_t *key, uint64_t *front_items,
-                              uint64_t *front_offsets, uint64_t *rank, uint32_t *ctl, uint64_t *scalars)
-{
-    const uint32_t t = threadIdx.x;
-    const uint32_t m = min(ld_relaxed(&ctl[0]), n - e);
-    // sort keys (position << 16 | index): index < n <= 32768 and position <
-    // front size <= 32768, so the keys are distinct
-    auto sort_key = [&](uint32_t i) {
-        const uint32_t j = min(ld_relaxed(&comp[i]), n - 1u);
-        return ((ld_relaxed(&key[j]) & 0xffffu) << 16) | j;
-    };
-    if (m <= kGridBlock) {
-        // rank of each key among the m keys: one pass, no stages
-        if (t < m) sort_lds[t] = sort_key(t);
-        __syncthreads();
-        if (t < m) {
-            const uint32_t v = sort_lds[t];
-            uint32_t r = 0;
-            for (uint32_t i = 0; i < m; ++i) r += sort_lds[i] < v;
-            front_items[e + r] = v & 0xffffu;
-            rank[v & 0xffffu] = L + 1u;
-        }
-    } else {
-        uint32_t P = 1;
-        while (P < m) P <<= 1;
-        for (uint32_t i = t; i < P; i += blockDim.x) sort_lds[i] = i < m ? sort_key(i) : 0xffffffffu;
-        __syncthreads();
-        block_bitonic_sort(sort_lds, P);
-        for (uint32_t i = t; i < m; i += blockDim.x) {
-            const uint32_t j = sort_lds[i] & 0xffffu;
-            front_items[e + i] = j;
-            rank[j] = L + 1u;
-        }
-    }
-    __syncthreads();
-    if (t == 0) {
-        if (m) {
-            front_offsets[L + 2u] = e + m;
-            ctl[0] = 0;
-            st_release(&ctl[1], L + 2u);
-        } else {
-            scalars[0] = L + 1u; // number of fronts
-            st_release(&ctl[2], 1u);
-        }
-    }
-}
-
-__global__ __launch_bounds__(kGridBlock) void k_peel_grid(const uint64_t *__restrict__ D, uint32_t n, uint32_t words,
-                                                          uint32_t *__restrict__ cnt, uint32_t *__restrict__ key,
-                                                          uint32_t *__restrict__ comp,
+// Dynamic LDS: F (P words: the front, then the next front's keys), then cnt,
+// key and list (64 wpw words each).  The grid is ceil(words / wpw)
+// workgroups, at most kColMaxG, all resident; P >= n is a power of two.
+__global__ __launch_bounds__(kGridBlock) void k_peel_cols(const uint64_t *__restrict__ D, uint32_t n, uint32_t words,
+                                                          uint32_t wpw, uint32_t P, const uint32_t *__restrict__ cnt0,
+                                                          uint32_t *__restrict__ comp, uint32_t *__restrict__ flag,
                                                           uint64_t *__restrict__ front_items,
                                                           uint64_t *__restrict__ front_offsets,
-                                                          uint64_t *__restrict__ rank, uint32_t *__restrict__ ctl,
-                                                          uint32_t *__restrict__ lvl, uint64_t *__restrict__ scalars)
+                                                          uint64_t *__restrict__ rank, uint64_t *__restrict__ scalars)
 {
-    extern __shared__ uint32_t sort_lds[];
-    __shared__ uint32_t sh_word[2];
-    const uint32_t t = threadIdx.x;
-    for (uint32_t L = 0; L < n; ++L) {
-        // wait until front L is published (or the last front is known)
-        if (t == 0) {
-            uint32_t state = 0, spins = 0;
-            for (;;) {
-                if (ld_relaxed(&ctl[1]) > L) {
-                    state = 1;
-                    break;
-                }
-                if (ld_relaxed(&ctl[2])) break;
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins == kSpinLimit) {
-                    atomicOr(reinterpret_cast<unsigned long long *>(&scalars[2]), 1ull);
-                    break;
-                }
+    extern __shared__ __align__(16) uint32_t lds[];
+    uint32_t *F = lds;
+    uint32_t *cnt = F + P;
+    uint32_t *key = cnt + 64u * wpw;
+    uint32_t *list = key + 64u * wpw;
+    __shared__ uint32_t s_off[kColMaxG + 1u];
+    __shared__ uint32_t s_rank[kColSortMax]; // the counting sort's ranks
+    __shared__ uint32_t s_len, s_abort;
+    const uint32_t G = gridDim.x, b = blockIdx.x, t = threadIdx.x;
+    const uint32_t w0 = b * wpw;
+    const uint32_t wn = min(wpw, words - w0); // >= 1 by the grid size
+    const uint32_t c0 = 64u * w0;
+    const uint32_t cn = min(64u * wn, n - c0);
+    for (uint32_t c = t; c < 64u * wpw; c += kGridBlock) {
+        cnt[c] = c < cn ? cnt0[c0 + c] : 0u;
+        key[c] = 0u;
+    }
+    uint32_t s = min(static_cast<uint32_t>(front_offsets[1]), n); // front 0 (k_front0)
+    for (uint32_t i = t; i < s; i += kGridBlock) F[i] = static_cast<uint32_t>(front_items[i]);
+    uint32_t e = s; // end of the current front in front_items
+    if (t == 0) s_abort = 0u;
+    for (uint32_t L = 0;; ++L) {
+        const uint32_t q = L & 1u;
+        const uint32_t tag = (L + 1u) << 16; // L + 1 <= n <= 2^15
+        if (t == 0) s_len = 0u;
+        __syncthreads();
+        // 1. the decrements by front L, in 64-, 32- or 16-bit pieces of the
+        // own words so that small fronts spread over the workgroup
+        const uint32_t items = s * wn;
+        const uint32_t ps = items * 4u <= kGridBlock ? 2u : (items * 2u <= kGridBlock ? 1u : 0u);
+        const uint32_t pbits = 64u >> ps;
+        const uint64_t pmask = ps ? (uint64_t(1) << pbits) - 1u : ~uint64_t(0);
+        for (uint32_t it = t; it < (items << ps); it += kGridBlock) {
+            const uint32_t wi = it >> ps, piece = it & ((1u << ps) - 1u);
+            const uint32_t i = wi / wn, w = wi - i * wn;
+            const uint64_t word = D[static_cast<uint64_t>(F[i] & 0xffffu) * words + w0 + w];
+            uint64_t bits = (word >> (piece * pbits)) & pmask;
+            const uint32_t base = w * 64u + piece * pbits;
+            while (bits) {
+                const uint32_t c = base + static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(bits)) - 1);
+                bits &= bits - 1u;
+                atomicMax(&key[c], tag | i);
+                if (atomicSub(&cnt[c], 1u) == 1u) list[atomicAdd(&s_len, 1u)] = c;
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            sh_word[0] = state;
         }
         __syncthreads();
-        if (!sh_word[0]) return;
-        // n <= 32768: at most 2^15 * 2^9 (member, word) items per level
-        const uint32_t s = static_cast<uint32_t>(ld_relaxed64(&front_offsets[L]));
-        const uint32_t e = static_cast<uint32_t>(ld_relaxed64(&front_offsets[L + 1u]));
-        // items: 16-bit pieces of the rows of front L (four per word), so
-        // that a level spreads over more lanes; each lane issues the atomics
-        // of up to four set bits before it waits for their results
-        const uint32_t pieces = 4u * words;
-        const uint32_t items = (e - s) * pieces;
-        const uint32_t tag = (L + 1u) << 16;
-        for (;;) {
-            __syncthreads();
-            if (t == 0) sh_word[1] = atomicAdd(&lvl[2u * L], kChunk);
-            __syncthreads();
-            const uint32_t c0 = sh_word[1];
-            if (c0 >= items) break;
-            const uint32_t c1 = min(c0 + kChunk, items);
-            for (uint32_t it = c0 + t; it < c1; it += kGridBlock) {
-                const uint32_t p = it / pieces, q = it - p * pieces, w = q >> 2;
-                uint32_t bits = static_cast<uint32_t>(D[ld_relaxed64(&front_items[s + p]) * words + w] >> (16u * (q & 3u)))
-                                & 0xffffu;
-                const uint32_t base = 16u * q;
-                while (bits) {
-                    uint32_t j[4], old[4];
-                    bool on[4];
+        // 2. publish (wavefront 0; its release covers its own stores) and the
+        // ranks (wavefront 1, read only after the kernel)
+        if (t < 64u) {
+            const uint32_t len = s_len;
+            uint32_t *dst = comp + static_cast<uint64_t>(q) * n + c0;
+            for (uint32_t k = t; k < len; k += 64u) {
+                const uint32_t c = list[k];
+                st_relaxed(&dst[k], ((key[c] & 0xffffu) << 16) | (c0 + c));
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            if (t == 0) st_relaxed(&flag[q * G + b], tag | len);
+            // 3. every workgroup's flag of level L: lane t waits for those of
+            // workgroups 4t .. 4t + 3 (G <= 256), then the exclusive offsets
+            // of the lists in workgroup order
+            uint32_t c4[4] = {0u, 0u, 0u, 0u}, pending = 0u, spins = 0u;
 #pragma unroll
-                    for (uint32_t b = 0; b < 4u; ++b) {
-                        on[b] = bits != 0u;
-                        j[b] = base + static_cast<uint32_t>(__ffs(bits) - 1);
-                        bits &= bits - 1u;
-                    }
+            for (uint32_t r = 0; r < 4u; ++r) pending |= (4u * t + r < G ? 1u : 0u) << r;
+            while (pending) {
+                uint32_t v[4];
 #pragma unroll
-                    for (uint32_t b = 0; b < 4u; ++b) {
-                        if (on[b]) {
-                            atomicMax(&key[j[b]], tag | p);
-                            old[b] = atomicSub(&cnt[j[b]], 1u);
-                        }
-                    }
+                for (uint32_t r = 0; r < 4u; ++r) v[r] = (pending >> r) & 1u ? ld_relaxed(&flag[q * G + 4u * t + r]) : 0u;
 #pragma unroll
-                    for (uint32_t b = 0; b < 4u; ++b) {
-                        if (on[b] && old[b] == 1u) {
-                            const uint32_t slot = atomicAdd(&ctl[0], 1u);
-                            if (slot < n) comp[slot] = j[b];
-                        }
+                for (uint32_t r = 0; r < 4u; ++r) {
+                    if (((pending >> r) & 1u) && (v[r] & 0xffff0000u) == tag) {
+                        c4[r] = v[r] & 0xffffu;
+                        pending &= ~(1u << r);
                     }
                 }
+                if (!pending) break;
+                // another workgroup's time-out, checked every 256 polls
+                if ((++spins & 255u) == 0u && (ld_relaxed64(&scalars[2]) != 0u || spins >= kSpinLimit)) {
+                    atomicOr(reinterpret_cast<unsigned long long *>(&scalars[2]), 1ull);
+                    s_abort = 1u;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
             }
-            __syncthreads();
-            if (t == 0) {
-                const uint32_t done = c1 - c0;
-                const uint32_t before =
-                    __hip_atomic_fetch_add(&lvl[2u * L + 1u], done, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                const uint32_t last = before + done == items;
-                if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                sh_word[0] = last;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            const uint32_t sum = c4[0] + c4[1] + c4[2] + c4[3];
+            uint32_t x = sum;
+#pragma unroll
+            for (uint32_t d = 1; d < 64u; d <<= 1) {
+                const uint32_t y = __shfl_up(x, d);
+                if (t >= d) x += y;
             }
-            __syncthreads();
-            if (sh_word[0]) {
-                publish_front(L, e, n, sort_lds, comp, key, front_items, front_offsets, rank, ctl, scalars);
-                break;
+            uint32_t o = x - sum;
+#pragma unroll
+            for (uint32_t r = 0; r < 4u; ++r) {
+                const uint32_t g = 4u * t + r;
+                if (g < G) s_off[g] = o;
+                o += c4[r];
             }
+            if (t == 63u) s_off[G] = x;
+        } else if (t < 128u) {
+            const uint32_t len = s_len;
+            for (uint32_t k = t - 64u; k < len; k += 64u) rank[c0 + list[k]] = L + 1u;
         }
+        __syncthreads();
+        if (s_abort) return;
+        const uint32_t m = min(s_off[G], n - e);
+        if (m == 0u) {
+            if (b == 0u && t == 0u) scalars[0] = L + 1u; // number of fronts
+            return;
+        }
+        // gather (front L is no longer read): entry k is in the list of the
+        // workgroup g with s_off[g] <= k < s_off[g + 1]
+        for (uint32_t k = t; k < m; k += kGridBlock) {
+            uint32_t lo = 0u, hi = G;
+            while (hi - lo > 1u) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_off[mid] <= k) {
+                    lo = mid;
+                } else {
+                    hi = mid;
+                }
+            }
+            F[k] = ld_relaxed(comp + static_cast<uint64_t>(q) * n + 64u * lo * wpw + (k - s_off[lo]));
+            if (k < kColSortMax) s_rank[k] = 0u;
+        }
+        __syncthreads();
+        // sort by key (distinct).  Up to kColSortMax keys: the rank of each
+        // by counting, wavefront (chunk, part) counting the keys of part
+        // `part` of F below each key of chunk `chunk` (a broadcast LDS read
+        // per key), the parts' counts added in s_rank; beyond, bitonic.
+        if (m <= kColSortMax) {
+            const uint32_t kw = (m + 63u) / 64u; // chunks of 64 keys, <= 16
+            const uint32_t parts = (kGridBlock / 64u) / kw;
+            const uint32_t wave = t / 64u;
+            if (wave < kw * parts) {
+                const uint32_t chunk = wave % kw, part = wave / kw;
+                const uint32_t k = 64u * chunk + (t % 64u);
+                const uint32_t x = k < m ? F[k] : 0xffffffffu;
+                const uint32_t i0 = part * m / parts, i1 = (part + 1u) * m / parts;
+                uint32_t r = 0u;
+#pragma unroll 8
+                for (uint32_t i = i0; i < i1; ++i) r += F[i] < x ? 1u : 0u;
+                if (k < m && r) atomicAdd(&s_rank[k], r);
+            }
+            __syncthreads();
+            const uint32_t x = t < m ? F[t] : 0u, r = t < m ? s_rank[t] : 0u;
+            __syncthreads();
+            if (t < m) F[r] = x;
+            __syncthreads();
+        } else {
+            uint32_t Pm = 1u;
+            while (Pm < m) Pm <<= 1;
+            for (uint32_t i = m + t; i < Pm; i += kGridBlock) F[i] = 0xffffffffu;
+            __syncthreads();
+            block_bitonic_sort(F, Pm);
+        }
+        if (b == 0u) {
+            for (uint32_t k = t; k < m; k += kGridBlock) front_items[e + k] = F[k] & 0xffffu;
+            if (t == 0u) front_offsets[L + 2u] = e + m;
+        }
+        e += m;
+        s = m;
     }
 }
 
@@ -548,22 +589,28 @@ uint32_t grid_for(uint64_t count)
     return static_cast<uint32_t>((count + kBlock - 1u) / kBlock);
 }
 
-
-// Launches k_front0 and the grid-wide peel.
-// One workgroup per 256 elements, at least 8 and at most one per CU (a sweep
-// of 64..1024 elements per workgroup and 512..8192 items per claim on the
-// 8192 x 2 and 16384 x 3 sorts put the optimum here).
-uint32_t default_peel_blocks(uint32_t n32, uint32_t cus)
-{
-    return std::min(cus, std::max(8u, n32 / 256u));
-}
-
 uint32_t device_cus()
 {
     int dev = 0, v = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 256u;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 256u;
     return static_cast<uint32_t>(std::max(v, 1));
+}
+
+// The peel's workgroup range for n points of w32 words: at least enough that
+// a workgroup owns at most kColWords words, at most one per word, per CU and
+// kColMaxG.
+void peel_range(uint32_t w32, uint32_t cus, uint32_t &gmin, uint32_t &gmax)
+{
+    gmin = (w32 + kColWords - 1u) / kColWords;
+    gmax = std::min({w32, cus, kColMaxG});
+}
+
+// Default: one workgroup per four words (256 points), the fastest of the
+// tuner's candidates at 8192 x 2 and 16384 x 3.
+uint32_t default_peel_blocks(uint32_t w32)
+{
+    return (w32 + 3u) / 4u;
 }
 
 pgh_status launch_grid_peel(pgh_fnds_ws *ws, hipStream_t s, uint32_t n32, uint32_t w32)
@@ -575,30 +622,50 @@ pgh_status launch_grid_peel(pgh_fnds_ws *ws, hipStream_t s, uint32_t n32, uint32
     if (dev != dev_cached) {
         int v = 0;
         PGH_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev));
-        // the sort of a level may use up to 128 KiB of dynamic LDS
-        PGH_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_peel_grid),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, 4 * static_cast<int>(kGridMaxN)));
+        PGH_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_peel_cols),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kColMaxLds)));
         cus = static_cast<uint32_t>(std::max(v, 1));
         dev_cached = dev;
     }
+    uint32_t gmin = 0, gmax = 0;
+    peel_range(w32, cus, gmin, gmax);
+    if (gmin > gmax) return fail(PGH_UNSUPPORTED, "fast_non_dominated_sorting: too few CUs for the grid peel");
+    const uint32_t want = ws->peel_blocks ? ws->peel_blocks : default_peel_blocks(w32);
+    const uint32_t g = std::min(std::max(want, gmin), gmax);
+    const uint32_t wpw = (w32 + g - 1u) / g; // <= kColWords, since g >= gmin
+    const uint32_t G = (w32 + wpw - 1u) / wpw; // <= g, and no workgroup without a word
     uint32_t P = 1;
     while (P < n32) P <<= 1;
-    const size_t lds = static_cast<size_t>(std::max(P, kGridBlock)) * 4u;
-    const uint32_t blocks = ws->peel_blocks ? ws->peel_blocks : default_peel_blocks(n32, cus);
-    auto *cnt = static_cast<uint32_t *>(ws->cnt32.ptr);
-    auto *key = static_cast<uint32_t *>(ws->key32.ptr);
+    const size_t lds = (static_cast<size_t>(P) + 3u * 64u * wpw) * 4u;
+    int per_cu = 0;
+    PGH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&k_peel_cols),
+                                                         static_cast<int>(kGridBlock), lds));
+    if (per_cu < 1 || static_cast<uint64_t>(per_cu) * cus < G) {
+        return fail(PGH_DEVICE_ERROR, "fast_non_dominated_sorting: the grid peel does not fit the device");
+    }
+    const uint64_t *D = static_cast<const uint64_t *>(ws->D.ptr);
+    const uint32_t *cnt = static_cast<const uint32_t *>(ws->cnt32.ptr);
     auto *comp = static_cast<uint32_t *>(ws->comp.ptr);
-    auto *ctl = static_cast<uint32_t *>(ws->ctl.ptr);
-    auto *lvl = static_cast<uint32_t *>(ws->lvl.ptr);
+    auto *flag = static_cast<uint32_t *>(ws->ctl.ptr);
     auto *fi = static_cast<uint64_t *>(ws->front_items.ptr);
     auto *fo = static_cast<uint64_t *>(ws->front_offsets.ptr);
     auto *rk = static_cast<uint64_t *>(ws->rank.ptr);
     auto *sc = static_cast<uint64_t *>(ws->scalars.ptr);
-    k_fill_u32<<<grid_for(2u * n32), kBlock, 0, s>>>(lvl, 2u * n32, 0u);
-    k_front0<<<1, kPeelBlock, 0, s>>>(cnt, n32, fi, fo, rk, ctl);
-    k_peel_grid<<<blocks, kGridBlock, lds, s>>>(static_cast<const uint64_t *>(ws->D.ptr), n32, w32, cnt, key, comp,
-                                                 fi, fo, rk, ctl, lvl, sc);
+    k_fill_u32<<<grid_for(2u * G), kBlock, 0, s>>>(flag, 2u * G, 0u);
+    k_front0<<<1, kPeelBlock, 0, s>>>(cnt, n32, fi, fo, rk);
     PGH_HIP(hipGetLastError());
+    uint32_t n_a = n32, w_a = w32, wpw_a = wpw, P_a = P;
+    void *args[] = {&D, &n_a, &w_a, &wpw_a, &P_a, &cnt, &comp, &flag, &fi, &fo, &rk, &sc};
+    const hipError_t le = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_peel_cols), dim3(G),
+                                                     dim3(kGridBlock), args, static_cast<unsigned int>(lds), s);
+    if (le != hipSuccess) {
+        // the runtime refused a co-resident grid: the single-workgroup peel
+        // gives the same fronts (it restarts from the dominator counts)
+        (void)hipGetLastError();
+        k_peel<<<1, kPeelBlock, 0, s>>>(D, n32, w32, static_cast<uint32_t *>(ws->cnt32.ptr),
+                                        static_cast<uint32_t *>(ws->key32.ptr), fi, fo, rk, sc);
+        PGH_HIP(hipGetLastError());
+    }
     return PGH_OK;
 }
 
@@ -750,9 +817,8 @@ try {
     k_scan<<<1, kPeelBlock, 0, s>>>(ndom, cnt, n32, dom_offsets, static_cast<uint64_t *>(ws->dom_count.ptr), scalars);
     PGH_HIP(hipGetLastError());
     if (n <= kGridMaxN) {
-        PGH_TRY(ws->comp.reserve(n * 4u));
-        PGH_TRY(ws->ctl.reserve(64u));
-        PGH_TRY(ws->lvl.reserve(n * 8u));
+        PGH_TRY(ws->comp.reserve(n * 8u));
+        PGH_TRY(ws->ctl.reserve(2u * kColMaxG * 4u));
         PGH_TRY(launch_grid_peel(ws, s, n32, w32));
     } else {
         k_peel<<<1, kPeelBlock, 0, s>>>(D, n32, w32, cnt, key, static_cast<uint64_t *>(ws->front_items.ptr),
@@ -789,10 +855,12 @@ try {
         return pgh_fnds_run(ws, objs, n, m, stream);
     }
     const uint32_t n32 = static_cast<uint32_t>(n);
-    const uint32_t cus = device_cus();
+    const uint32_t w32 = (n32 + 63u) / 64u;
+    uint32_t gmin = 0, gmax = 0;
+    peel_range(w32, device_cus(), gmin, gmax);
     std::vector<uint32_t> cands;
     for (uint32_t per : {64u, 128u, 256u, 512u, 1024u}) {
-        const uint32_t b = std::min(cus, std::max(8u, n32 / per));
+        const uint32_t b = std::min(gmax, std::max(gmin, (n32 + per - 1u) / per));
         if (std::find(cands.begin(), cands.end(), b) == cands.end()) cands.push_back(b);
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;

@@ -29,7 +29,7 @@ struct pgh_fnds_ws {
     uint64_t n = 0, m = 0, words = 0;
     pgh::dev_buf keys, D, cnt32, ndom32, key32, dom_count, rank, front_items, front_offsets, dom_offsets, dom_items,
         scalars;
-    pgh::dev_buf comp, ctl, lvl;
+    pgh::dev_buf comp, ctl; // the grid peel's sort-key lists and flags (two halves each)
     uint64_t h_sizes[3] = {0, 0, 0}; // n_fronts, n_dom_items, peel barrier timeout
     bool sized = false;
     // workgroups of the grid-wide front peel: 0 = the default rule, else the
