@@ -167,6 +167,99 @@ __global__ __launch_bounds__(kBlock) void k_rosen_wave(const double *__restrict_
     if (lane < rows && fvs) fvs[row0 + lane] = acc;
 }
 
+// ---- rosenbrock over even rows of 16-byte aligned matrices: 4 rows per ------
+// wavefront, column chunks of 128 K.  Lane l reads elements c0 + 128 k + 2l
+// and c0 + 128 k + 2l + 1 (k < K) of each row with 16-byte loads (contiguous
+// KiBs per row and instruction, which stream from HBM better than
+// k_rosen_wave's 512-byte rows); the neighbours x[i - 1] and x[i + 1] come
+// from the adjacent lanes, across pieces k and k + 1 from lanes 63 and 0, the
+// chunk's last one from one more load and the previous chunk's from a carried
+// register.  Terms, their in-order sum by lane r < 4 and the gradient are
+// k_rosen_wave's, bit for bit.
+// (K = 2 measured no faster than K = 1 at 65536 x 1000, with 104 VGPRs against 80.)
+constexpr uint32_t kRPRows = 4, kRPK = 1, kRPCols = 128u * kRPK, kRPStride = kRPCols + 1u;
+
+__device__ __forceinline__ double shfl_dbl(double v, int src)
+{
+    return __shfl(v, src, 64);
+}
+
+template <bool GRAD>
+__global__ __launch_bounds__(kBlock) void k_rosen_pairs(const double *__restrict__ dvs, uint64_t n, uint32_t nx,
+                                                        double *__restrict__ fvs, double *__restrict__ grad)
+{
+    constexpr uint32_t W = 64, kWaves = kBlock / W;
+    __shared__ double terms[kWaves * kRPRows * kRPStride];
+    const uint32_t wave = threadIdx.x / W, lane = threadIdx.x % W;
+    const uint64_t row0 = (static_cast<uint64_t>(blockIdx.x) * kWaves + wave) * kRPRows;
+    if (row0 >= n) return; // wavefront-uniform
+    const uint32_t rows = static_cast<uint32_t>(min(static_cast<uint64_t>(kRPRows), n - row0));
+    double *tile = terms + wave * kRPRows * kRPStride;
+    const uint32_t half = nx / 2u; // pairs per row (nx even)
+    const double *__restrict__ base = dvs + row0 * nx;
+    double2 v[kRPRows][kRPK];
+    double ext[kRPRows], carry[kRPRows];
+    // Clamped indices: lanes past the row's end take its last pair, rows past
+    // the batch the last row, so that every load stays in the batch; such
+    // lanes store nothing and their terms are not summed.
+    const auto load = [&](uint32_t c0) {
+        const uint32_t ce = min(c0 + kRPCols, nx - 1u);
+#pragma unroll
+        for (uint32_t r = 0; r < kRPRows; ++r) {
+            const double *x = base + static_cast<uint64_t>(min(r, rows - 1u)) * nx;
+#pragma unroll
+            for (uint32_t k = 0; k < kRPK; ++k) {
+                v[r][k] = reinterpret_cast<const double2 *>(x)[min(c0 / 2u + 64u * k + lane, half - 1u)];
+            }
+            ext[r] = x[ce];
+        }
+    };
+#pragma unroll
+    for (uint32_t r = 0; r < kRPRows; ++r) carry[r] = 0.;
+    load(0u);
+    double acc = 0.;
+    const int up_lane = static_cast<int>((lane + W - 1u) % W), down_lane = static_cast<int>((lane + 1u) % W);
+    for (uint32_t c0 = 0; c0 < nx; c0 += kRPCols) {
+#pragma unroll
+        for (uint32_t r = 0; r < kRPRows; ++r) {
+#pragma unroll
+            for (uint32_t k = 0; k < kRPK; ++k) {
+                const uint32_t i = c0 + 128u * k + 2u * lane; // this lane's first element of piece k
+                const double a = v[r][k].x, b = v[r][k].y;
+                const double down = shfl_dbl(a, down_lane); // x[i + 2] for lane < 63
+                // lane 63: x[i + 2] is lane 0's of piece k + 1, or the chunk's
+                // next element (every lane takes part in the shuffles)
+                const double after = k + 1u < kRPK ? shfl_dbl(v[r][k + 1u < kRPK ? k + 1u : k].x, 0) : ext[r];
+                const double xn = lane + 1u < W ? down : after;
+                const double da = a * a - b, db = b * b - xn;
+                tile[r * kRPStride + 128u * k + 2u * lane] = 100. * da * da + (a - 1) * (a - 1);
+                tile[r * kRPStride + 128u * k + 2u * lane + 1u] = 100. * db * db + (b - 1) * (b - 1);
+                if constexpr (GRAD) {
+                    const double up = shfl_dbl(b, up_lane); // x[i - 1] for lane > 0
+                    // lane 0: x[i - 1] is lane 63's of piece k - 1, or the carried one
+                    const double prev = k > 0u ? shfl_dbl(v[r][k > 0u ? k - 1u : 0u].y, 63) : carry[r];
+                    const double xm = lane ? up : prev;
+                    if (r < rows && i < nx) {
+                        store_pair_nt(reinterpret_cast<double2 *>(grad + (row0 + r) * nx + i),
+                                      rosenbrock_grad3(xm, a, b, nx, i), rosenbrock_grad3(a, b, xn, nx, i + 1u));
+                    }
+                }
+            }
+            if constexpr (GRAD) carry[r] = shfl_dbl(v[r][kRPK - 1u].y, 63); // the chunk's last element
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (c0 + kRPCols < nx) load(c0 + kRPCols);
+        // terms i = c0 .. c0 + cnt - 1 of row `lane`, in order
+        const uint32_t cnt = c0 + 1u < nx ? min(kRPCols, nx - 1u - c0) : 0u;
+        if (lane < rows) {
+            const double *t = tile + lane * kRPStride;
+            for (uint32_t j = 0; j < cnt; ++j) acc += t[j];
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (lane < rows && fvs) fvs[row0 + lane] = acc;
+}
+
 // ---- multi-objective sphere: one lane per individual ------------------------
 __global__ __launch_bounds__(kBlock) void k_mo_sphere(const double *__restrict__ dvs, uint64_t n, uint32_t nx,
                                                       uint32_t nobj, const double *__restrict__ centers,
@@ -414,6 +507,17 @@ pgh_status launch_rastrigin_fg(const double *dvs, uint64_t n, uint32_t nx, doubl
 
 pgh_status launch_rosen_rows(const double *dvs, uint64_t n, uint32_t nx, double *fvs, double *grad, hipStream_t s)
 {
+    // even rows of 16-byte aligned matrices: 16-byte loads and stores
+    const bool pairs = nx % 2u == 0u && reinterpret_cast<uintptr_t>(dvs) % 16u == 0u &&
+                       (!grad || reinterpret_cast<uintptr_t>(grad) % 16u == 0u);
+    if (pairs) {
+        constexpr uint64_t rpb = kBlock / 64u * kRPRows;
+        const dim3 grid(static_cast<uint32_t>((n + rpb - 1u) / rpb));
+        if (grad) k_rosen_pairs<true><<<grid, dim3(kBlock), 0, s>>>(dvs, n, nx, fvs, grad);
+        else k_rosen_pairs<false><<<grid, dim3(kBlock), 0, s>>>(dvs, n, nx, fvs, nullptr);
+        PGH_HIP(hipGetLastError());
+        return PGH_OK;
+    }
     constexpr uint64_t rows_per_block = kBlock / 64u * kRWRows;
     const dim3 grid(static_cast<uint32_t>((n + rows_per_block - 1u) / rows_per_block));
     if (grad) k_rosen_wave<true><<<grid, dim3(kBlock), 0, s>>>(dvs, n, nx, fvs, grad);
