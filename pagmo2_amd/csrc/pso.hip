@@ -483,6 +483,9 @@ __device__ __forceinline__ void eval_tile(const island_desc &di, const tile &tl,
             return;
         case LENNARD_JONES:
             switch (di.lanes) {
+                case 64:
+                    eval_lj<64>(di, tl, c, lds);
+                    return;
                 case 32:
                     eval_lj<32>(di, tl, c, lds);
                     return;
@@ -525,6 +528,15 @@ __device__ __forceinline__ void eval_tile(const island_desc &di, const tile &tl,
 // evolve (update_elem, eval_tile), bit for bit.
 
 enum : uint32_t { kStepRosen = 0u, kStepSep = 1u, kStepLj = 2u, kStepAll = 3u };
+
+// Lanes per Lennard-Jones individual in an archipelago: the batch evaluator's
+// (lj_lanes), or a whole wavefront from 80 atoms on (16 lanes own the atoms,
+// four quarters of them split the schedule's steps), so that one large
+// cluster's evaluation, which bounds a generation, takes half as many steps.
+__host__ __device__ constexpr uint32_t archi_lj_lanes(uint32_t atoms)
+{
+    return atoms >= 80u ? 64u : static_cast<uint32_t>(lj_lanes(atoms));
+}
 
 __host__ __device__ constexpr uint32_t step_class(int kind)
 {
@@ -572,12 +584,12 @@ __device__ __forceinline__ void step_move(const island_desc &di, const tile &tl,
         rw.sel[t] = (sel[p] ? 1u : 0u) | (sel[bn] ? 2u : 0u);
     }
     __syncthreads();
+    // item-local element e (count * nx < 2^32, build_items): row t, column dd
     const uint64_t e0 = static_cast<uint64_t>(tl.first) * di.nx;
-    const uint64_t e1 = e0 + static_cast<uint64_t>(tl.count) * di.nx;
-    for (uint64_t m = e0 + threadIdx.x; m < e1; m += kBlock) {
-        const uint32_t p = static_cast<uint32_t>(m / di.nx);
-        const uint32_t dd = static_cast<uint32_t>(m - static_cast<uint64_t>(p) * di.nx);
-        const uint32_t t = p - tl.first;
+    const uint32_t span = tl.count * di.nx;
+    for (uint32_t e = threadIdx.x; e < span; e += kBlock) {
+        const uint32_t t = e / di.nx, dd = e - t * di.nx;
+        const uint64_t m = e0 + e;
         const uint32_t bn = rw.bn[t], sb = rw.sel[t];
         const uint4 w = *reinterpret_cast<const uint4 *>(gen_words(di, a.words, a.gen, m));
         const double r1 = mt_canonical(w.x, w.y);
@@ -733,6 +745,9 @@ __device__ __forceinline__ void step_eval_lj_l(const island_desc &di, const tile
 __device__ __forceinline__ void step_eval_lj(const island_desc &di, const tile &tl, const step_args &a, double *lds)
 {
     switch (di.lanes) {
+        case 64:
+            step_eval_lj_l<64>(di, tl, a, lds);
+            return;
         case 32:
             step_eval_lj_l<32>(di, tl, a, lds);
             return;
@@ -1062,7 +1077,10 @@ pgh_status build_items(pgh_archi *a)
         uint32_t per = 0u;
         double unit = 0.; // per row, in units of one element's move
         if (cls == kStepRosen) {
-            per = kStepRosenRows;
+            // 32 rows (one per lane of the evaluation's 4 x 8), fewer for long
+            // rows so that an item's move stays near 4096 elements: the
+            // largest item bounds the generation
+            per = di.nx > 256u ? kStepRosenRows / 4u : (di.nx > 128u ? kStepRosenRows / 2u : kStepRosenRows);
             unit = 1.2 * di.nx;
         } else if (cls == kStepLj) {
             const uint64_t bytes = static_cast<uint64_t>(lj_stride(di.dim, false)) * sizeof(double);
@@ -1074,6 +1092,8 @@ pgh_status build_items(pgh_archi *a)
             per = kBlock / di.lanes;
             unit = 1.3 * di.nx;
         }
+        // an item's elements are counted in 32 bits (step_move)
+        per = std::max<uint32_t>(1u, std::min<uint64_t>(per, UINT32_MAX / std::max(di.nx, 1u)));
         for (uint32_t f = 0; f < di.pop; f += per) {
             const uint32_t c = std::min(per, di.pop - f);
             a->h_items.push_back(tile{static_cast<uint32_t>(k), f, c, 0u});
@@ -1457,8 +1477,8 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
         // evaluation tiles
         uint32_t per = 256u;
         if (p.kind == PGH_LENNARD_JONES) {
-            di.lanes = static_cast<uint32_t>(lj_lanes(p.dim));
-            per = lj_groups_for(p.dim, false);
+            di.lanes = archi_lj_lanes(p.dim);
+            per = std::min(lj_groups_for(p.dim, false), 256u / di.lanes);
             if (per == 0u) return fail(PGH_UNSUPPORTED, "lennard_jones cluster too large for one workgroup");
             a->lds_bytes =
                 std::max<size_t>(a->lds_bytes, static_cast<size_t>(per) * lj_stride(p.dim, false) * sizeof(double));
