@@ -502,6 +502,8 @@ def main():
     ap.add_argument("--archi-gens", type=int, default=200)
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 256 MiB device copy (the reference kernel for HBM byte counters)")
+    ap.add_argument("--skip-headline", action="store_true",
+                    help="run only the extra sections (to profile them on their own); the line has no headline")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the islands and individuals of this "
@@ -522,38 +524,41 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    log("separable bulk: generating inputs and timing")
-    sep = bench_separable(P, torch, args.steps, args.warmup)
-    ms_per_step = sep["wall"] / args.steps * 1e3
-    value = 2 * POP / (ms_per_step * 1e-3)
-    bytes_per_launch = POP * (8 * NX + 8)
-    dom_name, dom_ms = ("rastrigin", sep["ras_ms"]) if sep["ras_ms"] >= sep["ack_ms"] else ("ackley", sep["ack_ms"])
-    achieved = bytes_per_launch / (dom_ms * 1e-3) / 1e9
-    roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                traffic=None, kernel=f"k_separable<{dom_name}, {SEP_LANES[dom_name]} lanes, aligned> (one bfe call)", kernel_ms=dom_ms,
-                other={"rastrigin_ms": sep["ras_ms"], "ackley_ms": sep["ack_ms"],
-                       "rastrigin_graph_replay_ms": sep["ras_graph"], "ackley_graph_replay_ms": sep["ack_graph"],
-                       "rastrigin_GBps": bytes_per_launch / (sep["ras_ms"] * 1e-3) / 1e9,
-                       "ackley_GBps": bytes_per_launch / (sep["ack_ms"] * 1e-3) / 1e9})
-    traffic_file = os.path.join(ROOT, "profiles", "separable_hbm_traffic.json")
-    if os.path.exists(traffic_file):  # from rocprofv3 FETCH_SIZE / WRITE_SIZE runs, copy-kernel calibrated
-        with open(traffic_file) as f:
-            pmc = json.load(f)
-        roof["traffic"] = pmc.get(dom_name, {}).get("traffic_bytes")
-    line = {
-        "metric": METRIC, "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic: std::mt19937(42) draws on the device in batch_random_decision_vector order",
-        "config": {"workload": "separable bulk evaluation: rastrigin + ackley, n=1000, pop=65536, one bfe call "
-                               "each per step", "n": NX, "pop": POP, "functions": ["rastrigin", "ackley"]},
-        "roofline": roof,
-    }
-    if not args.no_cpu:
-        log("cpu baseline")
-        line["cpu_baseline"] = cpu_baseline(O, sep)
-    del sep
-    torch.cuda.empty_cache()
+    if args.skip_headline:
+        line = {"metric": METRIC, "skipped_headline": True}
+    else:
+        log("separable bulk: generating inputs and timing")
+        sep = bench_separable(P, torch, args.steps, args.warmup)
+        ms_per_step = sep["wall"] / args.steps * 1e3
+        value = 2 * POP / (ms_per_step * 1e-3)
+        bytes_per_launch = POP * (8 * NX + 8)
+        dom_name, dom_ms = ("rastrigin", sep["ras_ms"]) if sep["ras_ms"] >= sep["ack_ms"] else ("ackley", sep["ack_ms"])
+        achieved = bytes_per_launch / (dom_ms * 1e-3) / 1e9
+        roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                    traffic=None, kernel=f"k_separable<{dom_name}, {SEP_LANES[dom_name]} lanes, aligned> (one bfe call)", kernel_ms=dom_ms,
+                    other={"rastrigin_ms": sep["ras_ms"], "ackley_ms": sep["ack_ms"],
+                           "rastrigin_graph_replay_ms": sep["ras_graph"], "ackley_graph_replay_ms": sep["ack_graph"],
+                           "rastrigin_GBps": bytes_per_launch / (sep["ras_ms"] * 1e-3) / 1e9,
+                           "ackley_GBps": bytes_per_launch / (sep["ack_ms"] * 1e-3) / 1e9})
+        traffic_file = os.path.join(ROOT, "profiles", "separable_hbm_traffic.json")
+        if os.path.exists(traffic_file):  # from rocprofv3 FETCH_SIZE / WRITE_SIZE runs, copy-kernel calibrated
+            with open(traffic_file) as f:
+                pmc = json.load(f)
+            roof["traffic"] = pmc.get(dom_name, {}).get("traffic_bytes")
+        line = {
+            "metric": METRIC, "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic: std::mt19937(42) draws on the device in batch_random_decision_vector order",
+            "config": {"workload": "separable bulk evaluation: rastrigin + ackley, n=1000, pop=65536, one bfe call "
+                                   "each per step", "n": NX, "pop": POP, "functions": ["rastrigin", "ackley"]},
+            "roofline": roof,
+        }
+        if not args.no_cpu:
+            log("cpu baseline")
+            line["cpu_baseline"] = cpu_baseline(O, sep)
+        del sep
+        torch.cuda.empty_cache()
     if not args.no_extra:
         extra = {}
         for name, fn in (("plain_method", lambda: bench_plain_method(P, O)),
@@ -579,4 +584,7 @@ def main():
 
 
 if __name__ == "__main__":
-    sys.exit(main())
+    # a zero status ends the interpreter normally rather than through SystemExit
+    rc = main()
+    if rc:
+        sys.exit(rc)
