@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -59,6 +60,15 @@ struct island_desc {
     uint64_t bwords, swords;
     uint32_t segs, seg0, gps;
     uint32_t j_n0, j_pos, j_main, j_last;
+    // pso_gen's variant (1..6) and topology (1 gbest, 2 lbest, 3 von Neumann),
+    // the neighbours per particle (lbest 2 radius, von Neumann 4) and the von
+    // Neumann lattice's columns; the random words of a generation (wgen) and
+    // of one unit of it (wpu words per element, or per particle for variants
+    // 3 and 4, pso_gen.cpp:241-327); gbest: the swarm's best position (nx
+    // doubles at goff of the gbest buffer, its fitness after them)
+    uint32_t variant, topo, nbk, vcols;
+    uint64_t wgen, goff;
+    uint32_t wpu, pad_;
 };
 
 struct seg_desc {
@@ -148,7 +158,7 @@ __device__ __forceinline__ void store_state(mt_state *__restrict__ dst, const ui
 __device__ __forceinline__ void seg_span(const island_desc &di, uint32_t s, uint32_t b, uint64_t n0, uint64_t &start,
                                          uint64_t &count, uint64_t &end)
 {
-    const uint64_t W = 4ull * di.pop * di.nx;
+    const uint64_t W = di.wgen;
     end = n0 + static_cast<uint64_t>(di.gens) * W; // one past the island's last word of the call
     start = n0 + static_cast<uint64_t>(b) * di.bwords + static_cast<uint64_t>(s) * di.swords;
     const uint64_t share = s + 1u < di.segs ? di.swords : di.bwords - static_cast<uint64_t>(s) * di.swords;
@@ -272,6 +282,43 @@ __global__ __launch_bounds__(kBlock) void k_vinit(const island_desc *__restrict_
 // Best neighbour of particle p on the lbest ring (pso_gen.cpp:679-699):
 // radius..1 to the left, then 1..radius to the right; the later neighbour
 // wins ties (:615-617).  lbf: the island's memory fitness.
+// Neighbour n of particle p: lbest (pso_gen.cpp:679-699) left radius..1, then
+// right 1..radius; von Neumann (pso_gen.cpp:719-746) west, east, north,
+// south on a (pop / cols) x cols lattice with wrapping edges.
+__device__ __forceinline__ uint32_t neighbour(const island_desc &di, uint32_t p, uint32_t n)
+{
+    const uint32_t S = di.pop;
+    if (di.topo == 3u) {
+        const uint32_t cols = di.vcols, rows = S / cols;
+        const uint32_t px = p % cols, py = p / cols;
+        uint32_t qx = px, qy = py;
+        if (n == 0u) qx = (px + cols - 1u) % cols;
+        else if (n == 1u) qx = (px + 1u) % cols;
+        else if (n == 2u) qy = (py + rows - 1u) % rows;
+        else qy = (py + 1u) % rows;
+        return qy * cols + qx;
+    }
+    const uint32_t R = di.radius;
+    if (n < R) {
+        const uint32_t j = R - n;
+        return p < j ? p - j + S : p - j;
+    }
+    const uint32_t j = n - R + 1u;
+    return p + j >= S ? p + j - S : p + j;
+}
+
+// The best of the neighbours of p (pso_gen.cpp:593-622: a later neighbour wins
+// ties) for the lbest and von Neumann topologies.
+__device__ __noinline__ uint32_t best_neighbour_any(const island_desc &di, uint32_t p, const double *__restrict__ lbf)
+{
+    uint32_t bn = neighbour(di, p, 0u);
+    for (uint32_t n = 1u; n < di.nbk; ++n) {
+        const uint32_t q = neighbour(di, p, n);
+        if (le_f(lbf[q], lbf[bn])) bn = q;
+    }
+    return bn;
+}
+
 __device__ __forceinline__ uint32_t best_neighbour(uint32_t p, uint32_t S, uint32_t R, const double *__restrict__ lbf)
 {
     uint32_t bn = p < R ? p - R + S : p - R;
@@ -286,13 +333,29 @@ __device__ __forceinline__ uint32_t best_neighbour(uint32_t p, uint32_t S, uint3
     return bn;
 }
 
-// Variant 5 velocity (pso_gen.cpp:302-305), its clamp (:336-342) and the
-// bounded move (:345-361) of dimension dd: returns the new position, v the new
-// velocity.
-__device__ __forceinline__ double pso_move(const island_desc &di, uint32_t dd, double x, double v_old, double lbx,
-                                           double best, double r1, double r2, double &v)
+// The velocity of variants 1-5 (pso_gen.cpp:241-306), each expression in the
+// reference's order: 1 and 3 with r1 and r2 (drawn per element or per
+// particle), 2 and 4 with r1 for both terms, 5 with the constriction factor
+// over the whole sum.
+__device__ __forceinline__ double pso_velocity(const island_desc &di, double v_old, double x, double lbx, double best,
+                                               double r1, double r2)
 {
-    v = di.omega * (v_old + di.eta1 * r1 * (lbx - x) + di.eta2 * r2 * (best - x));
+    switch (di.variant) {
+        case 1u:
+        case 3u:
+            return di.omega * v_old + di.eta1 * r1 * (lbx - x) + di.eta2 * r2 * (best - x);
+        case 2u:
+        case 4u:
+            return di.omega * v_old + di.eta1 * r1 * (lbx - x) + di.eta2 * r1 * (best - x);
+        default:
+            return di.omega * (v_old + di.eta1 * r1 * (lbx - x) + di.eta2 * r2 * (best - x));
+    }
+}
+
+// The velocity clamp (pso_gen.cpp:336-342) and the bounded move (:345-361) of
+// dimension dd: returns the new position, v the new velocity.
+__device__ __forceinline__ double pso_clamp_move(const island_desc &di, uint32_t dd, double x, double &v)
+{
     double lb, ub;
     bounds_of(di.kind, dd, lb, ub);
     const double vwidth = (ub - lb) * di.max_vel;
@@ -313,13 +376,23 @@ __device__ __forceinline__ double pso_move(const island_desc &di, uint32_t dd, d
     return nxv;
 }
 
-// The r1, r2 words of element m in generation gen (4 words per element and
-// generation, in draw order).
+// Variant 5 velocity (pso_gen.cpp:302-305), its clamp (:336-342) and the
+// bounded move (:345-361) of dimension dd: returns the new position, v the new
+// velocity.
+__device__ __forceinline__ double pso_move(const island_desc &di, uint32_t dd, double x, double v_old, double lbx,
+                                           double best, double r1, double r2, double &v)
+{
+    v = di.omega * (v_old + di.eta1 * r1 * (lbx - x) + di.eta2 * r2 * (best - x));
+    return pso_clamp_move(di, dd, x, v);
+}
+
+// The words of unit u (an element, or a particle for variants 3 and 4) in
+// generation gen, in draw order: wpu words per unit, wgen per generation.
 __device__ __forceinline__ const uint32_t *gen_words(const island_desc &di, const uint32_t *__restrict__ words,
-                                                     uint32_t gen, uint64_t m)
+                                                     uint32_t gen, uint64_t u)
 {
     const uint64_t half = ((gen / kGenBatch) & 1u) ? di.whalf : 0u;
-    return words + di.woff + half + static_cast<uint64_t>(gen % kGenBatch) * (4ull * di.pop * di.nx) + 4u * m;
+    return words + di.woff + half + static_cast<uint64_t>(gen % kGenBatch) * di.wgen + di.wpu * u;
 }
 
 // Velocity and position of element m of island di in generation gen.
@@ -559,6 +632,7 @@ struct step_args {
     double *lbf_nxt;
     const uint32_t *sel_cur;
     uint32_t *sel_nxt;
+    const double *gbest; // the gbest islands' best positions (island_desc::goff)
     uint32_t gen;
 };
 
@@ -570,11 +644,69 @@ struct step_rows {
     uint32_t sel[kBlock]; // bit 0: the particle's memory is in lbX1, bit 1: the neighbour's
 };
 
+// step_move for every variant and topology but variant 5 with lbest
+// (pso_gen.cpp:232-327): the best neighbour of the island's topology, or the
+// swarm's best position (gbest); words per particle for variants 3 and 4; for
+// variant 6 (FIPS) every neighbour's memory with one draw each.
+__device__ __noinline__ void step_move_general(const island_desc &di, const tile &tl, const step_args &a,
+                                               step_rows &rw)
+{
+    const double *lbf = a.lbf_cur + di.foff;
+    const uint32_t *sel = a.sel_cur + di.foff;
+    const bool fips = di.variant == 6u, gbest = di.topo == 1u;
+    for (uint32_t t = threadIdx.x; t < tl.count; t += kBlock) {
+        const uint32_t p = tl.first + t;
+        const uint32_t bn = (fips || gbest) ? p : best_neighbour_any(di, p, lbf);
+        rw.bn[t] = bn;
+        rw.sel[t] = (sel[p] ? 1u : 0u) | (sel[bn] ? 2u : 0u);
+    }
+    __syncthreads();
+    const uint64_t e0 = static_cast<uint64_t>(tl.first) * di.nx;
+    const uint32_t span = tl.count * di.nx;
+    const double *gb = a.gbest + di.goff;
+    const bool per_particle = di.variant == 3u || di.variant == 4u;
+    const bool one_draw = di.variant == 2u || di.variant == 4u;
+    const double acc_coef = di.eta1 + di.eta2; // pso_gen.cpp:221
+    for (uint32_t e = threadIdx.x; e < span; e += kBlock) {
+        const uint32_t t = e / di.nx, dd = e - t * di.nx;
+        const uint64_t m = e0 + e;
+        const uint32_t p = tl.first + t;
+        const uint32_t sb = rw.sel[t];
+        const uint64_t xi = di.xoff + m;
+        const double x = a.X[xi], v_old = a.V[xi];
+        double v;
+        if (fips) {
+            const uint32_t *w = gen_words(di, a.words, a.gen, m);
+            double sum = 0.;
+            for (uint32_t n = 0; n < di.nbk; ++n) {
+                const uint32_t q = neighbour(di, p, n);
+                const double lq = (sel[q] ? a.lbX1 : a.lbX0)[di.xoff + static_cast<uint64_t>(q) * di.nx + dd];
+                sum += mt_canonical(w[2u * n], w[2u * n + 1u]) * acc_coef * (lq - x);
+            }
+            v = di.omega * (v_old + sum / static_cast<double>(di.nbk));
+        } else {
+            const uint32_t *w = gen_words(di, a.words, a.gen, per_particle ? p : m);
+            const double r1 = mt_canonical(w[0], w[1]);
+            const double r2 = one_draw ? 0. : mt_canonical(w[2], w[3]);
+            const double lbx = ((sb & 1u) ? a.lbX1 : a.lbX0)[xi];
+            const double best = gbest ? gb[dd]
+                                      : ((sb & 2u) ? a.lbX1 : a.lbX0)[di.xoff + static_cast<uint64_t>(rw.bn[t]) * di.nx + dd];
+            v = pso_velocity(di, v_old, x, lbx, best, r1, r2);
+        }
+        a.X[xi] = pso_clamp_move(di, dd, x, v);
+        a.V[xi] = v;
+    }
+}
+
 // Moves the rows of item tl (update_elem's arithmetic, the memory read through
 // the selectors).  All threads of the workgroup; a __syncthreads() must
 // follow.
 __device__ __forceinline__ void step_move(const island_desc &di, const tile &tl, const step_args &a, step_rows &rw)
 {
+    if (di.variant != 5u || di.topo != 2u) { // island-uniform
+        step_move_general(di, tl, a, rw);
+        return;
+    }
     const double *lbf = a.lbf_cur + di.foff;
     const uint32_t *sel = a.sel_cur + di.foff;
     for (uint32_t t = threadIdx.x; t < tl.count; t += kBlock) {
@@ -833,6 +965,108 @@ __global__ __launch_bounds__(kBlock, step_min_waves(CLASS)) void k_step(step_arg
     }
 }
 
+// gbest (pso_gen.cpp:644-662): the swarm's best position at the start of an
+// evolve call is pop.best_idx()'s (population.cpp:285-311 for one objective):
+// the first index of the minimum by operator<, so a NaN at index 0 stays and
+// no other NaN is taken.  One workgroup per island of gb_list; each thread
+// takes a contiguous chunk, and the chunks are combined in index order.
+__global__ __launch_bounds__(kBlock) void k_gbest_init(const island_desc *__restrict__ d,
+                                                       const uint32_t *__restrict__ gb_list,
+                                                       const double *__restrict__ X, const double *__restrict__ fit,
+                                                       double *__restrict__ gbest)
+{
+    __shared__ uint32_t s_idx[kBlock];
+    __shared__ double s_f[kBlock];
+    const island_desc di = d[gb_list[blockIdx.x]];
+    const double *f = fit + di.foff;
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (di.pop + kBlock - 1u) / kBlock;
+    const uint32_t c0 = min(t * per, di.pop), c1 = min(c0 + per, di.pop);
+    uint32_t bi = UINT32_MAX;
+    double bf = 0.;
+    for (uint32_t i = c0; i < c1; ++i) {
+        const double v = f[i];
+        if (v != v) continue;
+        if (bi == UINT32_MAX || v < bf) {
+            bi = i;
+            bf = v;
+        }
+    }
+    s_idx[t] = bi;
+    s_f[t] = bf;
+    __syncthreads();
+    if (t == 0u) {
+        uint32_t w = UINT32_MAX;
+        double wf = 0.;
+        for (uint32_t u = 0; u < kBlock; ++u) {
+            if (s_idx[u] != UINT32_MAX && (w == UINT32_MAX || s_f[u] < wf)) {
+                w = s_idx[u];
+                wf = s_f[u];
+            }
+        }
+        if (f[0] != f[0] || w == UINT32_MAX) w = 0u;
+        s_idx[0] = w;
+    }
+    __syncthreads();
+    const uint32_t w = s_idx[0];
+    double *g = gbest + di.goff;
+    for (uint32_t dd = t; dd < di.nx; dd += kBlock) g[dd] = X[di.xoff + static_cast<uint64_t>(w) * di.nx + dd];
+    if (t == 0u) g[di.nx] = f[w];
+}
+
+// gbest after a generation's memory update (pso_gen.cpp:443-459): in index
+// order, every particle that improved (its selector flipped) whose fitness is
+// <= the swarm's best (less_than_f or equal_to_f) becomes the best.  That is
+// the last improved particle of the least fitness, if it is <= the best.
+__global__ __launch_bounds__(kBlock) void k_gbest_update(const island_desc *__restrict__ d,
+                                                         const uint32_t *__restrict__ gb_list,
+                                                         const double *__restrict__ X, const double *__restrict__ fit,
+                                                         const uint32_t *__restrict__ sel_cur,
+                                                         const uint32_t *__restrict__ sel_nxt, double *__restrict__ gbest,
+                                                         uint32_t gen)
+{
+    __shared__ uint32_t s_idx[kBlock];
+    __shared__ double s_f[kBlock];
+    const island_desc di = d[gb_list[blockIdx.x]];
+    if (gen >= di.gens) return; // workgroup-uniform
+    const double *f = fit + di.foff;
+    const uint32_t *sc = sel_cur + di.foff, *sn = sel_nxt + di.foff;
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (di.pop + kBlock - 1u) / kBlock;
+    const uint32_t c0 = min(t * per, di.pop), c1 = min(c0 + per, di.pop);
+    uint32_t bi = UINT32_MAX;
+    double bf = 0.;
+    for (uint32_t i = c0; i < c1; ++i) {
+        if (sc[i] == sn[i]) continue;
+        const double v = f[i];
+        if (bi == UINT32_MAX || le_f(v, bf)) {
+            bi = i;
+            bf = v;
+        }
+    }
+    s_idx[t] = bi;
+    s_f[t] = bf;
+    __syncthreads();
+    double *g = gbest + di.goff;
+    if (t == 0u) {
+        uint32_t w = UINT32_MAX;
+        double wf = 0.;
+        for (uint32_t u = 0; u < kBlock; ++u) {
+            if (s_idx[u] != UINT32_MAX && (w == UINT32_MAX || le_f(s_f[u], wf))) {
+                w = s_idx[u];
+                wf = s_f[u];
+            }
+        }
+        if (w != UINT32_MAX && !le_f(wf, g[di.nx])) w = UINT32_MAX;
+        s_idx[0] = w;
+    }
+    __syncthreads();
+    const uint32_t w = s_idx[0];
+    if (w == UINT32_MAX) return; // workgroup-uniform
+    for (uint32_t dd = t; dd < di.nx; dd += kBlock) g[dd] = X[di.xoff + static_cast<uint64_t>(w) * di.nx + dd];
+    if (t == 0u) g[di.nx] = f[w];
+}
+
 __global__ __launch_bounds__(kBlock) void k_zero_u32(uint32_t *__restrict__ p, uint64_t n)
 {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
@@ -991,6 +1225,9 @@ struct pgh_archi {
     // and go up in one copy
     dev_buf arena, trace_buf;
     arena_view desc, tiles, segs, jump_table, states, X, V, lbX, fit, lbfit, words, seg_states;
+    // the gbest islands (their indices, in order) and their best positions
+    arena_view gblist, gbest;
+    std::vector<uint32_t> h_gb;
     // k_step: the second memory buffers, the selectors and one queue counter
     // per launch (in the arena); its items (per island, contiguous) and their
     // dispatch orders in a table of their own, rebuilt with the mapping
@@ -1224,7 +1461,18 @@ pgh_status launch_range(pgh_archi *a, uint32_t i0, uint32_t i1, bool v_valid, bo
     PGH_HIP(hipGetLastError());
     // a single island of one tile, evolved by the one-shot entry point: a
     // batch of generations per launch (k_gens_one_tile)
-    const bool fused = a->one_shot && i1 == i0 + 1u && nt == 1u;
+    const bool fused = a->one_shot && i1 == i0 + 1u && nt == 1u && a->h_desc[i0].variant == 5u &&
+                       a->h_desc[i0].topo == 2u;
+    // the gbest islands of the range: the swarm's best at the call's start
+    const auto gb_lo = std::lower_bound(a->h_gb.begin(), a->h_gb.end(), i0);
+    const auto gb_hi = std::lower_bound(a->h_gb.begin(), a->h_gb.end(), i1);
+    const uint32_t n_gb = static_cast<uint32_t>(gb_hi - gb_lo);
+    const uint32_t *gbl = static_cast<const uint32_t *>(a->gblist.ptr) + (gb_lo - a->h_gb.begin());
+    auto *gbest = static_cast<double *>(a->gbest.ptr);
+    if (n_gb) {
+        k_gbest_init<<<n_gb, kBlock, 0, s>>>(d, gbl, X, fit, gbest);
+        PGH_HIP(hipGetLastError());
+    }
     // the island in LDS when it fits beside the evaluation's staging area
     // within 64 KiB: X, V, lbX (pop x nx) and fit, lbfit (pop)
     const uint64_t stage_off = align_up(a->lds_bytes, 256u);
@@ -1308,6 +1556,7 @@ pgh_status launch_range(pgh_archi *a, uint32_t i0, uint32_t i1, bool v_valid, bo
         args.lbf_nxt = (g % 2u) ? lbfit : lbfitB;
         args.sel_cur = (g % 2u) ? selB : selA;
         args.sel_nxt = (g % 2u) ? selA : selB;
+        args.gbest = gbest;
         args.gen = g;
         uint32_t *qg = queues + static_cast<uint64_t>(g) * a->queue_stride;
         if (i1 == i0 + 1u) {
@@ -1329,12 +1578,51 @@ pgh_status launch_range(pgh_archi *a, uint32_t i0, uint32_t i1, bool v_valid, bo
                 PGH_TRY(launch_step(a, c, args, s));
             }
         }
+        if (n_gb) {
+            // gbest after the generation's memory update
+            k_gbest_update<<<n_gb, kBlock, 0, s>>>(d, gbl, X, fit, args.sel_cur, args.sel_nxt, gbest, g);
+            PGH_HIP(hipGetLastError());
+        }
     }
     if (!fused) {
         k_settle<<<blocks_for(total), kBlock, 0, s>>>(d, i0, i1, e0, total, lbX, lbX1, lbfit, lbfitB, selA, selB);
         PGH_HIP(hipGetLastError());
     }
     return PGH_OK;
+}
+
+// Random words of one generation (two per draw, pso_gen.cpp:241-327): per
+// element two draws (variants 1 and 5), one (2) or one per neighbour (6, K
+// neighbours); per particle two (3) or one (4).  wpu: words per unit.
+uint64_t words_per_gen(uint32_t variant, uint32_t K, uint64_t pop, uint64_t nx, uint32_t &wpu)
+{
+    switch (variant) {
+        case 2u:
+            wpu = 2u;
+            return 2u * pop * nx;
+        case 3u:
+            wpu = 4u;
+            return 4u * pop;
+        case 4u:
+            wpu = 2u;
+            return 2u * pop;
+        case 6u:
+            wpu = 2u * K;
+            return 2u * static_cast<uint64_t>(K) * pop * nx;
+        default:
+            wpu = 4u;
+            return 4u * pop * nx;
+    }
+}
+
+// Columns of the von Neumann lattice (pso_gen.cpp:719-746): rows is the
+// largest divisor of the swarm size not above its square root.
+uint32_t von_cols(uint32_t pop)
+{
+    uint32_t rows = static_cast<uint32_t>(std::sqrt(static_cast<double>(pop)));
+    while (rows > 1u && pop % rows != 0u) --rows;
+    rows = std::max(rows, 1u);
+    return pop / rows;
 }
 
 // Initial populations from host arrays (X0, F0) or, with `seeds`, drawn and
@@ -1346,7 +1634,7 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
     if (n_islands == 0u) return fail(PGH_INVALID_ARGUMENT, "an archipelago needs at least one island");
     auto a = std::make_unique<pgh_archi>();
     std::vector<prob> probs;
-    uint64_t xoff = 0, foff = 0, woff = 0, toff = 0, eoff = 0;
+    uint64_t xoff = 0, foff = 0, woff = 0, toff = 0, eoff = 0, gboff = 0;
     std::vector<seg_desc> seg_list;
     std::vector<uint64_t> dists;
     {
@@ -1371,7 +1659,10 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
             prob p;
             PGH_TRY(resolve(&islands[k].udp, p));
             const uint64_t gps = std::min<uint64_t>(kGenBatch, std::max<uint64_t>(islands[k].algo.gen, 1u));
-            bw[k] = gps * 4u * islands[k].pop * p.nx;
+            const pgh_pso_params &q = islands[k].algo;
+            const uint32_t K = q.neighb_type == 3u ? 4u : 2u * (q.neighb_param / 2u);
+            uint32_t wpu = 0u;
+            bw[k] = gps * words_per_gen(q.variant, K, islands[k].pop, p.nx, wpu);
             total += bw[k];
         }
         const uint64_t want = total / (2u * a->n_cus);
@@ -1430,12 +1721,17 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
             return fail(PGH_INVALID_ARGUMENT, "The neighborhood parameter must be in (0, inf), while a value of "
                                                   + std::to_string(q.neighb_param) + " was detected");
         }
-        if (q.variant != 5u || q.neighb_type != 2u) {
-            return fail(PGH_UNSUPPORTED, "the device pso_gen implements variant 5 with the lbest topology");
+        if (q.neighb_type == 4u) {
+            return fail(PGH_UNSUPPORTED, "the device pso_gen does not implement the adaptive random topology (its "
+                                         "draws depend on each generation's results)");
+        }
+        if (q.variant == 6u && q.neighb_type == 1u) {
+            return fail(PGH_UNSUPPORTED, "the device pso_gen does not implement FIPS (variant 6) with the gbest "
+                                         "topology");
         }
         if (sp.pop == 0u) return fail(PGH_INVALID_ARGUMENT, "pso_gen does not work on an empty population");
         const uint32_t radius = q.neighb_param / 2u;
-        if (radius == 0u || sp.pop < radius || sp.pop > (uint64_t(1) << 24)) {
+        if (sp.pop > (uint64_t(1) << 24) || (q.neighb_type == 2u && (radius == 0u || sp.pop < radius))) {
             return fail(PGH_UNSUPPORTED, "population size incompatible with the lbest radius");
         }
         island_desc di{};
@@ -1450,6 +1746,16 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
         di.eta2 = q.eta2;
         di.max_vel = q.max_vel;
         di.memory = q.memory ? 1u : 0u;
+        di.variant = q.variant;
+        di.topo = q.neighb_type;
+        di.nbk = q.neighb_type == 3u ? 4u : 2u * radius;
+        di.vcols = q.neighb_type == 3u ? von_cols(static_cast<uint32_t>(sp.pop)) : 1u;
+        di.wgen = words_per_gen(di.variant, di.nbk, sp.pop, p.nx, di.wpu);
+        if (di.topo == 1u) {
+            di.goff = gboff;
+            gboff += align_up(p.nx + 1u, 32u);
+            a->h_gb.push_back(static_cast<uint32_t>(k));
+        }
         const uint64_t elems = sp.pop * p.nx;
         di.xoff = xoff;
         di.foff = foff;
@@ -1464,7 +1770,7 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
         // generations run.  A lone island of up to 2^22 words per batch gets
         // room for a second batch too, which the one-shot evolve fills while it
         // evolves the first (k_gens_one_tile, its own alternation).
-        const uint64_t wbatch = align_up(elems * 4u * kGenBatch, 64u);
+        const uint64_t wbatch = align_up(di.wgen * kGenBatch, 64u);
         const bool second = n_islands == 1u && wbatch <= (uint64_t(1) << 22);
         const bool alternate = n_islands > 1u && q.gen > kGenBatch;
         di.whalf = alternate ? wbatch : 0u;
@@ -1491,7 +1797,7 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
         // the generator's segments (k_rng): batches of gps generations split in
         // shares of seg_words words (one share when the batch is smaller)
         {
-            const uint64_t W = 4u * elems;
+            const uint64_t W = di.wgen;
             di.gps = std::min<uint32_t>(kGenBatch, std::max<uint32_t>(q.gen, 1u));
             di.bwords = static_cast<uint64_t>(di.gps) * W;
             di.seg0 = static_cast<uint32_t>(seg_list.size());
@@ -1538,12 +1844,14 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
     place(a->segs, seg_list.size() * sizeof(seg_desc));
     place(a->jump_table, std::max<size_t>(tab.size(), 16u));
     place(a->states, n_islands * sizeof(mt_state));
+    place(a->gblist, a->h_gb.size() * sizeof(uint32_t));
     const uint64_t head_bytes = arena_bytes;
     place(a->X, xoff * 8u);
     place(a->V, xoff * 8u);
     place(a->lbX, xoff * 8u);
     place(a->fit, foff * 8u);
     place(a->lbfit, foff * 8u);
+    place(a->gbest, gboff * 8u);
     place(a->words, std::max<uint64_t>(woff, 1u) * 4u);
     place(a->seg_states, seg_list.size() * sizeof(mt_state));
     place(a->lbX1, xoff * 8u);
@@ -1554,9 +1862,9 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
     place(a->queues, std::max<uint64_t>(a->max_gens, 1u) * a->queue_stride * 4u);
     PGH_TRY(a->arena.reserve(arena_bytes));
     auto *base = static_cast<uint8_t *>(a->arena.ptr);
-    for (arena_view *v : {&a->desc, &a->tiles, &a->segs, &a->jump_table, &a->states, &a->X, &a->V, &a->lbX, &a->fit,
-                          &a->lbfit, &a->words, &a->seg_states, &a->lbX1, &a->lbfitB, &a->selA, &a->selB,
-                          &a->queues}) {
+    for (arena_view *v : {&a->desc, &a->tiles, &a->segs, &a->jump_table, &a->states, &a->gblist, &a->X, &a->V,
+                          &a->lbX, &a->fit, &a->lbfit, &a->gbest, &a->words, &a->seg_states, &a->lbX1, &a->lbfitB,
+                          &a->selA, &a->selB, &a->queues}) {
         v->ptr = base + v->off;
     }
     PGH_TRY(build_items(a.get()));
@@ -1570,6 +1878,7 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
         put(a->segs, seg_list.data(), seg_list.size() * sizeof(seg_desc));
         put(a->jump_table, tab.data(), tab.size());
         put(a->states, states.data(), n_islands * sizeof(mt_state));
+        put(a->gblist, a->h_gb.data(), a->h_gb.size() * sizeof(uint32_t));
         PGH_HIP(hipMemcpy(base, head.data(), head_bytes, hipMemcpyHostToDevice));
     }
     a->jt.lens = static_cast<const uint32_t *>(a->jump_table.ptr);

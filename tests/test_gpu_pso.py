@@ -243,11 +243,53 @@ def test_islands_against_oracle():
     a.close()
 
 
-def test_unsupported_variant():
+@pytest.mark.parametrize("variant,topo", [(5, 4), (1, 4), (6, 1)])
+def test_unsupported_variant(variant, topo):
+    # the adaptive random topology draws depend on each generation's results;
+    # FIPS with gbest would take every particle as a neighbour
     X0, F0 = O.population_init(P.ROSENBROCK, 4, 10, 1)
     with pytest.raises(P.PghError) as e:
-        P.Archipelago([dict(kind=P.ROSENBROCK, dim=4, X0=X0, F0=F0, algo=P.pso_params(variant=1))])
+        P.Archipelago([dict(kind=P.ROSENBROCK, dim=4, X0=X0, F0=F0,
+                            algo=P.pso_params(variant=variant, neighb_type=topo))])
     assert e.value.status == P.UNSUPPORTED
+
+
+VARIANT_TOPOLOGIES = [(v, t) for v in range(1, 7) for t in (1, 2, 3) if (v, t) not in ((6, 1), (5, 2))]
+
+
+@pytest.mark.parametrize("variant,topo", VARIANT_TOPOLOGIES)
+def test_variants_and_topologies_against_oracle(variant, topo):
+    """Every device variant (pso_gen.cpp:241-327) with the gbest, lbest and von
+    Neumann topologies, two evolve calls, batched and per-island: positions
+    equal to the oracle's, fitness at each kind's bar (rosenbrock bit-exact)."""
+    spec = [(P.ROSENBROCK, 6, 30), (P.RASTRIGIN, 5, 24), (P.LENNARD_JONES, 5, 20), (P.ROSENBROCK, 3, 7)]
+    isl = []
+    for k, (kind, dim, pop) in enumerate(spec):
+        X0, F0 = O.population_init(kind, dim, pop, 500 + k)
+        isl.append(dict(kind=kind, dim=dim, X0=X0, F0=F0,
+                        algo=P.pso_params(gen=6, seed=600 + k, variant=variant, neighb_type=topo)))
+    for mode in (P.ARCHI_BATCHED, P.ARCHI_PER_ISLAND):
+        a = P.Archipelago(isl)
+        a.evolve(mode)
+        a.evolve(mode)
+        for k, d in enumerate(isl):
+            xg, fg = a.get(k)
+            xo, fo = O.pso_gen_evolve(d["kind"], d["dim"], d["X0"], d["F0"], 6, 600 + k, variant=variant,
+                                      neighb_type=topo, calls=2)
+            assert np.array_equal(xg, xo), (mode, k)
+            assert_fitness(d["kind"], xg, fg, fo, (mode, k))
+        a.close()
+
+
+@pytest.mark.parametrize("variant,topo", [(2, 2), (3, 1), (6, 3)])
+def test_host_entry_point_other_variants(variant, topo):
+    """The one-shot entry point with a variant or topology other than the
+    defaults takes the archipelago's general path; the oracle's result."""
+    X0, F0 = O.population_init(P.ROSENBROCK, 8, 40, 13)
+    prm = P.pso_params(gen=40, seed=9, variant=variant, neighb_type=topo)
+    Xh, Fh = P.pso_gen_evolve_host(P.ROSENBROCK, 8, X0, F0, prm)
+    Xo, Fo = O.pso_gen_evolve(P.ROSENBROCK, 8, X0, F0, 40, 9, variant=variant, neighb_type=topo)
+    assert np.array_equal(Xh, Xo) and np.array_equal(Fh.reshape(-1), Fo.reshape(-1))
 
 
 def test_device_seeded_archipelago():

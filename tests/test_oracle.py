@@ -393,13 +393,30 @@ class _PsoPython:
     """
 
     def __init__(self, seed, lb, ub, fit, radius=2, omega=0.7298, eta1=2.05, eta2=2.05, max_vel=0.5, memory=False,
-                 mistake=None):
+                 mistake=None, variant=5, topo=2):
         self.words = np.random.RandomState(seed)
         self.lb, self.ub, self.fit = list(lb), list(ub), fit
         self.radius, self.omega, self.eta1, self.eta2, self.max_vel = radius, omega, eta1, eta2, max_vel
         self.memory, self.mistake = memory, mistake
+        self.variant, self.topo = variant, topo  # pso_gen.cpp:241-327; 1 gbest, 2 lbest, 3 von Neumann
         self.V = None
         self.cognitive_nonzero = 0  # visits where lbX[p][d] - X[p][d] != 0
+
+    def neighbours(self, pop):
+        if self.topo == 3:  # pso_gen.cpp:719-746, offsets (-1,0), (1,0), (0,-1), (0,1)
+            rows = int(math.sqrt(pop))
+            while pop % rows:
+                rows -= 1
+            cols = pop // rows
+            if self.mistake == "von_transposed":
+                rows, cols = cols, rows
+            return [[((p // cols + dy) % rows) * cols + (p % cols + dx) % cols
+                     for dx, dy in ((-1, 0), (1, 0), (0, -1), (0, 1))] for p in range(pop)]
+        if self.topo == 1:  # pso_gen.cpp:644-662: FIPS sees every particle
+            return [list(range(pop)) for _ in range(pop)]
+        R = self.radius  # pso_gen.cpp:679-699
+        return [[(p - j) % pop for j in range(R, 0, -1)] + [(p + j) % pop for j in range(1, R + 1)]
+                for p in range(pop)]
 
     def draw(self, a, b):
         u0, u1 = (int(w) for w in self.words.randint(0, 2 ** 32, size=2, dtype=np.uint64))
@@ -416,23 +433,50 @@ class _PsoPython:
         if self.V is None or len(self.V) != pop or not self.memory or self.mistake == "velocity_reset":
             self.V = [[self.draw(minv[j], maxv[j]) for j in range(nx)] for _ in range(pop)]
         V = self.V
-        R = self.radius
-        neigh = [[(p - j) % pop for j in range(R, 0, -1)] + [(p + j) % pop for j in range(1, R + 1)]
-                 for p in range(pop)]
+        neigh = self.neighbours(pop)
+        if self.topo == 1:
+            # pop.best_idx(): the first index of the minimum by operator<
+            g = 0
+            for i in range(1, pop):
+                if lbfit[i] < lbfit[g]:
+                    g = i
+            gbX, gbfit = list(X[g]), lbfit[g]
+        acc = self.eta1 + self.eta2
+        w, e1, e2 = self.omega, self.eta1, self.eta2
         for _ in range(gens):
             for p in range(pop):
-                b = neigh[p][0]
-                for q in neigh[p][1:]:
-                    if _le_f(lbfit[q], lbfit[b]) if self.mistake != "first_wins" else lbfit[q] < lbfit[b]:
-                        b = q
-                best = X[b] if self.mistake == "best_from_x" else lbX[b]
+                if self.topo == 1:
+                    best = gbX
+                elif self.variant != 6:
+                    b = neigh[p][0]
+                    for q in neigh[p][1:]:
+                        if _le_f(lbfit[q], lbfit[b]) if self.mistake != "first_wins" else lbfit[q] < lbfit[b]:
+                            b = q
+                    best = X[b] if self.mistake == "best_from_x" else lbX[b]
                 mine = X[p] if self.mistake == "cognitive_x" else lbX[p]
-                for d in range(nx):
+                v = self.variant
+                if v in (3, 4) and self.mistake != "v4_per_element":  # one draw (4) or two (3) per particle
                     r1 = self.draw(0.0, 1.0)
-                    r2 = self.draw(0.0, 1.0)
+                    r2 = self.draw(0.0, 1.0) if v == 3 else r1
+                for d in range(nx):
                     self.cognitive_nonzero += lbX[p][d] != X[p][d]
-                    V[p][d] = self.omega * (V[p][d] + self.eta1 * r1 * (mine[d] - X[p][d])
-                                            + self.eta2 * r2 * (best[d] - X[p][d]))
+                    if v == 6:
+                        sf = 0.0
+                        for n in neigh[p]:
+                            sf += self.draw(0.0, 1.0) * acc * (lbX[n][d] - X[p][d])
+                        V[p][d] = w * (V[p][d] + (sf if self.mistake == "fips_no_mean" else sf / float(len(neigh[p]))))
+                        continue
+                    if v == 4 and self.mistake == "v4_per_element":
+                        r1 = r2 = self.draw(0.0, 1.0)
+                    if v in (1, 5):
+                        r1 = self.draw(0.0, 1.0)
+                        r2 = self.draw(0.0, 1.0)
+                    elif v == 2:
+                        r1 = r2 = self.draw(0.0, 1.0)
+                    if v == 5:
+                        V[p][d] = w * (V[p][d] + e1 * r1 * (mine[d] - X[p][d]) + e2 * r2 * (best[d] - X[p][d]))
+                    else:
+                        V[p][d] = w * V[p][d] + e1 * r1 * (mine[d] - X[p][d]) + e2 * r2 * (best[d] - X[p][d])
             for p in range(pop):
                 for d in range(nx):
                     if V[p][d] > maxv[d]:
@@ -452,6 +496,8 @@ class _PsoPython:
                 if _le_f(fit[p], lbfit[p]):
                     lbfit[p] = fit[p]
                     lbX[p] = list(X[p])
+                    if self.topo == 1 and _le_f(fit[p], gbfit) and self.mistake != "gbest_frozen":
+                        gbX, gbfit = list(X[p]), fit[p]
         # pop.set_xf(i, lbX[i], lbfit[i]), pso_gen.cpp:526-528
         return np.array(lbX), np.array(lbfit)
 
@@ -504,6 +550,41 @@ def test_pso_gen_generations_independent(kind, dim, pop, gens, calls):
     # the memory differs from the position in many visits after generation
     # 1, so the cognitive term is not identically zero in this run
     assert py.cognitive_nonzero > pop * dim
+
+
+@pytest.mark.parametrize("variant,topo", [(v, t) for v in range(1, 7) for t in (1, 2, 3)])
+def test_pso_gen_variants_and_topologies_independent(variant, topo):
+    """Every variant (pso_gen.cpp:241-327) with the gbest, lbest and von
+    Neumann topologies, over two evolve calls: the oracle's pso_gen against the
+    independent Python loop, bit for bit."""
+    kind, dim, pop, gens = O.ROSENBROCK, 3, 12, 3
+    X, F = O.population_init(kind, dim, pop, 21)
+    F = F[:, 0]
+    _, _, lb, ub = O.problem_info(kind, dim)
+    Xo, Fo = O.pso_gen_evolve(kind, dim, X, F, gens, 55, variant=variant, neighb_type=topo, calls=2)
+    py = _PsoPython(55, lb, ub, _py_rosenbrock, variant=variant, topo=topo)
+    Xp, Fp = X, F
+    for _ in range(2):
+        Xp, Fp = py.evolve(Xp, Fp, gens)
+    assert np.array_equal(Xo, Xp)
+    assert np.array_equal(Fo, Fp)
+
+
+@pytest.mark.parametrize("mistake,variant,topo", [("v4_per_element", 4, 2), ("von_transposed", 5, 3),
+                                                  ("fips_no_mean", 6, 2), ("gbest_frozen", 1, 1)])
+def test_pso_gen_variant_loop_detects_mistakes(mistake, variant, topo):
+    """Planted errors in the variant and topology rules of the Python loop
+    change the result, so the comparison above would catch each of them."""
+    kind, dim, pop, gens = O.ROSENBROCK, 3, 12, 3
+    X, F = O.population_init(kind, dim, pop, 21)
+    F = F[:, 0]
+    _, _, lb, ub = O.problem_info(kind, dim)
+    Xo, Fo = O.pso_gen_evolve(kind, dim, X, F, gens, 55, variant=variant, neighb_type=topo, calls=2)
+    py = _PsoPython(55, lb, ub, _py_rosenbrock, variant=variant, topo=topo, mistake=mistake)
+    Xp, Fp = X, F
+    for _ in range(2):
+        Xp, Fp = py.evolve(Xp, Fp, gens)
+    assert not (np.array_equal(Xo, Xp) and np.array_equal(Fo, Fp))
 
 
 @pytest.mark.parametrize("mistake", ["cognitive_x", "best_from_x", "velocity_reset", "first_wins"])
