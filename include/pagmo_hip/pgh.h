@@ -255,8 +255,11 @@ pgh_status pgh_batch_random_dv(const pgh_udp *udp, uint64_t n, uint32_t seed, do
 
 /* ---- pso_gen (src/algorithms/pso_gen.cpp:120-530) on the device.
  *      Parameters and defaults of pso_gen.hpp:109-111.  Supported on the
- *      device: variant 5 with the lbest topology (neighb_type 2); other
- *      combinations return UNSUPPORTED. ------------------------------------ */
+ *      device: variants 1-6 (pso_gen.cpp:241-327) with the gbest (neighb_type
+ *      1), lbest (2) and von Neumann (3) topologies, FIPS (variant 6) with
+ *      lbest and von Neumann only; the adaptive random topology (4), whose
+ *      draws depend on each generation's results, and FIPS with gbest return
+ *      UNSUPPORTED. ------------------------------------------------------- */
 typedef struct pgh_pso_params {
     uint32_t gen;
     uint32_t variant;
