@@ -395,6 +395,16 @@ __device__ __forceinline__ const uint32_t *gen_words(const island_desc &di, cons
     return words + di.woff + half + static_cast<uint64_t>(gen % kGenBatch) * di.wgen + di.wpu * u;
 }
 
+// gen_words for the variants of 4 words per element (1 and 5): the element's
+// r1, r2 words as one aligned 16-byte unit.
+__device__ __forceinline__ const uint4 *gen_words4(const island_desc &di, const uint32_t *__restrict__ words,
+                                                   uint32_t gen, uint64_t m)
+{
+    const uint64_t half = ((gen / kGenBatch) & 1u) ? di.whalf : 0u;
+    return reinterpret_cast<const uint4 *>(words + di.woff + half + static_cast<uint64_t>(gen % kGenBatch) * di.wgen) +
+           m;
+}
+
 // Velocity and position of element m of island di in generation gen.
 __device__ __forceinline__ void update_elem(const island_desc &di, uint64_t m, const uint32_t *__restrict__ words,
                                             double *__restrict__ X, double *__restrict__ V,
@@ -404,9 +414,9 @@ __device__ __forceinline__ void update_elem(const island_desc &di, uint64_t m, c
     const uint32_t p = static_cast<uint32_t>(m / di.nx);
     const uint32_t dd = static_cast<uint32_t>(m - static_cast<uint64_t>(p) * di.nx);
     const uint32_t bn = best_neighbour(p, di.pop, di.radius, lbfit + di.foff);
-    const uint32_t *w = gen_words(di, words, gen, m);
-    const double r1 = mt_canonical(w[0], w[1]);
-    const double r2 = mt_canonical(w[2], w[3]);
+    const uint4 w = *gen_words4(di, words, gen, m);
+    const double r1 = mt_canonical(w.x, w.y);
+    const double r2 = mt_canonical(w.z, w.w);
     const uint64_t xi = di.xoff + m;
     const double best = lbX[di.xoff + static_cast<uint64_t>(bn) * di.nx + dd];
     double v;
@@ -701,11 +711,17 @@ __device__ __noinline__ void step_move_general(const island_desc &di, const tile
 // Moves the rows of item tl (update_elem's arithmetic, the memory read through
 // the selectors).  All threads of the workgroup; a __syncthreads() must
 // follow.
+// GEN: the kernel serves an archipelago with islands of other variants or
+// topologies (a separate instantiation, so that the call of the out-of-line
+// path does not weigh on the register allocation of the default kernel).
+template <bool GEN>
 __device__ __forceinline__ void step_move(const island_desc &di, const tile &tl, const step_args &a, step_rows &rw)
 {
-    if (di.variant != 5u || di.topo != 2u) { // island-uniform
-        step_move_general(di, tl, a, rw);
-        return;
+    if constexpr (GEN) {
+        if (di.variant != 5u || di.topo != 2u) { // island-uniform
+            step_move_general(di, tl, a, rw);
+            return;
+        }
     }
     const double *lbf = a.lbf_cur + di.foff;
     const uint32_t *sel = a.sel_cur + di.foff;
@@ -723,7 +739,7 @@ __device__ __forceinline__ void step_move(const island_desc &di, const tile &tl,
         const uint32_t t = e / di.nx, dd = e - t * di.nx;
         const uint64_t m = e0 + e;
         const uint32_t bn = rw.bn[t], sb = rw.sel[t];
-        const uint4 w = *reinterpret_cast<const uint4 *>(gen_words(di, a.words, a.gen, m));
+        const uint4 w = *gen_words4(di, a.words, a.gen, m);
         const double r1 = mt_canonical(w.x, w.y);
         const double r2 = mt_canonical(w.z, w.w);
         const uint64_t xi = di.xoff + m;
@@ -901,11 +917,11 @@ __device__ __forceinline__ void step_eval_lj(const island_desc &di, const tile &
     }
 }
 
-template <uint32_t CLASS>
+template <uint32_t CLASS, bool GEN>
 __device__ __forceinline__ void step_item(const island_desc &di, const tile &tl, const step_args &a, double *lds,
                                           step_rows &rw)
 {
-    step_move(di, tl, a, rw);
+    step_move<GEN>(di, tl, a, rw);
     __syncthreads(); // the item's new rows are visible to the whole workgroup
     const uint32_t cls = CLASS == kStepAll ? step_class(di.kind) : CLASS;
     if (cls == kStepRosen) {
@@ -946,7 +962,7 @@ constexpr int step_min_waves(uint32_t cls)
     return !PGH_STEP_BOUNDED ? 1 : (cls == kStepRosen || cls == kStepSep ? 8 : (cls == kStepLj ? 4 : 3));
 }
 
-template <uint32_t CLASS>
+template <uint32_t CLASS, bool GEN>
 __global__ __launch_bounds__(kBlock, step_min_waves(CLASS)) void k_step(step_args a)
 {
     extern __shared__ double lds[];
@@ -960,7 +976,7 @@ __global__ __launch_bounds__(kBlock, step_min_waves(CLASS)) void k_step(step_arg
         const uint32_t it = a.order ? a.order[i] : a.base + i;
         const tile tl = a.items[it];
         const island_desc di = a.d[tl.island];
-        if (a.gen < di.gens) step_item<CLASS>(di, tl, a, lds, rw);
+        if (a.gen < di.gens) step_item<CLASS, GEN>(di, tl, a, lds, rw);
         __syncthreads(); // LDS and s_next are reused by the next item
     }
 }
@@ -1228,6 +1244,7 @@ struct pgh_archi {
     // the gbest islands (their indices, in order) and their best positions
     arena_view gblist, gbest;
     std::vector<uint32_t> h_gb;
+    bool general = false; // an island other than variant 5 with lbest: the step kernels with that path
     // k_step: the second memory buffers, the selectors and one queue counter
     // per launch (in the arena); its items (per island, contiguous) and their
     // dispatch orders in a table of their own, rebuilt with the mapping
@@ -1372,10 +1389,14 @@ size_t step_lds(const pgh_archi *a, uint32_t cls);
 // as can be resident, or the mapping's count.
 void step_occupancy(pgh_archi *a)
 {
-    const void *fn[4] = {reinterpret_cast<const void *>(&k_step<kStepRosen>),
-                         reinterpret_cast<const void *>(&k_step<kStepSep>),
-                         reinterpret_cast<const void *>(&k_step<kStepLj>),
-                         reinterpret_cast<const void *>(&k_step<kStepAll>)};
+    const void *fn[4] = {a->general ? reinterpret_cast<const void *>(&k_step<kStepRosen, true>)
+                                    : reinterpret_cast<const void *>(&k_step<kStepRosen, false>),
+                         a->general ? reinterpret_cast<const void *>(&k_step<kStepSep, true>)
+                                    : reinterpret_cast<const void *>(&k_step<kStepSep, false>),
+                         a->general ? reinterpret_cast<const void *>(&k_step<kStepLj, true>)
+                                    : reinterpret_cast<const void *>(&k_step<kStepLj, false>),
+                         a->general ? reinterpret_cast<const void *>(&k_step<kStepAll, true>)
+                                    : reinterpret_cast<const void *>(&k_step<kStepAll, false>)};
     for (uint32_t c = 0; c < 4u; ++c) {
         int per_cu = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn[c], static_cast<int>(kBlock), step_lds(a, c)) !=
@@ -1411,19 +1432,36 @@ pgh_status launch_step(const pgh_archi *a, uint32_t cls, const step_args &args, 
     const size_t lds = step_lds(a, cls);
     const uint32_t grid = std::min<uint32_t>(args.n_items, a->step_per_cu[cls] * a->n_cus);
     if (grid == 0u) return PGH_OK;
-    switch (cls) {
-        case kStepRosen:
-            k_step<kStepRosen><<<grid, kBlock, lds, s>>>(args);
-            break;
-        case kStepSep:
-            k_step<kStepSep><<<grid, kBlock, lds, s>>>(args);
-            break;
-        case kStepLj:
-            k_step<kStepLj><<<grid, kBlock, lds, s>>>(args);
-            break;
-        default:
-            k_step<kStepAll><<<grid, kBlock, lds, s>>>(args);
-            break;
+    if (a->general) {
+        switch (cls) {
+            case kStepRosen:
+                k_step<kStepRosen, true><<<grid, kBlock, lds, s>>>(args);
+                break;
+            case kStepSep:
+                k_step<kStepSep, true><<<grid, kBlock, lds, s>>>(args);
+                break;
+            case kStepLj:
+                k_step<kStepLj, true><<<grid, kBlock, lds, s>>>(args);
+                break;
+            default:
+                k_step<kStepAll, true><<<grid, kBlock, lds, s>>>(args);
+                break;
+        }
+    } else {
+        switch (cls) {
+            case kStepRosen:
+                k_step<kStepRosen, false><<<grid, kBlock, lds, s>>>(args);
+                break;
+            case kStepSep:
+                k_step<kStepSep, false><<<grid, kBlock, lds, s>>>(args);
+                break;
+            case kStepLj:
+                k_step<kStepLj, false><<<grid, kBlock, lds, s>>>(args);
+                break;
+            default:
+                k_step<kStepAll, false><<<grid, kBlock, lds, s>>>(args);
+                break;
+        }
     }
     PGH_HIP(hipGetLastError());
     return PGH_OK;
@@ -1748,6 +1786,7 @@ pgh_status archi_create(pgh_archi **out, const pgh_island_spec *islands, uint64_
         di.memory = q.memory ? 1u : 0u;
         di.variant = q.variant;
         di.topo = q.neighb_type;
+        if (q.variant != 5u || q.neighb_type != 2u) a->general = true;
         di.nbk = q.neighb_type == 3u ? 4u : 2u * radius;
         di.vcols = q.neighb_type == 3u ? von_cols(static_cast<uint32_t>(sp.pop)) : 1u;
         di.wgen = words_per_gen(di.variant, di.nbk, sp.pop, p.nx, di.wpu);
