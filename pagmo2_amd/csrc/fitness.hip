@@ -408,9 +408,12 @@ __global__ __launch_bounds__(kBlock) void k_dtlz(const double *__restrict__ dvs,
 }
 
 // ---- lennard_jones: G individuals per workgroup staged in LDS --------------
+// Up to three owned slots per lane the energy fits 64 VGPRs: 8 wavefronts per
+// SIMD, so that a batch of 16384 individuals of 48 atoms (8 wavefronts per
+// SIMD) is resident at once.
 template <int L, uint32_t APL>
-__global__ __launch_bounds__(kBlock) void k_lj(const double *__restrict__ dvs, uint64_t n, uint32_t atoms,
-                                               uint32_t G, double *__restrict__ fvs)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(APL >= 1 && APL <= 3 ? 8 : 1))) void k_lj(
+    const double *__restrict__ dvs, uint64_t n, uint32_t atoms, uint32_t G, double *__restrict__ fvs)
 {
     extern __shared__ double lds[];
     const uint32_t nx = 3u * atoms - 6u;
@@ -581,7 +584,7 @@ pgh_status launch_lj(const double *dvs, uint64_t n, uint32_t atoms, double *fvs,
         return fail(PGH_UNSUPPORTED, "lennard_jones with " + std::to_string(atoms)
                                          + " atoms does not fit the LDS of one workgroup");
     }
-    switch (lj_lanes(atoms)) {
+    switch (lj_lanes(atoms, grad != nullptr)) {
         case 32:
             return launch_lj_l<32>(dvs, n, atoms, fvs, grad, s);
         case 16:

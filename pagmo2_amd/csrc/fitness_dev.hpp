@@ -95,15 +95,24 @@ __host__ __device__ constexpr int sep_lanes(int kind, uint32_t nx)
     return nx >= 256u ? (kind == ACKLEY ? 32 : 64) : (nx >= 32u ? 16 : 4);
 }
 
+// Lennard-Jones gradients of 22 to 64 atoms visit every pair once, in 16-lane
+// groups (lj_energy_grad_blk); the others visit each pair from both atoms
+// (below 22 atoms most of the blocks' slots would be padding).
+__host__ __device__ constexpr bool lj_grad_blocks(uint32_t atoms)
+{
+    return atoms >= 22u && atoms <= 64u;
+}
+
 // Lennard-Jones: lanes per individual.  At least ~8 pair evaluations per
 // lane, at most 16 lanes owning atoms; from 32 atoms on, 32 lanes: two halves
 // of 16 that own the same atoms and split the schedule's steps between them
 // (twice the wavefronts for the same work, to hide latency when a batch fills
-// the device only once).
-__host__ __device__ constexpr int lj_lanes(uint32_t atoms)
+// the device only once).  Gradients of 22 to 64 atoms: 16 (lj_grad_blocks).
+__host__ __device__ constexpr int lj_lanes(uint32_t atoms, bool grad)
 {
     const uint32_t pairs = atoms * (atoms - 1u) / 2u;
     const uint32_t want = pairs / 8u;
+    if (grad && lj_grad_blocks(atoms)) return 16;
     return atoms >= 32u ? 32 : (want >= 16u ? 16 : (want >= 8u ? 8 : (want >= 4u ? 4 : (want >= 2u ? 2 : 1))));
 }
 
@@ -131,7 +140,7 @@ __host__ __device__ constexpr uint32_t lj_stride(uint32_t atoms, bool grad)
 // of LDS.
 __host__ __device__ constexpr uint32_t lj_group_count(uint32_t atoms, bool grad)
 {
-    const uint32_t L = static_cast<uint32_t>(lj_lanes(atoms));
+    const uint32_t L = static_cast<uint32_t>(lj_lanes(atoms, grad));
     const uint32_t bytes = 8u * lj_stride(atoms, grad);
     uint32_t g = 256u / L;
     while (g > 1u && g * bytes > 65536u) g >>= 1;
@@ -673,8 +682,12 @@ __device__ __forceinline__ void lj_slot(uint32_t e, uint32_t &atom, uint32_t &co
 // Stages `count` individuals (rows of nx = 3*atoms-6) into LDS, lj_stride
 // doubles each: atom a's (x, y, z) at [3a, 3a+3), and for a < lj_copy_atoms
 // again at [3(a+A), 3(a+A)+3), so that a partner a + k is read without a wrap
-// and at a constant offset from atom a.  Called by all threads of the
-// workgroup; a __syncthreads() must follow.
+// and at a constant offset from atom a.  The rows are contiguous in global
+// memory and read as one flat range, kStageLoads loads in flight per thread
+// before any LDS write.  Called by all threads of the workgroup; a
+// __syncthreads() must follow.
+constexpr uint32_t kStageLoads = 4;
+
 __device__ __forceinline__ void lj_stage(const double *__restrict__ rows, uint32_t count, uint32_t atoms, bool grad,
                                          double *__restrict__ lds)
 {
@@ -682,28 +695,43 @@ __device__ __forceinline__ void lj_stage(const double *__restrict__ rows, uint32
     const uint32_t stride = lj_stride(atoms, grad);
     const uint32_t copy = 3u * atoms;
     const uint32_t copied = 3u * lj_copy_atoms(atoms, grad); // flat positions below this are copied
-    // each wavefront (or the partial one of a small workgroup) copies whole
-    // rows, its lanes taking consecutive elements: no element index is divided
-    const uint32_t wave = threadIdx.x / 64u, lane = threadIdx.x % 64u;
-    const uint32_t waves = (blockDim.x + 63u) / 64u;
-    const uint32_t width = min(64u, blockDim.x - 64u * wave);
-    for (uint32_t g = wave; g < count; g += waves) {
+    const uint32_t nt = blockDim.x, tid = threadIdx.x;
+    // the six fixed coordinates of each individual: atom 0 (x,y,z), atom 1
+    // (x,y), atom 2 (x) at flat positions 0..4 and 6
+    for (uint32_t z = tid; z < 6u * count; z += nt) {
+        const uint32_t g = z / 6u, zz = z - 6u * g;
+        const uint32_t f = zz < 5u ? zz : 6u;
         double *ind = lds + g * stride;
-        const double *row = rows + static_cast<uint64_t>(g) * nx;
-        // the six fixed coordinates: atom 0 (x,y,z), atom 1 (x,y), atom 2 (x)
-        // at flat positions 0..4 and 6
-        for (uint32_t z = lane; z < 6u; z += width) {
-            const uint32_t f = z < 5u ? z : 6u;
-            ind[f] = 0.0;
-            if (f < copied) ind[copy + f] = 0.0;
+        ind[f] = 0.0;
+        if (f < copied) ind[copy + f] = 0.0;
+    }
+    // element e of the flat range is element el = e mod nx of row g = e / nx,
+    // at flat position el + 6, except x[0] (atom 1's z) at 5 (lj_slot).  A
+    // thread's e advances by nt: (g, el) by (nt / nx, nt mod nx) and a carry.
+    const uint32_t total = count * nx;
+    const uint32_t dq = nt / nx, dr = nt - dq * nx;
+    uint32_t g = tid / nx, el = tid - g * nx;
+    for (uint32_t base = 0; base < total; base += kStageLoads * nt) {
+        double v[kStageLoads];
+#pragma unroll
+        for (uint32_t u = 0; u < kStageLoads; ++u) {
+            const uint32_t e = base + u * nt + tid;
+            v[u] = e < total ? rows[e] : 0.0;
         }
-        // element e at flat position e + 6, except x[0] (atom 1's z) at 5
-        // (lj_slot)
-        for (uint32_t e = lane; e < nx; e += width) {
-            const uint32_t f = e == 0u ? 5u : e + 6u;
-            const double v = row[e];
-            ind[f] = v;
-            if (f < copied) ind[copy + f] = v;
+#pragma unroll
+        for (uint32_t u = 0; u < kStageLoads; ++u) {
+            if (base + u * nt + tid < total) {
+                const uint32_t f = el == 0u ? 5u : el + 6u;
+                double *ind = lds + g * stride;
+                ind[f] = v[u];
+                if (f < copied) ind[copy + f] = v[u];
+            }
+            g += dq;
+            el += dr;
+            if (el >= nx) {
+                el -= nx;
+                ++g;
+            }
         }
     }
 }
@@ -827,7 +855,8 @@ __device__ __noinline__ double lj_zero_rule(const double *__restrict__ P, uint32
 // the group's lanes share the pairs as in lj_energy.  Two chains, not more:
 // the kernels that call this function reserve its registers as well.
 constexpr uint32_t kLjCarefulChains = 2;
-template <int L>
+// TAG gives a caller its own copy, compiled for that caller's register budget.
+template <int L, uint32_t TAG = 0>
 __device__ __noinline__ double lj_energy_careful(const double *__restrict__ P, uint32_t atoms, int r)
 {
     constexpr int LA = lj_atom_lanes(L);
@@ -975,6 +1004,9 @@ struct lj_slots {
     const double *p[NS];
     double x[NS], y[NS], z[NS], w[NS];
     uint32_t a[NS];
+    // RANGE: a coordinate beyond 2^100 (or NaN) makes the slot's weight NaN,
+    // so that the individual takes the careful path (see lj_recip_group)
+    template <bool RANGE>
     __device__ __forceinline__ void load(const double *__restrict__ P, uint32_t a0, uint32_t r, uint32_t atoms)
     {
 #pragma unroll
@@ -986,6 +1018,10 @@ struct lj_slots {
             y[q] = p[q][1];
             z[q] = p[q][2];
             w[q] = own ? 1.0 : 0.0;
+            if constexpr (RANGE) {
+                const bool in_range = fabs(x[q]) <= 0x1p100 && fabs(y[q]) <= 0x1p100 && fabs(z[q]) <= 0x1p100;
+                if (!in_range) w[q] = __builtin_nan("");
+            }
         }
     }
 };
@@ -1011,27 +1047,106 @@ struct lj_pair {
     }
 };
 
+// 1 / d[q] for N <= 4 values by one reciprocal of their product (Montgomery's
+// batch inversion): one v_rcp_f64 refined by one Newton step as in lj_pair,
+// then 3 (N - 1) products, each adding a rounding (a relative error of a few
+// 1e-16 beyond lj_pair's).  For squared distances in [2^-250, 2^204] every
+// product stays inside [2^-1000, 2^1000], where the reciprocal is accurate.
+// Larger ones need a coordinate beyond 2^100, which the callers test for and
+// send to a careful path; a smaller one makes its own r^-12 term overflow,
+// so the sum is not finite and the individual takes the careful path too.
+template <uint32_t N>
+__device__ __forceinline__ void lj_recip_group(const double *d, double *inv)
+{
+    double p[N];
+    p[0] = d[0];
+#pragma unroll
+    for (uint32_t q = 1; q < N; ++q) p[q] = p[q - 1] * d[q];
+    const double r0 = __builtin_amdgcn_rcp(p[N - 1]);
+    double t = fma(r0, fma(-p[N - 1], r0, 1.0), r0);
+#pragma unroll
+    for (uint32_t q = N - 1; q > 0; --q) {
+        inv[q] = t * p[q - 1];
+        t = t * d[q];
+    }
+    inv[0] = t;
+}
+
+// 1 / d[q] for NS values, in groups of at most four.
+template <uint32_t NS>
+__device__ __forceinline__ void lj_recip_slots(const double (&d)[NS], double (&inv)[NS])
+{
+#pragma unroll
+    for (uint32_t q0 = 0; q0 < NS; q0 += 4u) {
+        if (NS - q0 >= 4u) lj_recip_group<4>(d + q0, inv + q0);
+        else if (NS - q0 == 3u) lj_recip_group<3>(d + q0, inv + q0);
+        else if (NS - q0 == 2u) lj_recip_group<2>(d + q0, inv + q0);
+        else lj_recip_group<1>(d + q0, inv + q0);
+    }
+}
+
+// One step of the cyclic schedule for every owned slot: the pairs (slot q,
+// partner at sl.p[q] + 3 k), their reciprocals taken together
+// (lj_recip_slots), s = r^-6.
 template <int L, uint32_t NS>
+struct lj_visit {
+    double dx[NS], dy[NS], dz[NS], inv[NS], s[NS];
+    __device__ __forceinline__ lj_visit(const lj_slots<L, NS> &sl, uint32_t k)
+    {
+        double r2[NS];
+#pragma unroll
+        for (uint32_t q = 0; q < NS; ++q) {
+            const double *pb = sl.p[q] + 3u * k;
+            dx[q] = sl.x[q] - pb[0];
+            dy[q] = sl.y[q] - pb[1];
+            dz[q] = sl.z[q] - pb[2];
+            r2[q] = fma(dz[q], dz[q], fma(dy[q], dy[q], dx[q] * dx[q]));
+        }
+        lj_recip_slots<NS>(r2, inv);
+#pragma unroll
+        for (uint32_t q = 0; q < NS; ++q) s[q] = (inv[q] * inv[q]) * inv[q];
+    }
+};
+
+// The energy terms of the owned slots over the lane's steps.  BATCH: each
+// step's reciprocals taken together (lj_visit); else one per pair (lj_pair),
+// which holds fewer registers (for callers that are register-bound).
+template <int L, uint32_t NS, bool BATCH>
 __device__ __forceinline__ void lj_energy_slots(const double *__restrict__ P, uint32_t atoms, uint32_t a0, uint32_t r,
                                                 const lj_steps &st, double &acc)
 {
     const lj_cycle cyc(atoms);
     lj_slots<L, NS> sl;
-    sl.load(P, a0, r, atoms);
-    for (uint32_t i = 0u; i + 1u < st.n; ++i) {
-        const uint32_t k = st.first + i * st.stride;
+    sl.template load<BATCH>(P, a0, r, atoms);
+    const bool partial = !st.dummy && st.last == cyc.half && !(atoms & 1u);
+    if constexpr (BATCH) {
+        for (uint32_t i = 0u; i + 1u < st.n; ++i) {
+            const uint32_t k = st.first + i * st.stride;
+            const lj_visit<L, NS> v(sl, k);
+#pragma unroll
+            for (uint32_t q = 0; q < NS; ++q) acc = fma(sl.w[q], fma(v.s[q], v.s[q], -v.s[q]), acc);
+        }
+        const lj_visit<L, NS> v(sl, st.last);
 #pragma unroll
         for (uint32_t q = 0; q < NS; ++q) {
-            const lj_pair v(sl.x[q], sl.y[q], sl.z[q], sl.p[q] + 3u * k);
-            acc = fma(sl.w[q], fma(v.s, v.s, -v.s), acc);
+            const double w = st.dummy || (partial && sl.a[q] >= cyc.half) ? 0.0 : sl.w[q];
+            acc = fma(w, fma(v.s[q], v.s[q], -v.s[q]), acc);
         }
-    }
-    const bool partial = !st.dummy && st.last == cyc.half && !(atoms & 1u);
+    } else {
+        for (uint32_t i = 0u; i + 1u < st.n; ++i) {
+            const uint32_t k = st.first + i * st.stride;
 #pragma unroll
-    for (uint32_t q = 0; q < NS; ++q) {
-        const lj_pair v(sl.x[q], sl.y[q], sl.z[q], sl.p[q] + 3u * st.last);
-        const double w = st.dummy || (partial && sl.a[q] >= cyc.half) ? 0.0 : sl.w[q];
-        acc = fma(w, fma(v.s, v.s, -v.s), acc);
+            for (uint32_t q = 0; q < NS; ++q) {
+                const lj_pair v(sl.x[q], sl.y[q], sl.z[q], sl.p[q] + 3u * k);
+                acc = fma(sl.w[q], fma(v.s, v.s, -v.s), acc);
+            }
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < NS; ++q) {
+            const lj_pair v(sl.x[q], sl.y[q], sl.z[q], sl.p[q] + 3u * st.last);
+            const double w = st.dummy || (partial && sl.a[q] >= cyc.half) ? 0.0 : sl.w[q];
+            acc = fma(w, fma(v.s, v.s, -v.s), acc);
+        }
     }
 }
 
@@ -1041,7 +1156,7 @@ __device__ __forceinline__ void lj_energy_slots(const double *__restrict__ P, ui
 // zero, tiny or NaN distance makes the sum NaN or infinite (see lj_pair), and
 // such an individual goes to lj_energy_careful, which handles the reference's
 // special cases.
-template <int L, uint32_t APL>
+template <int L, uint32_t APL, bool BATCH = true>
 __device__ __forceinline__ double lj_energy(const double *__restrict__ P, uint32_t atoms, int r)
 {
     constexpr int LA = lj_atom_lanes(L);
@@ -1050,14 +1165,14 @@ __device__ __forceinline__ double lj_energy(const double *__restrict__ P, uint32
     const lj_steps st(atoms / 2u, h, KS);
     double acc = 0.;
     if constexpr (APL > 0) {
-        lj_energy_slots<LA, APL>(P, atoms, ra, ra, st, acc);
+        lj_energy_slots<LA, APL, BATCH>(P, atoms, ra, ra, st, acc);
     } else {
         for (uint32_t a0 = ra; a0 < atoms; a0 += 4u * LA) {
-            lj_energy_slots<LA, 4>(P, atoms, a0, ra, st, acc);
+            lj_energy_slots<LA, 4, BATCH>(P, atoms, a0, ra, st, acc);
         }
     }
     acc = group_sum<L>(acc);
-    if (!(acc - acc == 0.0)) return lj_energy_careful<L>(P, atoms, r);
+    if (!(acc - acc == 0.0)) return lj_energy_careful<L, BATCH ? APL : 0u>(P, atoms, r);
     return 4 * acc;
 }
 
@@ -1067,7 +1182,7 @@ __device__ __forceinline__ void lj_grad_slots(const double *__restrict__ P, uint
                                               double *__restrict__ g)
 {
     lj_slots<LA, NS> sl;
-    sl.load(P, a0, r, atoms);
+    sl.template load<false>(P, a0, r, atoms);
     double fx[NS], fy[NS], fz[NS];
 #pragma unroll
     for (uint32_t q = 0; q < NS; ++q) fx[q] = fy[q] = fz[q] = 0.;
@@ -1076,7 +1191,7 @@ __device__ __forceinline__ void lj_grad_slots(const double *__restrict__ P, uint
 #pragma unroll
         for (uint32_t q = 0; q < NS; ++q) {
             const lj_pair v(sl.x[q], sl.y[q], sl.z[q], sl.p[q] + 3u * k);
-            const double coef = (fma(-48., v.s, 24.) * v.s) * v.inv;
+            const double coef = (fma(-2., v.s, 1.) * v.s) * v.inv;
             fx[q] = fma(coef, v.dx, fx[q]);
             fy[q] = fma(coef, v.dy, fy[q]);
             fz[q] = fma(coef, v.dz, fz[q]);
@@ -1090,7 +1205,7 @@ __device__ __forceinline__ void lj_grad_slots(const double *__restrict__ P, uint
 #pragma unroll
         for (uint32_t q = 0; q < NS; ++q) {
             const lj_pair v(sl.x[q], sl.y[q], sl.z[q], sl.p[q] + 3u * st.last);
-            const double coef = ((fma(-48., v.s, 24.) * v.s) * v.inv) * wk;
+            const double coef = ((fma(-2., v.s, 1.) * v.s) * v.inv) * wk;
             fx[q] = fma(coef, v.dx, fx[q]);
             fy[q] = fma(coef, v.dy, fy[q]);
             fz[q] = fma(coef, v.dz, fz[q]);
@@ -1110,23 +1225,203 @@ __device__ __forceinline__ void lj_grad_slots(const double *__restrict__ P, uint
 #pragma unroll
     for (uint32_t q = 0; q < NS; ++q) {
         fsum += fabs(fx[q]) + fabs(fy[q]) + fabs(fz[q]); // inf or NaN if a force is
-        if (sl.w[q] != 0.0) lj_write_grad(sl.a[q], fx[q], fy[q], fz[q], g);
+        if (sl.w[q] != 0.0) lj_write_grad(sl.a[q], 24. * fx[q], 24. * fy[q], 24. * fz[q], g);
     }
+}
+
+// ---- energy and gradient with every pair visited once (22 to 64 atoms) -------
+// A group of 16 lanes.  The A atoms are dealt to 16 blocks, block I holding
+// atoms I + 16 j (j < S = ceil(A / 16)); lane I keeps block I's positions and
+// force accumulators in registers.  Every unordered block pair {I, I + d mod
+// 16} is visited once, by lane I at distance d = 1..8 (at d = 8 only I < 8
+// counts), plus the pairs inside block I.  The partner block's positions come
+// from LDS; its force accumulator travels: at distance d lane I holds the
+// accumulator of block I + d, passes it to lane I - 1 before the next distance
+// (which then holds block I + d + 1's), and after d = 8 it goes home from lane
+// I - 8 to lane I.  Each sum is taken in an order fixed by the schedule alone.
+// A slot past the last atom (only j = S - 1 can be one) holds a distinct point
+// at least 2^12 from every other and weight 0 (PAD).
+constexpr uint32_t kLjBlkRow = 16u;
+
+__device__ __forceinline__ double lj_shfl_row(double v, uint32_t src)
+{
+    return __shfl(v, static_cast<int>(src), static_cast<int>(kLjBlkRow));
+}
+
+template <uint32_t S, bool PAD>
+struct lj_block {
+    double x[S], y[S], z[S];
+    double w = 1.0; // weight of slot S - 1
+    // block B's atoms from the individual staged at P
+    __device__ __forceinline__ void load(const double *__restrict__ P, uint32_t B, uint32_t atoms)
+    {
+#pragma unroll
+        for (uint32_t j = 0; j < S; ++j) {
+            const uint32_t a = B + kLjBlkRow * j;
+            if (!PAD || j + 1u < S || a < atoms) {
+                x[j] = P[3u * a];
+                y[j] = P[3u * a + 1u];
+                z[j] = P[3u * a + 2u];
+            } else {
+                x[j] = 4096.0 * static_cast<double>(1u + a);
+                y[j] = -4096.0;
+                z[j] = 4096.0;
+                w = 0.0;
+            }
+        }
+    }
+};
+
+// Force factor c (dE/dp_a = 24 c (p_a - p_b)) and energy term s^2 - s of one
+// pair, 1/r^2 given.
+__device__ __forceinline__ void lj_blk_terms(double inv, double &c, double &e)
+{
+    const double sv = (inv * inv) * inv;
+    c = (fma(-2., sv, 1.) * sv) * inv;
+    e = fma(sv, sv, -sv);
+}
+
+// The S x S pairs of the own block o (forces f) with the partner block t
+// (forces g); energy terms into acc.
+template <uint32_t S, bool PAD>
+__device__ __forceinline__ void lj_blk_pairs(const lj_block<S, PAD> &o, const lj_block<S, PAD> &t, double (&fx)[S],
+                                             double (&fy)[S], double (&fz)[S], double (&gx)[S], double (&gy)[S],
+                                             double (&gz)[S], double &acc)
+{
+#pragma unroll
+    for (uint32_t j1 = 0; j1 < S; ++j1) {
+        double dx[S], dy[S], dz[S], r2[S], inv[S];
+#pragma unroll
+        for (uint32_t j2 = 0; j2 < S; ++j2) {
+            dx[j2] = o.x[j1] - t.x[j2];
+            dy[j2] = o.y[j1] - t.y[j2];
+            dz[j2] = o.z[j1] - t.z[j2];
+            r2[j2] = fma(dz[j2], dz[j2], fma(dy[j2], dy[j2], dx[j2] * dx[j2]));
+        }
+        lj_recip_group<S>(r2, inv);
+#pragma unroll
+        for (uint32_t j2 = 0; j2 < S; ++j2) {
+            double c, e;
+            lj_blk_terms(inv[j2], c, e);
+            if constexpr (PAD) {
+                if (j1 + 1u == S || j2 + 1u == S) {
+                    const double w = j1 + 1u < S ? t.w : (j2 + 1u < S ? o.w : o.w * t.w);
+                    c *= w;
+                    e *= w;
+                }
+            }
+            fx[j1] = fma(c, dx[j2], fx[j1]);
+            fy[j1] = fma(c, dy[j2], fy[j1]);
+            fz[j1] = fma(c, dz[j2], fz[j1]);
+            gx[j2] = fma(-c, dx[j2], gx[j2]);
+            gy[j2] = fma(-c, dy[j2], gy[j2]);
+            gz[j2] = fma(-c, dz[j2], gz[j2]);
+            acc += e;
+        }
+    }
+}
+
+// Energy and gradient of the individual staged at P (lj_stage with the
+// gradient's layout) by the 16 lanes of a group (lane r); the gradient row is
+// written to g.  A coordinate beyond 2^100, or a non-finite energy or force,
+// sends the individual to lj_energy_grad_careful.
+template <uint32_t S, bool PAD>
+__device__ __forceinline__ double lj_energy_grad_blk(const double *__restrict__ P, uint32_t atoms, int r,
+                                                     double *__restrict__ g)
+{
+    const uint32_t I = static_cast<uint32_t>(r);
+    lj_block<S, PAD> o;
+    o.load(P, I, atoms);
+    bool in_range = true;
+#pragma unroll
+    for (uint32_t j = 0; j < S; ++j)
+        in_range = in_range && fabs(o.x[j]) <= 0x1p100 && fabs(o.y[j]) <= 0x1p100 && fabs(o.z[j]) <= 0x1p100;
+    double fx[S], fy[S], fz[S], gx[S], gy[S], gz[S];
+#pragma unroll
+    for (uint32_t j = 0; j < S; ++j) fx[j] = fy[j] = fz[j] = gx[j] = gy[j] = gz[j] = 0.0;
+    double acc = 0.0;
+    // the pairs inside block I: both forces in the own accumulators
+#pragma unroll
+    for (uint32_t j1 = 0; j1 < S; ++j1) {
+#pragma unroll
+        for (uint32_t j2 = j1 + 1u; j2 < S; ++j2) {
+            const double dx = o.x[j1] - o.x[j2], dy = o.y[j1] - o.y[j2], dz = o.z[j1] - o.z[j2];
+            double r2 = fma(dz, dz, fma(dy, dy, dx * dx)), inv, c, e;
+            lj_recip_group<1>(&r2, &inv);
+            lj_blk_terms(inv, c, e);
+            if constexpr (PAD) {
+                if (j2 + 1u == S) {
+                    c *= o.w;
+                    e *= o.w;
+                }
+            }
+            fx[j1] = fma(c, dx, fx[j1]);
+            fy[j1] = fma(c, dy, fy[j1]);
+            fz[j1] = fma(c, dz, fz[j1]);
+            fx[j2] = fma(-c, dx, fx[j2]);
+            fy[j2] = fma(-c, dy, fy[j2]);
+            fz[j2] = fma(-c, dz, fz[j2]);
+            acc += e;
+        }
+    }
+    const uint32_t next = (I + 1u) % kLjBlkRow;
+#pragma unroll 1
+    for (uint32_t d = 1u; d <= kLjBlkRow / 2u; ++d) {
+        if (d > 1u) {
+#pragma unroll
+            for (uint32_t j = 0; j < S; ++j) {
+                gx[j] = lj_shfl_row(gx[j], next);
+                gy[j] = lj_shfl_row(gy[j], next);
+                gz[j] = lj_shfl_row(gz[j], next);
+            }
+        }
+        if (d < kLjBlkRow / 2u || I < kLjBlkRow / 2u) {
+            lj_block<S, PAD> t;
+            t.load(P, (I + d) % kLjBlkRow, atoms);
+            lj_blk_pairs<S, PAD>(o, t, fx, fy, fz, gx, gy, gz, acc);
+        }
+    }
+    // home: block I's travelling accumulator is at lane I - 8
+    const uint32_t from = (I + kLjBlkRow / 2u) % kLjBlkRow;
+    double fsum = 0.0;
+#pragma unroll
+    for (uint32_t j = 0; j < S; ++j) {
+        const double Fx = fx[j] + lj_shfl_row(gx[j], from);
+        const double Fy = fy[j] + lj_shfl_row(gy[j], from);
+        const double Fz = fz[j] + lj_shfl_row(gz[j], from);
+        fsum += fabs(Fx) + fabs(Fy) + fabs(Fz); // inf or NaN if a force is
+        const uint32_t a = I + kLjBlkRow * j;
+        if (!PAD || a < atoms) lj_write_grad(a, 24. * Fx, 24. * Fy, 24. * Fz, g);
+    }
+    if (!in_range) acc = __builtin_nan("");
+    acc = group_sum<16>(acc);
+    fsum = group_sum<16>(fsum);
+    const bool fast_ok = acc - acc == 0.0 && fsum - fsum == 0.0;
+    if (!fast_ok) return lj_energy_grad_careful<16>(P, atoms, r, g);
+    return 4 * acc;
 }
 
 // Energy and gradient: lane r owns atoms a = r, r+LA, ... (LA =
 // lj_atom_lanes(L); with L = 2 LA the two halves take the steps k = 1..A-1 in
 // two ranges and add their forces) and visits every
 // other atom a + k, k = 1..A-1, accumulating the force
-// dE/dp_a = sum_b (p_a - p_b) (24 s - 48 s^2) / r^2 (s = r^-6; 1/r^2 = s r^4)
-// and the energy terms of all its visits (each pair twice, so the sum is
-// halved).  The gradient row (nx doubles) is written to g; returns the
+// dE/dp_a = 24 sum_b (p_a - p_b) (s - 2 s^2) / r^2 (s = r^-6; the factor 24
+// is applied to the sum, so that the FP64 constants of the loop are inline
+// ones) and the energy terms of all its visits (each pair twice, so the sum
+// is halved).  The gradient row (nx doubles) is written to g; returns the
 // energy.  A non-finite energy or force (a zero, tiny or NaN distance, see
 // lj_pair) sends the individual to lj_energy_grad_careful.
 template <int L, uint32_t APL>
 __device__ __forceinline__ double lj_energy_grad(const double *__restrict__ P, uint32_t atoms, int r,
                                                  double *__restrict__ g)
 {
+    if constexpr (L == 16 && APL >= 2u && APL <= 4u) {
+        // 16-lane groups with up to 4 slots: 17 to 64 atoms (lj_lanes)
+        if (lj_grad_blocks(atoms)) {
+            if (atoms % kLjBlkRow == 0u) return lj_energy_grad_blk<APL, false>(P, atoms, r, g);
+            return lj_energy_grad_blk<APL, true>(P, atoms, r, g);
+        }
+    }
     constexpr int LA = lj_atom_lanes(L);
     constexpr uint32_t KS = static_cast<uint32_t>(L / LA);
     const uint32_t ra = static_cast<uint32_t>(r) % LA, h = static_cast<uint32_t>(r) / LA;

@@ -496,7 +496,7 @@ __device__ __forceinline__ void eval_lj_a(const island_desc &di, const tile &tl,
     const uint32_t g = threadIdx.x / L, r = threadIdx.x % L;
     if (g >= lj_group_count(atoms, false)) return;
     const uint32_t gg = g < tl.count ? g : tl.count - 1u;
-    const double f = lj_energy<L, APL>(lds + gg * lj_stride(atoms, false), atoms, static_cast<int>(r));
+    const double f = lj_energy<L, APL, false>(lds + gg * lj_stride(atoms, false), atoms, static_cast<int>(r));
     if (g < tl.count) remember(di, c, tl.first + gg, f, r, L);
 }
 
@@ -618,7 +618,7 @@ enum : uint32_t { kStepRosen = 0u, kStepSep = 1u, kStepLj = 2u, kStepAll = 3u };
 // cluster's evaluation, which bounds a generation, takes half as many steps.
 __host__ __device__ constexpr uint32_t archi_lj_lanes(uint32_t atoms)
 {
-    return atoms >= 80u ? 64u : static_cast<uint32_t>(lj_lanes(atoms));
+    return atoms >= 80u ? 64u : static_cast<uint32_t>(lj_lanes(atoms, false));
 }
 
 __host__ __device__ constexpr uint32_t step_class(int kind)
@@ -846,7 +846,7 @@ __device__ __forceinline__ void step_eval_lj_a(const island_desc &di, const tile
     __syncthreads();
     const uint32_t g = threadIdx.x / L, r = threadIdx.x % L;
     if (g * L >= kBlock || g >= tl.count) return; // the item's groups
-    const double f = lj_energy<L, APL>(lds + g * lj_stride(atoms, false), atoms, static_cast<int>(r));
+    const double f = lj_energy<L, APL, false>(lds + g * lj_stride(atoms, false), atoms, static_cast<int>(r));
     step_remember(di, a, tl.first + g, f, r, L);
 }
 

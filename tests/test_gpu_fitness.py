@@ -223,7 +223,11 @@ def test_rastrigin_gradient(dim, n):
     assert np.array_equal(f.cpu().numpy(), gpu_bfe(P.RASTRIGIN, dim, x), equal_nan=True)
 
 
-@pytest.mark.parametrize("atoms,n", [(3, 4), (7, 20), (48, 64), (120, 5)])
+# 22 to 64 atoms take the pair-once block kernel (multiples of 16 without
+# padded slots, the others with); the rest visit each pair from both atoms
+@pytest.mark.parametrize("atoms,n", [(3, 4), (7, 20), (16, 9), (17, 40), (21, 9), (22, 30), (24, 30), (31, 20),
+                                     (32, 64), (33, 50), (47, 20), (48, 64), (49, 9), (63, 10), (64, 30), (65, 6),
+                                     (120, 5)])
 def test_lennard_jones_gradient(atoms, n):
     x = inputs(P.LENNARD_JONES, atoms, n, 11 + atoms, lo=-3.0, hi=3.0)
     nx = x.shape[1]
@@ -262,6 +266,43 @@ def test_lennard_jones_gradient_special_distances():
         assert np.linalg.norm(g[i][ok] - ref[i][ok]) <= 1e-12 * max(np.linalg.norm(ref[i][ok]), 1e-300), i
     for i in (2, 3):
         assert abs(f[i] - rf[i]) <= 1e-12 * lj_abs_scale(x[i]), (i, f[i], rf[i])
+
+
+@pytest.mark.parametrize("atoms", [40, 48])
+def test_lennard_jones_gradient_special_distances_blocks(atoms):
+    """The same special rows at sizes of the pair-once kernel, plus an atom at
+    1e25 (inside the range of its batched reciprocals) and one at 1e40 (beyond
+    2^100: the careful path)."""
+    n = 6
+    x = inputs(P.LENNARD_JONES, atoms, n, 7 + atoms, lo=-3.0, hi=3.0)
+
+    def at(a):  # flat elements of atom a >= 3
+        return slice(3 * a - 6, 3 * a - 3)
+
+    x[0, at(3)] = 0.0                 # atom 3 on atom 0
+    x[1, at(20)] = x[1, at(21)]       # atoms 20 and 21 coincide
+    x[2, at(atoms - 1)] = 1.0e120     # the last atom far away
+    x[3, at(10)] = 1.0e25
+    x[4, at(17)] = 1.0e40
+    g = torch.empty((n, x.shape[1]), dtype=torch.float64, device=DEV)
+    f = torch.empty((n, 1), dtype=torch.float64, device=DEV)
+    P.batch_gradient(P.LENNARD_JONES, atoms, dev(x), g, f)
+    torch.cuda.synchronize()
+    g, f = g.cpu().numpy(), f.cpu().numpy()[:, 0]
+    ref = O.gradient(P.LENNARD_JONES, atoms, x)
+    rf = O.bfe(P.LENNARD_JONES, atoms, x)[:, 0]
+    assert f[0] == math.inf and f[1] == math.inf
+    for i in range(n):
+        assert np.array_equal(np.isnan(g[i]), np.isnan(ref[i])), i
+        ok = ~np.isnan(ref[i])
+        assert np.linalg.norm(g[i][ok] - ref[i][ok]) <= 1e-12 * max(np.linalg.norm(ref[i][ok]), 1e-300), i
+    for i in range(2, n):
+        assert abs(f[i] - rf[i]) <= 1e-12 * lj_abs_scale(x[i]), (i, f[i], rf[i])
+    # the energy kernel on the same rows (its reciprocals are batched too)
+    fb = gpu_bfe(P.LENNARD_JONES, atoms, x)[:, 0]
+    assert fb[0] == math.inf and fb[1] == math.inf
+    for i in range(2, n):
+        assert abs(fb[i] - rf[i]) <= 1e-12 * lj_abs_scale(x[i]), (i, fb[i], rf[i])
 
 
 def test_lennard_jones_gradient_finite_differences():
