@@ -44,12 +44,14 @@ def lj_run(atoms, x, grad):
 @pytest.fixture
 def lj_defaults():
     yield
-    for atoms in (5, 13, 20, 48):
+    for atoms in (5, 13, 20, 37, 48):
         for grad in (0, 1):
             P.lj_set_groups(atoms, grad, 0)
 
 
-@pytest.mark.parametrize("atoms,n", [(5, 301), (13, 1000), (48, 2049)])
+# 37 and 48 atoms: the gradient's pair-once kernel, with and without padded
+# block slots
+@pytest.mark.parametrize("atoms,n", [(5, 301), (13, 1000), (37, 515), (48, 2049)])
 @pytest.mark.parametrize("grad", [False, True])
 def test_lj_groups_change_no_result(atoms, n, grad, lj_defaults):
     x = dev(lj_inputs(atoms, n, 3 + atoms))
