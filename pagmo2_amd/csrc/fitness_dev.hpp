@@ -1239,8 +1239,10 @@ __device__ __forceinline__ void lj_grad_slots(const double *__restrict__ P, uint
 // accumulator of block I + d, passes it to lane I - 1 before the next distance
 // (which then holds block I + d + 1's), and after d = 8 it goes home from lane
 // I - 8 to lane I.  Each sum is taken in an order fixed by the schedule alone.
-// A slot past the last atom (only j = S - 1 can be one) holds a distinct point
-// at least 2^12 from every other and weight 0 (PAD).
+// A slot past the last atom (only j = S - 1 can be one) holds weight 0 and a
+// point of its own, 2^12 apart from the other such points and far outside the
+// problem's bounds (PAD); a real atom placed exactly on one makes the sum NaN,
+// which sends the individual to the careful path.
 constexpr uint32_t kLjBlkRow = 16u;
 
 __device__ __forceinline__ double lj_shfl_row(double v, uint32_t src)
