@@ -95,24 +95,27 @@ __host__ __device__ constexpr int sep_lanes(int kind, uint32_t nx)
     return nx >= 256u ? (kind == ACKLEY ? 32 : 64) : (nx >= 32u ? 16 : 4);
 }
 
-// Lennard-Jones gradients of 22 to 64 atoms visit every pair once, in 16-lane
-// groups (lj_energy_grad_blk); the others visit each pair from both atoms
-// (below 22 atoms most of the blocks' slots would be padding).
+// Lennard-Jones gradients of 22 to 128 atoms visit every pair once, in
+// 16-lane groups up to 64 atoms and 32-lane groups beyond
+// (lj_energy_grad_blk); the others visit each pair from both atoms (below 22
+// atoms most of the blocks' slots would be padding, above 128 the blocks no
+// longer fit the registers).
 __host__ __device__ constexpr bool lj_grad_blocks(uint32_t atoms)
 {
-    return atoms >= 22u && atoms <= 64u;
+    return atoms >= 22u && atoms <= 128u;
 }
 
 // Lennard-Jones: lanes per individual.  At least ~8 pair evaluations per
 // lane, at most 16 lanes owning atoms; from 32 atoms on, 32 lanes: two halves
 // of 16 that own the same atoms and split the schedule's steps between them
 // (twice the wavefronts for the same work, to hide latency when a batch fills
-// the device only once).  Gradients of 22 to 64 atoms: 16 (lj_grad_blocks).
+// the device only once).  Gradients of 22 to 64 atoms: 16, of 65 to 128: 32
+// (lj_grad_blocks).
 __host__ __device__ constexpr int lj_lanes(uint32_t atoms, bool grad)
 {
     const uint32_t pairs = atoms * (atoms - 1u) / 2u;
     const uint32_t want = pairs / 8u;
-    if (grad && lj_grad_blocks(atoms)) return 16;
+    if (grad && lj_grad_blocks(atoms)) return atoms <= 64u ? 16 : 32;
     return atoms >= 32u ? 32 : (want >= 16u ? 16 : (want >= 8u ? 8 : (want >= 4u ? 4 : (want >= 2u ? 2 : 1))));
 }
 
@@ -1229,28 +1232,28 @@ __device__ __forceinline__ void lj_grad_slots(const double *__restrict__ P, uint
     }
 }
 
-// ---- energy and gradient with every pair visited once (22 to 64 atoms) -------
-// A group of 16 lanes.  The A atoms are dealt to 16 blocks, block I holding
-// atoms I + 16 j (j < S = ceil(A / 16)); lane I keeps block I's positions and
-// force accumulators in registers.  Every unordered block pair {I, I + d mod
-// 16} is visited once, by lane I at distance d = 1..8 (at d = 8 only I < 8
-// counts), plus the pairs inside block I.  The partner block's positions come
-// from LDS; its force accumulator travels: at distance d lane I holds the
-// accumulator of block I + d, passes it to lane I - 1 before the next distance
-// (which then holds block I + d + 1's), and after d = 8 it goes home from lane
-// I - 8 to lane I.  Each sum is taken in an order fixed by the schedule alone.
+// ---- energy and gradient with every pair visited once (22 to 128 atoms) ------
+// A group of R lanes (R = 16 up to 64 atoms, 32 up to 128).  The A atoms are
+// dealt to R blocks, block I holding atoms I + R j (j < S = ceil(A / R) <= 4);
+// lane I keeps block I's positions and force accumulators in registers.  Every
+// unordered block pair {I, I + d mod R} is visited once, by lane I at distance
+// d = 1..R/2 (at d = R/2 only I < R/2 counts), plus the pairs inside block I.
+// The partner block's positions come from LDS; its force accumulator travels:
+// at distance d lane I holds the accumulator of block I + d, passes it to lane
+// I - 1 before the next distance (which then holds block I + d + 1's), and
+// after d = R/2 it goes home from lane I - R/2 to lane I.  Each sum is taken in
+// an order fixed by the schedule alone.
 // A slot past the last atom (only j = S - 1 can be one) holds weight 0 and a
 // point of its own, 2^12 apart from the other such points and far outside the
 // problem's bounds (PAD); a real atom placed exactly on one makes the sum NaN,
 // which sends the individual to the careful path.
-constexpr uint32_t kLjBlkRow = 16u;
-
+template <uint32_t R>
 __device__ __forceinline__ double lj_shfl_row(double v, uint32_t src)
 {
-    return __shfl(v, static_cast<int>(src), static_cast<int>(kLjBlkRow));
+    return __shfl(v, static_cast<int>(src), static_cast<int>(R));
 }
 
-template <uint32_t S, bool PAD>
+template <uint32_t R, uint32_t S, bool PAD>
 struct lj_block {
     double x[S], y[S], z[S];
     double w = 1.0; // weight of slot S - 1
@@ -1259,7 +1262,7 @@ struct lj_block {
     {
 #pragma unroll
         for (uint32_t j = 0; j < S; ++j) {
-            const uint32_t a = B + kLjBlkRow * j;
+            const uint32_t a = B + R * j;
             if (!PAD || j + 1u < S || a < atoms) {
                 x[j] = P[3u * a];
                 y[j] = P[3u * a + 1u];
@@ -1285,8 +1288,8 @@ __device__ __forceinline__ void lj_blk_terms(double inv, double &c, double &e)
 
 // The S x S pairs of the own block o (forces f) with the partner block t
 // (forces g); energy terms into acc.
-template <uint32_t S, bool PAD>
-__device__ __forceinline__ void lj_blk_pairs(const lj_block<S, PAD> &o, const lj_block<S, PAD> &t, double (&fx)[S],
+template <uint32_t R, uint32_t S, bool PAD>
+__device__ __forceinline__ void lj_blk_pairs(const lj_block<R, S, PAD> &o, const lj_block<R, S, PAD> &t, double (&fx)[S],
                                              double (&fy)[S], double (&fz)[S], double (&gx)[S], double (&gy)[S],
                                              double (&gz)[S], double &acc)
 {
@@ -1324,15 +1327,15 @@ __device__ __forceinline__ void lj_blk_pairs(const lj_block<S, PAD> &o, const lj
 }
 
 // Energy and gradient of the individual staged at P (lj_stage with the
-// gradient's layout) by the 16 lanes of a group (lane r); the gradient row is
+// gradient's layout) by the R lanes of a group (lane r); the gradient row is
 // written to g.  A coordinate beyond 2^100, or a non-finite energy or force,
 // sends the individual to lj_energy_grad_careful.
-template <uint32_t S, bool PAD>
+template <uint32_t R, uint32_t S, bool PAD>
 __device__ __forceinline__ double lj_energy_grad_blk(const double *__restrict__ P, uint32_t atoms, int r,
                                                      double *__restrict__ g)
 {
     const uint32_t I = static_cast<uint32_t>(r);
-    lj_block<S, PAD> o;
+    lj_block<R, S, PAD> o;
     o.load(P, I, atoms);
     bool in_range = true;
 #pragma unroll
@@ -1366,40 +1369,40 @@ __device__ __forceinline__ double lj_energy_grad_blk(const double *__restrict__ 
             acc += e;
         }
     }
-    const uint32_t next = (I + 1u) % kLjBlkRow;
+    const uint32_t next = (I + 1u) % R;
 #pragma unroll 1
-    for (uint32_t d = 1u; d <= kLjBlkRow / 2u; ++d) {
+    for (uint32_t d = 1u; d <= R / 2u; ++d) {
         if (d > 1u) {
 #pragma unroll
             for (uint32_t j = 0; j < S; ++j) {
-                gx[j] = lj_shfl_row(gx[j], next);
-                gy[j] = lj_shfl_row(gy[j], next);
-                gz[j] = lj_shfl_row(gz[j], next);
+                gx[j] = lj_shfl_row<R>(gx[j], next);
+                gy[j] = lj_shfl_row<R>(gy[j], next);
+                gz[j] = lj_shfl_row<R>(gz[j], next);
             }
         }
-        if (d < kLjBlkRow / 2u || I < kLjBlkRow / 2u) {
-            lj_block<S, PAD> t;
-            t.load(P, (I + d) % kLjBlkRow, atoms);
-            lj_blk_pairs<S, PAD>(o, t, fx, fy, fz, gx, gy, gz, acc);
+        if (d < R / 2u || I < R / 2u) {
+            lj_block<R, S, PAD> t;
+            t.load(P, (I + d) % R, atoms);
+            lj_blk_pairs<R, S, PAD>(o, t, fx, fy, fz, gx, gy, gz, acc);
         }
     }
-    // home: block I's travelling accumulator is at lane I - 8
-    const uint32_t from = (I + kLjBlkRow / 2u) % kLjBlkRow;
+    // home: block I's travelling accumulator is at lane I - R/2
+    const uint32_t from = (I + R / 2u) % R;
     double fsum = 0.0;
 #pragma unroll
     for (uint32_t j = 0; j < S; ++j) {
-        const double Fx = fx[j] + lj_shfl_row(gx[j], from);
-        const double Fy = fy[j] + lj_shfl_row(gy[j], from);
-        const double Fz = fz[j] + lj_shfl_row(gz[j], from);
+        const double Fx = fx[j] + lj_shfl_row<R>(gx[j], from);
+        const double Fy = fy[j] + lj_shfl_row<R>(gy[j], from);
+        const double Fz = fz[j] + lj_shfl_row<R>(gz[j], from);
         fsum += fabs(Fx) + fabs(Fy) + fabs(Fz); // inf or NaN if a force is
-        const uint32_t a = I + kLjBlkRow * j;
+        const uint32_t a = I + R * j;
         if (!PAD || a < atoms) lj_write_grad(a, 24. * Fx, 24. * Fy, 24. * Fz, g);
     }
     if (!in_range) acc = __builtin_nan("");
-    acc = group_sum<16>(acc);
-    fsum = group_sum<16>(fsum);
+    acc = group_sum<static_cast<int>(R)>(acc);
+    fsum = group_sum<static_cast<int>(R)>(fsum);
     const bool fast_ok = acc - acc == 0.0 && fsum - fsum == 0.0;
-    if (!fast_ok) return lj_energy_grad_careful<16>(P, atoms, r, g);
+    if (!fast_ok) return lj_energy_grad_careful<static_cast<int>(R)>(P, atoms, r, g);
     return 4 * acc;
 }
 
@@ -1418,10 +1421,18 @@ __device__ __forceinline__ double lj_energy_grad(const double *__restrict__ P, u
                                                  double *__restrict__ g)
 {
     if constexpr (L == 16 && APL >= 2u && APL <= 4u) {
-        // 16-lane groups with up to 4 slots: 17 to 64 atoms (lj_lanes)
+        // 16-lane groups, 17 to 64 atoms (lj_lanes): blocks of APL atoms
         if (lj_grad_blocks(atoms)) {
-            if (atoms % kLjBlkRow == 0u) return lj_energy_grad_blk<APL, false>(P, atoms, r, g);
-            return lj_energy_grad_blk<APL, true>(P, atoms, r, g);
+            if (atoms % 16u == 0u) return lj_energy_grad_blk<16u, APL, false>(P, atoms, r, g);
+            return lj_energy_grad_blk<16u, APL, true>(P, atoms, r, g);
+        }
+    }
+    if constexpr (L == 32 && APL >= 5u && APL <= 8u) {
+        // 32-lane groups, 65 to 128 atoms: blocks of ceil(A / 32) atoms
+        constexpr uint32_t S = (APL + 1u) / 2u;
+        if (lj_grad_blocks(atoms)) {
+            if (atoms % 32u == 0u) return lj_energy_grad_blk<32u, S, false>(P, atoms, r, g);
+            return lj_energy_grad_blk<32u, S, true>(P, atoms, r, g);
         }
     }
     constexpr int LA = lj_atom_lanes(L);

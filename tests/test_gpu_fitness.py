@@ -223,11 +223,12 @@ def test_rastrigin_gradient(dim, n):
     assert np.array_equal(f.cpu().numpy(), gpu_bfe(P.RASTRIGIN, dim, x), equal_nan=True)
 
 
-# 22 to 64 atoms take the pair-once block kernel (multiples of 16 without
-# padded slots, the others with); the rest visit each pair from both atoms
+# 22 to 128 atoms take the pair-once block kernel (16-lane rows up to 64
+# atoms, 32-lane rows beyond; multiples of the row without padded slots, the
+# others with); the rest visit each pair from both atoms
 @pytest.mark.parametrize("atoms,n", [(3, 4), (7, 20), (16, 9), (17, 40), (21, 9), (22, 30), (24, 30), (31, 20),
                                      (32, 64), (33, 50), (47, 20), (48, 64), (49, 9), (63, 10), (64, 30), (65, 6),
-                                     (120, 5)])
+                                     (80, 9), (96, 7), (97, 5), (120, 5), (128, 6), (129, 3), (150, 2)])
 def test_lennard_jones_gradient(atoms, n):
     x = inputs(P.LENNARD_JONES, atoms, n, 11 + atoms, lo=-3.0, hi=3.0)
     nx = x.shape[1]
@@ -268,7 +269,7 @@ def test_lennard_jones_gradient_special_distances():
         assert abs(f[i] - rf[i]) <= 1e-12 * lj_abs_scale(x[i]), (i, f[i], rf[i])
 
 
-@pytest.mark.parametrize("atoms", [40, 48])
+@pytest.mark.parametrize("atoms", [40, 48, 80])
 def test_lennard_jones_gradient_special_distances_blocks(atoms):
     """The same special rows at sizes of the pair-once kernel, plus an atom at
     1e25 (inside the range of its batched reciprocals) and one at 1e40 (beyond
