@@ -105,18 +105,33 @@ __host__ __device__ constexpr bool lj_grad_blocks(uint32_t atoms)
     return atoms >= 22u && atoms <= 128u;
 }
 
+// The energy takes the same blocks (lj_energy_blk) only where at most 1/16 of
+// the block slots are padding; otherwise its cyclic schedule wastes less.
+__host__ __device__ constexpr bool lj_energy_blocks(uint32_t atoms)
+{
+    const uint32_t R = atoms <= 64u ? 16u : 32u;
+    const uint32_t slots = R * ((atoms + R - 1u) / R);
+    return lj_grad_blocks(atoms) && 16u * (slots - atoms) <= slots;
+}
+
 // Lennard-Jones: lanes per individual.  At least ~8 pair evaluations per
 // lane, at most 16 lanes owning atoms; from 32 atoms on, 32 lanes: two halves
 // of 16 that own the same atoms and split the schedule's steps between them
 // (twice the wavefronts for the same work, to hide latency when a batch fills
-// the device only once).  Gradients of 22 to 64 atoms: 16, of 65 to 128: 32
-// (lj_grad_blocks).
-__host__ __device__ constexpr int lj_lanes(uint32_t atoms, bool grad)
+// the device only once) (lj_lanes_cyclic).  The pair-once blocks of 22 to 64
+// atoms take 16 lanes, of 65 to 128 atoms 32 (lj_grad_blocks,
+// lj_energy_blocks).
+__host__ __device__ constexpr int lj_lanes_cyclic(uint32_t atoms)
 {
     const uint32_t pairs = atoms * (atoms - 1u) / 2u;
     const uint32_t want = pairs / 8u;
-    if (grad && lj_grad_blocks(atoms)) return atoms <= 64u ? 16 : 32;
     return atoms >= 32u ? 32 : (want >= 16u ? 16 : (want >= 8u ? 8 : (want >= 4u ? 4 : (want >= 2u ? 2 : 1))));
+}
+
+__host__ __device__ constexpr int lj_lanes(uint32_t atoms, bool grad)
+{
+    if (grad ? lj_grad_blocks(atoms) : lj_energy_blocks(atoms)) return atoms <= 64u ? 16 : 32;
+    return lj_lanes_cyclic(atoms);
 }
 
 // Lanes of a group that own atoms (the rest repeat them for other steps).
@@ -1159,9 +1174,23 @@ __device__ __forceinline__ void lj_energy_slots(const double *__restrict__ P, ui
 // zero, tiny or NaN distance makes the sum NaN or infinite (see lj_pair), and
 // such an individual goes to lj_energy_careful, which handles the reference's
 // special cases.
+template <uint32_t R, uint32_t S, bool PAD>
+__device__ __forceinline__ double lj_energy_blk_sum(const double *__restrict__ P, uint32_t atoms, int r);
+
 template <int L, uint32_t APL, bool BATCH = true>
 __device__ __forceinline__ double lj_energy(const double *__restrict__ P, uint32_t atoms, int r)
 {
+    if constexpr (BATCH && ((L == 16 && APL >= 2u && APL <= 4u) || (L == 32 && APL >= 5u && APL <= 8u))) {
+        // the pair-once blocks (lj_lanes): 16 lanes and blocks of APL atoms, or
+        // 32 lanes and blocks of ceil(APL / 2)
+        if (lj_energy_blocks(atoms)) {
+            constexpr uint32_t R = static_cast<uint32_t>(L), S = L == 16 ? APL : (APL + 1u) / 2u;
+            const double acc = atoms % R == 0u ? lj_energy_blk_sum<R, S, false>(P, atoms, r)
+                                               : lj_energy_blk_sum<R, S, true>(P, atoms, r);
+            if (!(acc - acc == 0.0)) return lj_energy_careful<L, APL>(P, atoms, r);
+            return 4 * acc;
+        }
+    }
     constexpr int LA = lj_atom_lanes(L);
     constexpr uint32_t KS = static_cast<uint32_t>(L / LA);
     const uint32_t ra = static_cast<uint32_t>(r) % LA, h = static_cast<uint32_t>(r) / LA;
@@ -1404,6 +1433,64 @@ __device__ __forceinline__ double lj_energy_grad_blk(const double *__restrict__ 
     const bool fast_ok = acc - acc == 0.0 && fsum - fsum == 0.0;
     if (!fast_ok) return lj_energy_grad_careful<static_cast<int>(R)>(P, atoms, r, g);
     return 4 * acc;
+}
+
+// Energy alone on the blocks of lj_energy_grad_blk (no forces, nothing
+// travels): the group's sum of s^2 - s over every pair, NaN if a coordinate
+// lies beyond 2^100.  The caller tests it and takes the careful path.
+template <uint32_t R, uint32_t S, bool PAD>
+__device__ __forceinline__ double lj_energy_blk_sum(const double *__restrict__ P, uint32_t atoms, int r)
+{
+    const uint32_t I = static_cast<uint32_t>(r);
+    lj_block<R, S, PAD> o;
+    o.load(P, I, atoms);
+    bool in_range = true;
+#pragma unroll
+    for (uint32_t j = 0; j < S; ++j)
+        in_range = in_range && fabs(o.x[j]) <= 0x1p100 && fabs(o.y[j]) <= 0x1p100 && fabs(o.z[j]) <= 0x1p100;
+    double acc = 0.0;
+#pragma unroll
+    for (uint32_t j1 = 0; j1 < S; ++j1) {
+#pragma unroll
+        for (uint32_t j2 = j1 + 1u; j2 < S; ++j2) {
+            const double dx = o.x[j1] - o.x[j2], dy = o.y[j1] - o.y[j2], dz = o.z[j1] - o.z[j2];
+            double r2 = fma(dz, dz, fma(dy, dy, dx * dx)), inv, c, e;
+            lj_recip_group<1>(&r2, &inv);
+            lj_blk_terms(inv, c, e);
+            if constexpr (PAD) {
+                if (j2 + 1u == S) e *= o.w;
+            }
+            acc += e;
+        }
+    }
+#pragma unroll 1
+    for (uint32_t d = 1u; d <= R / 2u; ++d) {
+        if (d < R / 2u || I < R / 2u) {
+            lj_block<R, S, PAD> t;
+            t.load(P, (I + d) % R, atoms);
+#pragma unroll
+            for (uint32_t j1 = 0; j1 < S; ++j1) {
+                double r2[S], inv[S];
+#pragma unroll
+                for (uint32_t j2 = 0; j2 < S; ++j2) {
+                    const double dx = o.x[j1] - t.x[j2], dy = o.y[j1] - t.y[j2], dz = o.z[j1] - t.z[j2];
+                    r2[j2] = fma(dz, dz, fma(dy, dy, dx * dx));
+                }
+                lj_recip_group<S>(r2, inv);
+#pragma unroll
+                for (uint32_t j2 = 0; j2 < S; ++j2) {
+                    const double sv = (inv[j2] * inv[j2]) * inv[j2];
+                    double e = fma(sv, sv, -sv);
+                    if constexpr (PAD) {
+                        if (j1 + 1u == S || j2 + 1u == S) e *= j1 + 1u < S ? t.w : (j2 + 1u < S ? o.w : o.w * t.w);
+                    }
+                    acc += e;
+                }
+            }
+        }
+    }
+    if (!in_range) acc = __builtin_nan("");
+    return group_sum<static_cast<int>(R)>(acc);
 }
 
 // Energy and gradient: lane r owns atoms a = r, r+LA, ... (LA =

@@ -613,12 +613,13 @@ __device__ __forceinline__ void eval_tile(const island_desc &di, const tile &tl,
 enum : uint32_t { kStepRosen = 0u, kStepSep = 1u, kStepLj = 2u, kStepAll = 3u };
 
 // Lanes per Lennard-Jones individual in an archipelago: the batch evaluator's
-// (lj_lanes), or a whole wavefront from 80 atoms on (16 lanes own the atoms,
-// four quarters of them split the schedule's steps), so that one large
-// cluster's evaluation, which bounds a generation, takes half as many steps.
+// for its cyclic schedule (lj_lanes_cyclic), or a whole wavefront from 80
+// atoms on (16 lanes own the atoms, four quarters of them split the
+// schedule's steps), so that one large cluster's evaluation, which bounds a
+// generation, takes half as many steps.
 __host__ __device__ constexpr uint32_t archi_lj_lanes(uint32_t atoms)
 {
-    return atoms >= 80u ? 64u : static_cast<uint32_t>(lj_lanes(atoms, false));
+    return atoms >= 80u ? 64u : static_cast<uint32_t>(lj_lanes_cyclic(atoms));
 }
 
 __host__ __device__ constexpr uint32_t step_class(int kind)

@@ -102,7 +102,11 @@ def test_separable_unaligned_rows_same_order():
     assert np.array_equal(out.cpu().numpy(), gpu_bfe(P.RASTRIGIN, 1000, x))
 
 
-@pytest.mark.parametrize("atoms,n", [(3, 5), (4, 33), (5, 17), (9, 100), (13, 7), (48, 300), (100, 20), (169, 9)])
+# 30-32, 45-48, 60-64, 90-96 and 120-128 atoms: the energy on pair-once
+# blocks; the other sizes on the cyclic schedule
+@pytest.mark.parametrize("atoms,n", [(3, 5), (4, 33), (5, 17), (9, 100), (13, 7), (30, 40), (32, 33), (40, 20),
+                                     (45, 30), (48, 300), (60, 9), (64, 20), (90, 7), (96, 20), (100, 20),
+                                     (120, 5), (128, 9), (169, 9)])
 def test_lennard_jones(atoms, n):
     x = inputs(P.LENNARD_JONES, atoms, n, atoms, lo=-3.0, hi=3.0)
     g = gpu_bfe(P.LENNARD_JONES, atoms, x)[:, 0]

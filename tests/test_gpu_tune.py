@@ -82,9 +82,12 @@ def test_lj_groups_change_no_result(atoms, n, grad, lj_defaults):
 
 
 def test_lj_set_groups_limits(lj_defaults):
-    # 48 atoms take 32 lanes each: 8 individuals fill a 256-lane workgroup
-    with pytest.raises(P.PghError, match="at most 8 individuals per workgroup for 48 atoms"):
-        P.lj_set_groups(48, 0, 9)
+    # 48 atoms take 16 lanes each (pair-once blocks): 16 individuals fill a
+    # 256-lane workgroup; 40 atoms take the cyclic schedule's 32 lanes: 8
+    with pytest.raises(P.PghError, match="at most 16 individuals per workgroup for 48 atoms"):
+        P.lj_set_groups(48, 0, 17)
+    with pytest.raises(P.PghError, match="at most 8 individuals per workgroup for 40 atoms"):
+        P.lj_set_groups(40, 0, 9)
     with pytest.raises(P.PghError, match="atoms out of range"):
         P.lj_set_groups(2, 0, 1)
     x = dev(lj_inputs(13, 4, 1))
