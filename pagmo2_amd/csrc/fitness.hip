@@ -409,8 +409,8 @@ __global__ __launch_bounds__(kBlock) void k_dtlz(const double *__restrict__ dvs,
 
 // ---- lennard_jones: G individuals per workgroup staged in LDS --------------
 // Up to three owned slots per lane the energy fits 64 VGPRs: 8 wavefronts per
-// SIMD, so that a batch of 16384 individuals of 48 atoms (8 wavefronts per
-// SIMD) is resident at once.
+// SIMD, so that a batch of 16384 individuals on 32-lane groups (8 wavefronts
+// per SIMD) is resident at once.
 template <int L, uint32_t APL>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(APL >= 1 && APL <= 3 ? 8 : 1))) void k_lj(
     const double *__restrict__ dvs, uint64_t n, uint32_t atoms, uint32_t G, double *__restrict__ fvs)
