@@ -366,6 +366,13 @@ inline void prefetch_jump_polys(const std::vector<uint64_t> &Js, unsigned thread
 inline void prefetch_multiples(uint64_t S, uint64_t count)
 {
     if (count == 0u || S == 0u) return;
+    {
+        // made by an earlier call (the multiples come in order, the last one
+        // is entered with the rest)
+        poly_cache &c = the_cache();
+        std::lock_guard<std::mutex> lk(c.mu);
+        if (count == 1u ? c.map.count(S) != 0u : c.map.count(count * S) != 0u) return;
+    }
     const field &f = the_field();
     const poly step = jump_poly(S);
     poly cur = step;

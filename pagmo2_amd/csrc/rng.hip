@@ -28,8 +28,8 @@ __global__ __launch_bounds__(kMtBlock) void k_mt19937(uint32_t seed, uint64_t sk
 
 // Segment blockIdx.x = s of a stream split in segments of B words: words
 // [s B, min((s + 1) B, n)) of std::mt19937(seed), written to out at their
-// stream positions.  The segment's generator reaches s B by the jumps
-// z^(B 2^b) (table b of the jump table) over the set bits b of s.
+// stream positions.  The segment's generator reaches s B by one jump,
+// z^(s B) (entry s - 1 of the jump table).
 __global__ __launch_bounds__(kMtJumpBlock) void k_mt19937_seg(uint32_t seed, uint64_t n, uint64_t B, uint32_t nd,
                                                                const void *__restrict__ table,
                                                                uint32_t *__restrict__ out)
@@ -41,8 +41,9 @@ __global__ __launch_bounds__(kMtJumpBlock) void k_mt19937_seg(uint32_t seed, uin
     const uint32_t s = blockIdx.x;
     const uint32_t *lens = static_cast<const uint32_t *>(table);
     const uint16_t *lists = reinterpret_cast<const uint16_t *>(static_cast<const char *>(table) + jump_table_head(nd));
-    for (uint32_t b = 0; (s >> b) != 0u; ++b) {
-        if ((s >> b) & 1u) mt_jump(ring, K, C, B << b, lists + static_cast<uint64_t>(b) * kMtIdxStride, lens[b], lds_idx);
+    if (s > 0u) {
+        mt_jump(ring, K, C, static_cast<uint64_t>(s) * B, lists + static_cast<uint64_t>(s - 1u) * kMtIdxStride,
+                lens[s - 1u], lds_idx);
     }
     const uint64_t begin = static_cast<uint64_t>(s) * B;
     mt_run(ring, K, C, min(B, n - begin), out + begin);
@@ -64,7 +65,9 @@ __global__ __launch_bounds__(256) void k_uniform_inplace(double *__restrict__ io
 
 // The first n words of std::mt19937(seed) into out: one workgroup up to
 // kMtSplitMin words, else up to kMtMaxSegs workgroups, each starting its
-// segment by jump-ahead.
+// segment by one jump.  The jump polynomials z^(s B) depend on B alone: the
+// host makes them once per process, as consecutive multiples
+// (jump_polys_multiples), and the device table is kept per distance list.
 constexpr uint64_t kMtSplitMin = uint64_t(1) << 21;
 constexpr uint32_t kMtMaxSegs = 256;
 
@@ -80,7 +83,8 @@ pgh_status launch_mt_words(uint32_t seed, uint64_t n, uint32_t *out, hipStream_t
     while (B * kMtMaxSegs < n) B <<= 1;
     const uint32_t segs = static_cast<uint32_t>((n + B - 1u) / B);
     std::vector<uint64_t> dists;
-    for (uint32_t b = 0; (segs - 1u) >> b; ++b) dists.push_back(B << b);
+    for (uint32_t k = 1; k < segs; ++k) dists.push_back(static_cast<uint64_t>(k) * B);
+    PGH_TRY(jump_polys_multiples(B, segs - 1u));
     const void *table = nullptr;
     PGH_TRY(jump_table_device(dists, &table));
     k_mt19937_seg<<<dim3(segs), dim3(kMtJumpBlock), 0, s>>>(seed, n, B, static_cast<uint32_t>(dists.size()), table,
