@@ -1,24 +1,25 @@
 // pso.hip -- pso_gen (src/algorithms/pso_gen.cpp:120-530) for an archipelago
 // of independent islands of uneven size, on the device.
 //
-// Per island the device holds, contiguously: X, V, lbX (pop x nx), fit, lbfit
-// (pop), the island's std::mt19937 state and a word buffer for a batch of
-// kGenBatch generations (pop x nx x 4 words each: r1 and r2 of
-// pso_gen.cpp:300-301 in draw order).  A generation is up to three launches:
-//   k_rng     (every kGenBatch generations) the island's generator split in
-//             segments, one workgroup each, each handing out whole
-//             generations of the batch and reaching its own by jump-ahead
-//             (mt_dev.hpp, mt_jump): exactly the draws of the reference's
-//             sequential loop
-//   k_update  one lane per (island, particle, dimension): best neighbour
-//             (pso_gen.cpp:593-622), velocity (variant 5, :302-305), clamp and
-//             position (:336-361) -- arithmetic in the reference's order
-//   k_eval    one workgroup per tile of individuals: fitness with the same
-//             device routine as hip_bfe, then the memory update
-//             (fit <= lbfit, pso_gen.cpp:443-449) and the optional trace
-// Batched mode launches each phase once for all islands and captures the
-// whole generation loop in a hipGraph; per-island mode launches the same
-// kernels island by island.  Both run identical arithmetic per island.
+// Per island the device holds, contiguously: X, V, lbX (pop x nx, the memory
+// double-buffered with a per-row selector), fit, lbfit (pop), the island's
+// std::mt19937 state and a word buffer for a batch of kGenBatch generations
+// (in draw order: r1 and r2 of pso_gen.cpp:300-301 for variants 1 and 5, the
+// other variants' words as they draw them).  A generation is:
+//   k_step      one persistent launch (or one per problem class): workgroups
+//               take (island, rows) items from a queue and, per item, find
+//               each row's best neighbour (pso_gen.cpp:593-622), move the
+//               rows (:302-361), evaluate them with the batch evaluator's
+//               device routines and update the memory (:443-449)
+//   k_rng,      on a second stream, during the current batch: the next
+//   k_rng_gen   batch's words, the island's generator split in segments that
+//               reach their own positions by jump-ahead (mt_dev.hpp,
+//               mt_jump): exactly the draws of the reference's sequential loop
+// plus k_gbest_update for islands of the gbest topology.  Batched mode
+// captures a whole evolve call, every island at once, in a hipGraph;
+// per-island mode launches the same kernels island by island.  Both run
+// identical arithmetic per island.
+
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
