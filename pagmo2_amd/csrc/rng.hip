@@ -67,7 +67,9 @@ __global__ __launch_bounds__(256) void k_uniform_inplace(double *__restrict__ io
 // kMtSplitMin words, else up to kMtMaxSegs workgroups, each starting its
 // segment by one jump.  The jump polynomials z^(s B) depend on B alone: the
 // host makes them once per process, as consecutive multiples
-// (jump_polys_multiples), and the device table is kept per distance list.
+// (jump_polys_multiples), and one device table per B holds all kMtMaxSegs - 1
+// of them (kept for the process: at most one per power of two B, 12 MB each),
+// whatever the number of segments of a call.
 constexpr uint64_t kMtSplitMin = uint64_t(1) << 21;
 constexpr uint32_t kMtMaxSegs = 256;
 
@@ -83,8 +85,8 @@ pgh_status launch_mt_words(uint32_t seed, uint64_t n, uint32_t *out, hipStream_t
     while (B * kMtMaxSegs < n) B <<= 1;
     const uint32_t segs = static_cast<uint32_t>((n + B - 1u) / B);
     std::vector<uint64_t> dists;
-    for (uint32_t k = 1; k < segs; ++k) dists.push_back(static_cast<uint64_t>(k) * B);
-    PGH_TRY(jump_polys_multiples(B, segs - 1u));
+    for (uint32_t k = 1; k < kMtMaxSegs; ++k) dists.push_back(static_cast<uint64_t>(k) * B);
+    PGH_TRY(jump_polys_multiples(B, kMtMaxSegs - 1u));
     const void *table = nullptr;
     PGH_TRY(jump_table_device(dists, &table));
     k_mt19937_seg<<<dim3(segs), dim3(kMtJumpBlock), 0, s>>>(seed, n, B, static_cast<uint32_t>(dists.size()), table,
