@@ -1108,17 +1108,15 @@ __device__ __forceinline__ void lj_recip_slots(const double (&d)[NS], double (&i
 // (lj_recip_slots), s = r^-6.
 template <int L, uint32_t NS>
 struct lj_visit {
-    double dx[NS], dy[NS], dz[NS], inv[NS], s[NS];
+    double s[NS];
     __device__ __forceinline__ lj_visit(const lj_slots<L, NS> &sl, uint32_t k)
     {
-        double r2[NS];
+        double r2[NS], inv[NS];
 #pragma unroll
         for (uint32_t q = 0; q < NS; ++q) {
             const double *pb = sl.p[q] + 3u * k;
-            dx[q] = sl.x[q] - pb[0];
-            dy[q] = sl.y[q] - pb[1];
-            dz[q] = sl.z[q] - pb[2];
-            r2[q] = fma(dz[q], dz[q], fma(dy[q], dy[q], dx[q] * dx[q]));
+            const double dx = sl.x[q] - pb[0], dy = sl.y[q] - pb[1], dz = sl.z[q] - pb[2];
+            r2[q] = fma(dz, dz, fma(dy, dy, dx * dx));
         }
         lj_recip_slots<NS>(r2, inv);
 #pragma unroll
