@@ -167,17 +167,18 @@ __global__ __launch_bounds__(kBlock) void k_rosen_wave(const double *__restrict_
     if (lane < rows && fvs) fvs[row0 + lane] = acc;
 }
 
-// ---- rosenbrock over even rows of 16-byte aligned matrices: 4 rows per ------
+// ---- rosenbrock over even rows of 16-byte aligned matrices: 8 rows per ------
 // wavefront, column chunks of 128 K.  Lane l reads elements c0 + 128 k + 2l
 // and c0 + 128 k + 2l + 1 (k < K) of each row with 16-byte loads (contiguous
 // KiBs per row and instruction, which stream from HBM better than
 // k_rosen_wave's 512-byte rows); the neighbours x[i - 1] and x[i + 1] come
 // from the adjacent lanes, across pieces k and k + 1 from lanes 63 and 0, the
 // chunk's last one from one more load and the previous chunk's from a carried
-// register.  Terms, their in-order sum by lane r < 4 and the gradient are
+// register.  Terms, their in-order sum by lane r < 8 and the gradient are
 // k_rosen_wave's, bit for bit.
-// (K = 2 measured no faster than K = 1 at 65536 x 1000, with 104 VGPRs against 80.)
-constexpr uint32_t kRPRows = 4, kRPK = 1, kRPCols = 128u * kRPK, kRPStride = kRPCols + 1u;
+// (At 65536 x 1000 with the gradient: 8 rows 0.217 ms, 4 rows 0.235, 16 rows
+// 0.222, 2 rows 0.289; K = 2 no faster than K = 1, with 104 VGPRs against 80.)
+constexpr uint32_t kRPRows = 8, kRPK = 1, kRPCols = 128u * kRPK, kRPStride = kRPCols + 1u;
 
 __device__ __forceinline__ double shfl_dbl(double v, int src)
 {
