@@ -310,6 +310,35 @@ def test_lennard_jones_gradient_special_distances_blocks(atoms):
         assert abs(fb[i] - rf[i]) <= 1e-12 * lj_abs_scale(x[i]), (i, fb[i], rf[i])
 
 
+def test_lennard_jones_every_size_to_140():
+    """Every cluster size from 3 to 140 atoms goes through the lane and block
+    rules that pick its kernels (cyclic schedule, pair-once blocks of 16 or 32
+    lanes, with and without padded slots): energy and gradient of two random
+    rows against the oracle, and a third row with two coincident atoms that
+    takes the careful paths."""
+    for atoms in range(3, 141):
+        x = inputs(P.LENNARD_JONES, atoms, 3, 1000 + atoms, lo=-3.0, hi=3.0)
+        if atoms >= 5:
+            x[2, 3:6] = x[2, 6:9]  # atoms 3 and 4 coincide
+        fb = gpu_bfe(P.LENNARD_JONES, atoms, x)[:, 0]
+        g = torch.empty((3, x.shape[1]), dtype=torch.float64, device=DEV)
+        f = torch.empty((3, 1), dtype=torch.float64, device=DEV)
+        P.batch_gradient(P.LENNARD_JONES, atoms, dev(x), g, f)
+        torch.cuda.synchronize()
+        g, f = g.cpu().numpy(), f.cpu().numpy()[:, 0]
+        rf = O.bfe(P.LENNARD_JONES, atoms, x)[:, 0]
+        rg = O.gradient(P.LENNARD_JONES, atoms, x)
+        for i in range(3):
+            if math.isfinite(rf[i]):
+                sc = lj_abs_scale(x[i])
+                assert abs(fb[i] - rf[i]) <= 1e-12 * sc, (atoms, i)
+                assert abs(f[i] - rf[i]) <= 1e-12 * sc, (atoms, i)
+                assert np.linalg.norm(g[i] - rg[i]) <= 1e-12 * np.linalg.norm(rg[i]), (atoms, i)
+            else:
+                assert fb[i] == rf[i] and f[i] == rf[i], (atoms, i, fb[i], f[i], rf[i])
+                assert np.array_equal(np.isnan(g[i]), np.isnan(rg[i])), (atoms, i)
+
+
 def test_lennard_jones_gradient_finite_differences():
     # central differences, h = 1e-6, and estimate_gradient_h
     # (gradients_and_hessians.hpp:199-244) at dx = 1e-4; <= 1e-7 relative
