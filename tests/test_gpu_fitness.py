@@ -69,6 +69,26 @@ def test_separable_rel_1e12(kind, dim, n):
     np.testing.assert_allclose(g, r, rtol=1e-12, atol=0)
 
 
+def test_every_dimension_to_300():
+    """Every dimension from 1 (2 for rosenbrock) to 300 and a few beyond, for
+    the five problems of the row kernels (lanes per individual, staged and
+    streaming forms, 16-byte pairs for even rosenbrock rows), batches of 3 and
+    67 rows: rosenbrock bit for bit, the others within 1e-12 relative."""
+    dims = list(range(1, 301)) + [511, 512, 513, 1023, 1024, 1025]
+    for kind in (P.ROSENBROCK, P.RASTRIGIN, P.ACKLEY, P.SCHWEFEL, P.GRIEWANK):
+        for dim in dims:
+            if kind == P.ROSENBROCK and dim < 2:
+                continue
+            for n in (3, 67):
+                x = inputs(kind, dim, n, 31 * dim + n + kind)
+                g = gpu_bfe(kind, dim, x)
+                r = O.bfe(kind, dim, x, threads=8)
+                if kind == P.ROSENBROCK:
+                    assert np.array_equal(g, r), (dim, n)
+                else:
+                    np.testing.assert_allclose(g, r, rtol=1e-12, atol=0, err_msg=f"{kind} {dim} {n}")
+
+
 @pytest.mark.parametrize("kind", [P.RASTRIGIN, P.ACKLEY])
 def test_separable_out_of_range_arguments(kind):
     """Arguments beyond the streaming kernel's cosine range (|x| >= 2^10), inf
