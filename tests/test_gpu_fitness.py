@@ -221,6 +221,28 @@ def test_rosenbrock_gradient_bit_exact(dim, n):
     assert np.array_equal(f.cpu().numpy(), O.bfe(P.ROSENBROCK, dim, x))
 
 
+def test_gradients_every_dimension_to_300():
+    """Fitness + gradient of rosenbrock (bit for bit) and rastrigin (within
+    1e-13 (|2x| + 20 pi) per component, the fitness equal to a bfe call's) at
+    every dimension from 2 to 300 and around 512 and 1024, batches of 3 and
+    67 rows: the fused, paired, staged and element-wise kernels' thresholds."""
+    for dim in list(range(2, 301)) + [511, 512, 513, 1023, 1024, 1025]:
+        for n in (3, 67):
+            for kind in (P.ROSENBROCK, P.RASTRIGIN):
+                x = inputs(kind, dim, n, 17 * dim + n + kind)
+                g = torch.empty((n, dim), dtype=torch.float64, device=DEV)
+                f = torch.empty((n, 1), dtype=torch.float64, device=DEV)
+                P.batch_gradient(kind, dim, dev(x), g, f)
+                torch.cuda.synchronize()
+                got, fit, ref = g.cpu().numpy(), f.cpu().numpy(), O.gradient(kind, dim, x)
+                if kind == P.ROSENBROCK:
+                    assert np.array_equal(got, ref), (dim, n)
+                    assert np.array_equal(fit, O.bfe(kind, dim, x)), (dim, n)
+                else:
+                    assert np.all(np.abs(got - ref) <= 1e-13 * (np.abs(2 * x) + 20 * np.pi)), (dim, n)
+                    assert np.array_equal(fit, gpu_bfe(kind, dim, x)), (dim, n)
+
+
 @pytest.mark.parametrize("dim,n", [(300, 40), (1000, 257), (301, 33), (12, 1000), (2, 5)])
 def test_rastrigin_gradient(dim, n):
     # even dimensions take the fused fitness + gradient pass, odd ones the
