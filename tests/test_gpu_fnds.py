@@ -60,6 +60,19 @@ def test_random_with_ties(m, n):
     assert_same(objs)
 
 
+def test_every_size_to_300():
+    """Every population size from 2 to 300 and a few around the peel's
+    thresholds (the bit matrix's 32-bit words, the column partition), with
+    ties (integer objectives) and without: all six outputs bit for bit."""
+    sizes = list(range(2, 301)) + [511, 512, 513, 1023, 1024, 1025, 4095, 4096, 4097]
+    for n in sizes:
+        m = 2 + n % 3
+        rng = np.random.default_rng(n)
+        assert_same(rng.integers(0, 6, size=(n, m)).astype(np.float64))
+        if n % 7 == 0 or n > 300:
+            assert_same(rng.uniform(0, 1, (n, m)))
+
+
 @pytest.mark.parametrize("n,m", [(500, 2), (777, 3)])
 def test_continuous(n, m):
     rng = np.random.default_rng(n)
