@@ -163,7 +163,13 @@ __device__ __forceinline__ void seg_span(const island_desc &di, uint32_t s, uint
     end = n0 + static_cast<uint64_t>(di.gens) * W; // one past the island's last word of the call
     start = n0 + static_cast<uint64_t>(b) * di.bwords + static_cast<uint64_t>(s) * di.swords;
     const uint64_t share = s + 1u < di.segs ? di.swords : di.bwords - static_cast<uint64_t>(s) * di.swords;
-    count = start < end ? min(share, end - start) : 0u;
+    // the words left from start on, 0 past the end: end - min(start, end)
+    // cannot wrap.  (Written as start < end ? min(share, end - start) : 0,
+    // the compiled code lost the comparison and took a full share past the
+    // end, overwriting the live batch of an island whose last batch was
+    // already in hand.)
+    const uint64_t left = end - min(start, end);
+    count = min(share, left);
 }
 
 __device__ __forceinline__ uint64_t velocity_words(const island_desc &di, uint32_t v_valid)

@@ -161,6 +161,38 @@ def test_batched_equals_per_island():
     b.close()
 
 
+def test_random_archipelagos_batched_equal_per_island():
+    """Twelve random archipelagos: 1 to 7 islands of the six problems,
+    populations of 5 to 500, 2 to 80 dimensions (3 to 30 atoms), 1 to 40
+    generations (across the 32-generation word batches), random PSO
+    parameters of the default variant; two evolve calls, batched and
+    per-island, bit for bit."""
+    kinds = [P.ROSENBROCK, P.RASTRIGIN, P.ACKLEY, P.SCHWEFEL, P.GRIEWANK, P.LENNARD_JONES]
+    for t in range(12):
+        rng = np.random.default_rng(500 + t)
+        isl = []
+        for k in range(int(rng.integers(1, 8))):
+            kind = kinds[int(rng.integers(0, len(kinds)))]
+            pop = int(round(np.exp(rng.uniform(np.log(5), np.log(500)))))
+            dim = int(rng.integers(3, 31)) if kind == P.LENNARD_JONES else int(rng.integers(2, 81))
+            X0, F0 = O.population_init(kind, dim, pop, 7 * t + k)
+            algo = P.pso_params(gen=int(rng.integers(1, 41)), seed=100 * t + k, omega=float(rng.uniform(0.3, 0.9)),
+                                eta1=float(rng.uniform(1.0, 3.0)), eta2=float(rng.uniform(1.0, 3.0)),
+                                max_vel=float(rng.uniform(0.1, 1.0)))
+            isl.append(dict(kind=kind, dim=dim, X0=X0, F0=F0, algo=algo))
+        a = P.Archipelago(isl)
+        b = P.Archipelago(isl)
+        for _ in range(2):
+            a.evolve(P.ARCHI_BATCHED)
+            b.evolve(P.ARCHI_PER_ISLAND)
+        for k in range(len(isl)):
+            xa, fa = a.get(k)
+            xb, fb = b.get(k)
+            assert np.array_equal(xa, xb) and np.array_equal(fa, fb), (t, k)
+        a.close()
+        b.close()
+
+
 @pytest.mark.parametrize("form,lj_lds,bpc", [
     (P.ARCHI_STEP_SPLIT, 0, 0),
     (P.ARCHI_STEP_ONE, 0, 0),
