@@ -650,18 +650,21 @@ __device__ __forceinline__ void dtlz_eval(uint32_t id, const double *__restrict_
         f[M - 1u] = 0.5 * (1. - x[0]) * (1. + g);
         return;
     }
-    // DTLZ2-6 (dtlz.cpp:285-383): products of cos(v_j pi/2) and one sin
+    // DTLZ2-6 (dtlz.cpp:285-383): f[M-1-k] = (1+g) c_0 ... c_{k-1} s_k and
+    // f[0] = (1+g) c_0 ... c_{M-2}, with c_j, s_j = cos, sin(v_j pi/2).  The
+    // reference multiplies each objective's product from (1+g) left to right;
+    // the running product P_k = (1+g) c_0 ... c_{k-1} is the same sequence of
+    // roundings, so each position variable's sine and cosine are taken once
+    // (M-1 sincos instead of M(M-1)/2 + M-1 separate calls).
     const double a = static_cast<double>(alpha);
-    double f0 = (1. + g);
-    for (uint32_t i = 0u; i + 1u < M; ++i) f0 *= cos(dtlz_pos(id, x, i, g, a) * kPiHalf);
-    f[0] = f0;
-    for (uint32_t i = 1u; i + 1u < M; ++i) {
-        double fi = (1. + g);
-        for (uint32_t j = 0u; j < M - (i + 1u); ++j) fi *= cos(dtlz_pos(id, x, j, g, a) * kPiHalf);
-        fi *= sin(dtlz_pos(id, x, M - (i + 1u), g, a) * kPiHalf);
-        f[i] = fi;
+    double P = 1. + g;
+    for (uint32_t k = 0u; k + 1u < M; ++k) {
+        double s, c;
+        sincos(dtlz_pos(id, x, k, g, a) * kPiHalf, &s, &c);
+        f[M - 1u - k] = P * s;
+        P *= c;
     }
-    f[M - 1u] = (1. + g) * sin(dtlz_pos(id, x, 0u, g, a) * kPiHalf);
+    f[0] = P;
 }
 
 // ---------------------------------------------------------------------------
